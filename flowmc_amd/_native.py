@@ -1,0 +1,181 @@
+"""ctypes binding of ``libflowmc_hip.so`` (the C ABI in ``include/flowmc.h``).
+
+The library is the only compute path: there is no Python or CPU fallback.
+If the shared library is missing, or a tensor argument is not a contiguous
+tensor of the right dtype on a HIP device, the call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+from typing import Optional
+
+import torch
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libflowmc_hip.so"
+
+ABI_VERSION = 1
+
+
+class FlowMCNativeError(RuntimeError):
+    """A C-ABI call returned a non-zero fmc_status."""
+
+    def __init__(self, fn: str, status: int, message: str):
+        super().__init__(f"{fn} failed with status {status}: {message}")
+        self.fn = fn
+        self.status = status
+        self.message = message
+
+
+# --------------------------------------------------------------- ABI types --
+class fmc_target(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("n_dim", C.c_int32),
+        ("n_modes", C.c_int32),
+        ("reserved", C.c_int32),
+        ("params", C.c_void_p),
+        ("n_params", C.c_int64),
+    ]
+
+
+class fmc_launch_cfg(C.Structure):
+    _fields_ = [
+        ("lanes_per_chain", C.c_int32),
+        ("vec_per_lane", C.c_int32),
+        ("block_size", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class fmc_hmc_slot(C.Structure):
+    _fields_ = [
+        ("target", fmc_target),
+        ("x", C.c_void_p),
+        ("logp", C.c_void_p),
+        ("metric", C.c_void_p),
+        ("noise", C.c_void_p),
+        ("log_u", C.c_void_p),
+        ("out_pos", C.c_void_p),
+        ("out_logp", C.c_void_p),
+        ("out_acc", C.c_void_p),
+        ("n_chains", C.c_int64),
+        ("buf_len", C.c_int64),
+        ("cursor", C.c_int64),
+        ("key0", C.c_uint32),
+        ("key1", C.c_uint32),
+        ("lanes_per_chain", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class fmc_work_item(C.Structure):
+    _fields_ = [
+        ("slot", C.c_int32),
+        ("chain0", C.c_int32),
+        ("n_chains", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+P = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+U32 = C.c_uint32
+F32 = C.c_float
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGNATURES: dict[str, list] = {
+    "fmc_abi_version": [],
+    "fmc_last_error": [],
+    "fmc_device_query": [C.c_int, P, P, P, P, P],
+    "fmc_target_eval": [P, C.POINTER(fmc_target), P, I64, P, P, C.POINTER(fmc_launch_cfg)],
+    "fmc_mala_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, U32, U32, P, P, P, P, P, I64, I64, I32,
+                     C.POINTER(fmc_launch_cfg)],
+    "fmc_hmc_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, I32, P, U32, U32, P, P, P, P, P, I64, I64,
+                    I32, C.POINTER(fmc_launch_cfg)],
+    "fmc_hmc_run_batched": [P, P, I32, P, I32, I32, I32, I32, F32, I32, I32, I32, I32],
+    "fmc_mala_propose": [P, P, P, P, I64, I32, F32, U32, U32, I32, P],
+    "fmc_mala_accept": [P, P, P, P, P, P, P, I64, I32, F32, U32, U32, I32, P, P, P, P, I64, I64, I32],
+    "fmc_hmc_momentum": [P, P, P, I64, I32, P, U32, U32, I32, P],
+    "fmc_hmc_drift": [P, P, P, P, I64, I32, F32],
+    "fmc_hmc_kick": [P, P, P, I64, I32, F32],
+    "fmc_hmc_accept": [P, P, P, P, P, P, P, P, I64, I32, U32, U32, I32, P, P, P, P, I64, I64, I32],
+    "fmc_nf_mh": [P, P, P, P, P, P, P, I64, I32, I32, U32, U32, P, P, P, P, P, P, P, I64, I64, I32],
+    "fmc_philox_blocks": [P, U32, U32, U32, U32, U32, U32, I64, P],
+    "fmc_rng_normal": [P, U32, U32, I64, I32, I32, I32, I32, P],
+    "fmc_rng_log_uniform": [P, U32, U32, I64, I32, I32, I32, P],
+}
+_RESTYPES = {"fmc_last_error": C.c_char_p}
+
+_lib: Optional[C.CDLL] = None
+
+
+def exported_names() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def load(path: Optional[os.PathLike] = None) -> C.CDLL:
+    """Load (once) and type the shared library. Raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(
+            f"flowmc_amd native library not found at {p}; build it with `python -m flowmc_amd.build`"
+        )
+    lib = C.CDLL(str(p), mode=C.RTLD_GLOBAL)
+    for name, argtypes in _SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError if an export is missing
+        fn.argtypes = argtypes
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    v = lib.fmc_abi_version()
+    if v != ABI_VERSION:
+        raise RuntimeError(f"libflowmc_hip ABI version {v} != expected {ABI_VERSION}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().fmc_last_error().decode(errors="replace")
+
+
+def call(name: str, *args) -> None:
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        raise FlowMCNativeError(name, rc, last_error())
+
+
+# ---------------------------------------------------------- tensor checks --
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("flowmc_amd kernels need a HIP GPU (torch.cuda.is_available() is False)")
+
+
+def dptr(t: Optional[torch.Tensor], dtype: torch.dtype = torch.float32, name: str = "tensor",
+         numel: Optional[int] = None) -> Optional[int]:
+    """Device pointer of a contiguous HIP tensor (None passes through as NULL)."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live on the HIP device, got {t.device}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} must have {numel} elements, got {t.numel()}")
+    return t.data_ptr()
+
+
+def stream_ptr(stream: Optional[torch.cuda.Stream] = None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream

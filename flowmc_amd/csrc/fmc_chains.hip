@@ -1,0 +1,878 @@
+// Chain kernels for gfx950: fused K-step MALA / HMC over built-in targets,
+// per-step MALA/HMC pieces for externally differentiated targets, the
+// normalizing-flow independence MH scan, and the RNG streams they use.
+//
+// Work mapping: a chain is owned by `lpc` lanes of one 64-wide wavefront
+// (lpc a power of two <= 64, groups aligned inside the wave); each lane keeps
+// NV float4 chunks of position / gradient in registers for the whole launch,
+// so the K fused steps touch HBM only for the thinned buffer rows, the noise
+// (injected-noise mode) and the state at entry/exit.
+#include "fmc_common.h"
+#include "fmc_targets.h"
+
+namespace fmc {
+
+struct ChainIO {
+  float* x;
+  float* logp;
+  const float* metric;
+  const float* noise;
+  const float* log_u;
+  float* out_pos;
+  float* out_logp;
+  float* out_acc;
+  int64_t C;
+  int64_t buf_len;
+  int64_t cursor;
+  uint32_t key0, key1;
+};
+
+struct StepCfg {
+  int n_steps;
+  int step0;
+  float dt;
+  int n_leapfrog;
+  int thinning;
+};
+
+template <int NV>
+__device__ __forceinline__ void load_row(const float* __restrict__ src, int d, int r, int lpc,
+                                         float4 (&x)[NV]) {
+  const bool vec = (d & 3) == 0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    int j0 = (v * lpc + r) * 4;
+    if (vec) {
+      x[v] = (j0 < d) ? *reinterpret_cast<const float4*>(src + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) f4(x[v], e) = (j0 + e < d) ? src[j0 + e] : 0.f;
+    }
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ void store_row(float* __restrict__ dst, int d, int r, int lpc,
+                                          const float4 (&x)[NV]) {
+  const bool vec = (d & 3) == 0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    int j0 = (v * lpc + r) * 4;
+    if (vec) {
+      if (j0 < d) *reinterpret_cast<float4*>(dst + j0) = x[v];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (j0 + e < d) dst[j0 + e] = f4c(x[v], e);
+    }
+  }
+}
+
+// Standard normals of (chain, step) for this lane's chunks; padding zeroed.
+template <int NV>
+__device__ __forceinline__ void draw_normals(const ChainIO& io, int d, int r, int lpc, int64_t c,
+                                             int s, int t_glob, uint32_t sid, float4 (&xi)[NV]) {
+  if (io.noise) {
+    load_row<NV>(io.noise + ((int64_t)s * io.C + c) * d, d, r, lpc, xi);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      int chunk = v * lpc + r;
+      xi[v] = philox_normal4(io.key0, io.key1, (uint32_t)chunk, (uint32_t)c, (uint32_t)t_glob, sid);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (chunk * 4 + e >= d) f4(xi[v], e) = 0.f;
+    }
+  }
+}
+
+__device__ __forceinline__ float draw_log_u(const ChainIO& io, int64_t c, int s, int t_glob,
+                                            uint32_t sid) {
+  if (io.log_u) return io.log_u[(int64_t)s * io.C + c];
+  return philox_log_uniform(io.key0, io.key1, (uint32_t)c, (uint32_t)t_glob, sid);
+}
+
+template <int NV>
+__device__ __forceinline__ void write_rows(const ChainIO& io, int d, int r, int lpc, int64_t c,
+                                           int t_glob, int thinning, const float4 (&x)[NV], float lp,
+                                           bool acc) {
+  if (t_glob % thinning != 0) return;
+  int64_t row = io.cursor + t_glob / thinning;
+  if (io.out_pos) store_row<NV>(io.out_pos + (c * io.buf_len + row) * d, d, r, lpc, x);
+  if (r == 0) {
+    if (io.out_logp) io.out_logp[c * io.buf_len + row] = lp;
+    if (io.out_acc) io.out_acc[c * io.buf_len + row] = acc ? 1.f : 0.f;
+  }
+}
+
+// ------------------------------------------------------------------ MALA ---
+// One chain group runs cfg.n_steps MALA steps (src/flowMC/resource/kernel/
+// MALA.py:26-89). The gradient at the current point is carried from the
+// previous step instead of being re-evaluated: MALA.py:59 recomputes it at the
+// same point, which for a deterministic target is the same value.
+template <int NV>
+__device__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
+                           int r, int lpc, int gbase) {
+  const int d = t.d;
+  float4 x[NV], g[NV];
+  load_row<NV>(io.x + c * d, d, r, lpc, x);
+  float lp = target_vg<NV>(t, x, g, r, lpc, gbase);
+  const float dt = cfg.dt;
+  const float h = 0.5f * dt * dt;
+  const float inv2dt2 = 1.f / (2.f * dt * dt);
+  for (int s = 0; s < cfg.n_steps; ++s) {
+    const int tg = cfg.step0 + s;
+    float4 xi[NV], prop[NV], g1[NV];
+    draw_normals<NV>(io, d, r, lpc, c, s, tg, RNG_MALA_NOISE, xi);
+    float s1 = 0.f;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float n = dt * f4c(xi[v], e);
+        f4(prop[v], e) = f4c(x[v], e) + h * f4c(g[v], e) + n;
+        s1 += n * n;
+      }
+    float lp1 = target_vg<NV>(t, prop, g1, r, lpc, gbase);
+    float s2 = 0.f;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float q = f4c(x[v], e) - f4c(prop[v], e) - h * f4c(g1[v], e);
+        s2 += q * q;
+      }
+    // padding lanes contribute exactly 0 to s1 and s2
+    float ds = group_sum(s1 - s2, lpc);
+    float ratio = (lp1 - lp) + ds * inv2dt2;
+    float lu = draw_log_u(io, c, s, tg, RNG_MALA_UNIFORM);
+    bool acc = lu < ratio;
+    if (acc) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        x[v] = prop[v];
+        g[v] = g1[v];
+      }
+      lp = lp1;
+    }
+    write_rows<NV>(io, d, r, lpc, c, tg, cfg.thinning, x, lp, acc);
+  }
+  store_row<NV>(io.x + c * d, d, r, lpc, x);
+  if (r == 0 && io.logp) io.logp[c] = lp;
+}
+
+// ------------------------------------------------------------------- HMC ---
+// HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) with the leapfrog
+// coefficient table of HMC.py:47-50: row 0 = (0, 1/2), rows 1..L = (1, 1),
+// row L+1 = (1, 1/2): L+1 drifts x += eps*c0*metric*p and L+2 kicks
+// p += eps*c1*grad logp(x). H0 - H1 is accumulated in FP64.
+template <int NV>
+__device__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
+                          int r, int lpc, int gbase) {
+  const int d = t.d;
+  float4 x[NV], g[NV], m[NV];
+  load_row<NV>(io.x + c * d, d, r, lpc, x);
+  if (io.metric) {
+    load_row<NV>(io.metric, d, r, lpc, m);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) m[v] = make_float4(1.f, 1.f, 1.f, 1.f);
+  }
+  float lp = target_vg<NV>(t, x, g, r, lpc, gbase);
+  const float eps = cfg.dt;
+  const int L = cfg.n_leapfrog;
+  for (int s = 0; s < cfg.n_steps; ++s) {
+    const int tg = cfg.step0 + s;
+    float4 p[NV], xl[NV], gl[NV];
+    draw_normals<NV>(io, d, r, lpc, c, s, tg, RNG_HMC_MOMENTUM, p);
+    double k0 = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int j = dim_of(v, e, r, lpc);
+        float mj = f4c(m[v], e);
+        float pj = (j < d) ? f4c(p[v], e) / sqrtf(mj) : 0.f;
+        f4(p[v], e) = pj;
+        k0 += 0.5 * (double)mj * (double)pj * (double)pj;
+      }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      xl[v] = x[v];
+      gl[v] = g[v];
+    }
+    k0 = group_sum(k0, lpc);
+    // row 0: drift coefficient 0, kick 1/2
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) f4(p[v], e) += eps * 0.5f * f4c(gl[v], e);
+    float lpl = lp;
+    for (int i = 1; i <= L + 1; ++i) {
+      const float kick = (i <= L) ? 1.f : 0.5f;
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) f4(xl[v], e) += eps * (f4c(m[v], e) * f4c(p[v], e));
+      lpl = target_vg<NV>(t, xl, gl, r, lpc, gbase);
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) f4(p[v], e) += eps * kick * f4c(gl[v], e);
+    }
+    double k1 = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int j = dim_of(v, e, r, lpc);
+        if (j < d) k1 += 0.5 * (double)f4c(m[v], e) * (double)f4c(p[v], e) * (double)f4c(p[v], e);
+      }
+    k1 = group_sum(k1, lpc);
+    double h0 = -(double)lp + k0;
+    double h1 = -(double)lpl + k1;
+    double log_acc = h0 - h1;
+    float lu = draw_log_u(io, c, s, tg, RNG_HMC_UNIFORM);
+    bool acc = (double)lu < log_acc;
+    if (acc) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        x[v] = xl[v];
+        g[v] = gl[v];
+      }
+      lp = lpl;
+    }
+    write_rows<NV>(io, d, r, lpc, c, tg, cfg.thinning, x, lp, acc);
+  }
+  store_row<NV>(io.x + c * d, d, r, lpc, x);
+  if (r == 0 && io.logp) io.logp[c] = lp;
+}
+
+// Copies the target parameter block to LDS when it fits (n_lds > 0).
+__device__ __forceinline__ TargetDev stage_target(const fmc_target& tg, float* smem, int n_lds) {
+  TargetDev t{tg.kind, tg.n_dim, tg.n_modes, tg.params};
+  if (n_lds > 0) {
+    for (int i = threadIdx.x; i < n_lds; i += blockDim.x) smem[i] = tg.params[i];
+    __syncthreads();
+    t.p = smem;
+  }
+  return t;
+}
+
+enum KernelKind { KK_MALA = 0, KK_HMC = 1 };
+
+template <int NV, int KIND>
+__global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, StepCfg cfg,
+                                                     int lpc, int n_lds) {
+  extern __shared__ float smem[];
+  TargetDev t = stage_target(tg, smem, n_lds);
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (lpc - 1);
+  const int gbase = lane - r;
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x / lpc) + threadIdx.x / lpc;
+  if (c >= io.C) return;  // whole group leaves together
+  if (KIND == KK_MALA)
+    mala_group<NV>(t, io, cfg, c, r, lpc, gbase);
+  else
+    hmc_group<NV>(t, io, cfg, c, r, lpc, gbase);
+}
+
+// Many HMC problems of uneven dimension in one launch: one work item per block.
+template <int NV>
+__global__ void __launch_bounds__(1024) hmc_batched_kernel(const fmc_hmc_slot* __restrict__ slots,
+                                                           const fmc_work_item* __restrict__ items,
+                                                           StepCfg cfg) {
+  const fmc_work_item it = items[blockIdx.x];
+  const fmc_hmc_slot sl = slots[it.slot];
+  const int lpc = sl.lanes_per_chain;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (lpc - 1);
+  const int gbase = lane - r;
+  const int local = threadIdx.x / lpc;
+  if (local >= it.n_chains) return;
+  const int64_t c = (int64_t)it.chain0 + local;
+  TargetDev t{sl.target.kind, sl.target.n_dim, sl.target.n_modes, sl.target.params};
+  ChainIO io{sl.x,        sl.logp,     sl.metric,  sl.noise,  sl.log_u,  sl.out_pos, sl.out_logp,
+             sl.out_acc,  sl.n_chains, sl.buf_len, sl.cursor, sl.key0,   sl.key1};
+  hmc_group<NV>(t, io, cfg, c, r, lpc, gbase);
+}
+
+// ------------------------------------------------------- target eval ------
+template <int NV>
+__global__ void __launch_bounds__(1024) target_eval_kernel(fmc_target tg, const float* __restrict__ x,
+                                                           int64_t n, float* logp, float* grad,
+                                                           int lpc, int n_lds) {
+  extern __shared__ float smem[];
+  TargetDev t = stage_target(tg, smem, n_lds);
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (lpc - 1);
+  const int gbase = lane - r;
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x / lpc) + threadIdx.x / lpc;
+  if (c >= n) return;
+  const int d = t.d;
+  float4 xv[NV], g[NV];
+  load_row<NV>(x + c * d, d, r, lpc, xv);
+  float lp = target_vg<NV>(t, xv, g, r, lpc, gbase);
+  if (grad) store_row<NV>(grad + c * d, d, r, lpc, g);
+  if (r == 0 && logp) logp[c] = lp;
+}
+
+// ------------------------------------------- per-step pieces (any target) --
+// One 64-lane wave per chain; lanes stride over dimensions.
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ float noise_at(const float* noise, uint32_t k0, uint32_t k1, int64_t c,
+                                          int64_t C, int d, int j, int step, uint32_t sid) {
+  if (noise) return noise[c * d + j];
+  // same (chunk, chain, step, stream) counter as the fused kernels
+  float4 z = philox_normal4(k0, k1, (uint32_t)(j >> 2), (uint32_t)c, (uint32_t)step, sid);
+  return f4c(z, j & 3);
+}
+
+__global__ void mala_propose_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                                    float* __restrict__ prop, int64_t C, int d, float dt,
+                                    uint32_t k0, uint32_t k1, int step, const float* noise) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C * d) return;
+  const int64_t c = i / d;
+  const int j = (int)(i - c * d);
+  float xi = noise_at(noise, k0, k1, c, C, d, j, step, RNG_MALA_NOISE);
+  prop[i] = x[i] + 0.5f * dt * dt * g[i] + dt * xi;
+}
+
+__global__ void mala_accept_kernel(float* x, float* logp, float* g, const float* __restrict__ prop,
+                                   const float* __restrict__ lp1, const float* __restrict__ g1,
+                                   int64_t C, int d, float dt, uint32_t k0, uint32_t k1, int step,
+                                   const float* log_u, float* out_pos, float* out_logp,
+                                   float* out_acc, int64_t buf_len, int64_t cursor, int thinning) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= C) return;
+  const float h = 0.5f * dt * dt;
+  float s1 = 0.f, s2 = 0.f;
+  for (int j = lane; j < d; j += 64) {
+    int64_t k = c * d + j;
+    float a = prop[k] - x[k] - h * g[k];
+    float b = x[k] - prop[k] - h * g1[k];
+    s1 += a * a;
+    s2 += b * b;
+  }
+  float ds = wave_sum(s1 - s2);
+  float ratio = (lp1[c] - logp[c]) + ds / (2.f * dt * dt);
+  float lu = log_u ? log_u[c] : philox_log_uniform(k0, k1, (uint32_t)c, (uint32_t)step, RNG_MALA_UNIFORM);
+  bool acc = lu < ratio;
+  const bool wr = (step % thinning) == 0;
+  const int64_t row = cursor + step / thinning;
+  for (int j = lane; j < d; j += 64) {
+    int64_t k = c * d + j;
+    float xv = acc ? prop[k] : x[k];
+    if (acc) {
+      x[k] = xv;
+      g[k] = g1[k];
+    }
+    if (wr && out_pos) out_pos[(c * buf_len + row) * d + j] = xv;
+  }
+  if (lane == 0) {
+    float lpv = acc ? lp1[c] : logp[c];
+    logp[c] = lpv;
+    if (wr && out_logp) out_logp[c * buf_len + row] = lpv;
+    if (wr && out_acc) out_acc[c * buf_len + row] = acc ? 1.f : 0.f;
+  }
+}
+
+__global__ void hmc_momentum_kernel(float* p, double* kinetic, int64_t C, int d, const float* metric,
+                                    uint32_t k0, uint32_t k1, int step, const float* noise) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= C) return;
+  double k = 0.0;
+  for (int j = lane; j < d; j += 64) {
+    float mj = metric ? metric[j] : 1.f;
+    float pj = noise_at(noise, k0, k1, c, C, d, j, step, RNG_HMC_MOMENTUM) / sqrtf(mj);
+    p[c * d + j] = pj;
+    k += 0.5 * (double)mj * (double)pj * (double)pj;
+  }
+  k = wave_sum(k);
+  if (lane == 0) kinetic[c] = k;
+}
+
+__global__ void hmc_drift_kernel(float* x, const float* __restrict__ p, const float* metric,
+                                 int64_t n, int d, float coef) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float mj = metric ? metric[i % d] : 1.f;
+  x[i] += coef * (mj * p[i]);
+}
+
+__global__ void hmc_kick_kernel(float* p, const float* __restrict__ g, int64_t n, float coef) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] += coef * g[i];
+}
+
+__global__ void hmc_accept_kernel(float* x, float* logp, const float* __restrict__ xn,
+                                  const float* __restrict__ lpn, const float* __restrict__ pn,
+                                  const double* __restrict__ k0, const float* metric, int64_t C,
+                                  int d, uint32_t key0, uint32_t key1, int step, const float* log_u,
+                                  float* out_pos, float* out_logp, float* out_acc, int64_t buf_len,
+                                  int64_t cursor, int thinning) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= C) return;
+  double k1 = 0.0;
+  for (int j = lane; j < d; j += 64) {
+    float mj = metric ? metric[j] : 1.f;
+    float pj = pn[c * d + j];
+    k1 += 0.5 * (double)mj * (double)pj * (double)pj;
+  }
+  k1 = wave_sum(k1);
+  double log_acc = (-(double)logp[c] + k0[c]) - (-(double)lpn[c] + k1);
+  float lu = log_u ? log_u[c] : philox_log_uniform(key0, key1, (uint32_t)c, (uint32_t)step, RNG_HMC_UNIFORM);
+  bool acc = (double)lu < log_acc;
+  const bool wr = (step % thinning) == 0;
+  const int64_t row = cursor + step / thinning;
+  for (int j = lane; j < d; j += 64) {
+    int64_t k = c * d + j;
+    float xv = acc ? xn[k] : x[k];
+    x[k] = xv;
+    if (wr && out_pos) out_pos[(c * buf_len + row) * d + j] = xv;
+  }
+  if (lane == 0) {
+    float lpv = acc ? lpn[c] : logp[c];
+    logp[c] = lpv;
+    if (wr && out_logp) out_logp[c * buf_len + row] = lpv;
+    if (wr && out_acc) out_acc[c * buf_len + row] = acc ? 1.f : 0.f;
+  }
+}
+
+// --------------------------------------------------------- NF MH scan -----
+// NF_proposal.py:91-126: sequential independence MH over the S proposals of
+// each chain. Phase 1 (one thread per chain) decides and records which
+// proposal the chain sits on after every step; phase 2 gathers the rows.
+__global__ void nf_mh_scan_kernel(const float* __restrict__ lp0, const float* __restrict__ q0,
+                                  const float* __restrict__ lpp, const float* __restrict__ qp,
+                                  int64_t C, int S, uint32_t k0, uint32_t k1, const float* log_u,
+                                  int* out_idx, float* lp_last, float* out_logp, float* out_acc,
+                                  int64_t buf_len, int64_t cursor, int thinning) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  float lpc = lp0[c], qc = q0[c], lp_keep = lp0[c];
+  int idx = -1;
+  const int s_last = ((S - 1) / thinning) * thinning;
+  for (int s = 0; s < S; ++s) {
+    const int64_t k = c * S + s;
+    float lpv = lpp[k], qv = qp[k];
+    float ratio = (lpv - lpc) - (qv - qc);
+    float lu = log_u ? log_u[k] : philox_log_uniform(k0, k1, (uint32_t)c, (uint32_t)s, RNG_NF_UNIFORM);
+    bool acc = lu < ratio;
+    if (acc) {
+      idx = s;
+      lpc = lpv;
+      qc = qv;
+    }
+    out_idx[k] = idx;
+    if (s == s_last) lp_keep = lpc;
+    if (s % thinning == 0) {
+      int64_t row = cursor + s / thinning;
+      if (out_logp) out_logp[c * buf_len + row] = lpc;
+      if (out_acc) out_acc[c * buf_len + row] = acc ? 1.f : 0.f;
+    }
+  }
+  if (lp_last) lp_last[c] = lp_keep;
+}
+
+// One (chain, thinned step) row per 64 lanes.
+__global__ void nf_mh_gather_kernel(const float* __restrict__ x0, const float* __restrict__ prop,
+                                    const int* __restrict__ idx, int64_t C, int S, int d,
+                                    float* out_pos, int64_t buf_len, int64_t cursor, int thinning,
+                                    int n_rows) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= C * n_rows) return;
+  const int64_t c = w / n_rows;
+  const int rr = (int)(w - c * n_rows);
+  const int s = rr * thinning;
+  const int id = idx[c * S + s];
+  const float* src = (id < 0) ? x0 + c * d : prop + (c * S + id) * d;
+  float* dst = out_pos + (c * buf_len + cursor + rr) * d;
+  for (int j = lane; j < d; j += 64) dst[j] = src[j];
+}
+
+// Final position of each chain after the scan.
+__global__ void nf_mh_last_kernel(const float* __restrict__ x0, const float* __restrict__ prop,
+                                  const int* __restrict__ idx, int64_t C, int S, int d, int s_last,
+                                  float* x_last) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C * d) return;
+  const int64_t c = i / d;
+  const int j = (int)(i - c * d);
+  const int id = idx[c * S + s_last];
+  x_last[i] = (id < 0) ? x0[i] : prop[(c * S + id) * d + j];
+}
+
+// -------------------------------------------------------------- RNG -------
+__global__ void philox_blocks_kernel(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2,
+                                     uint32_t c3, int64_t n, uint32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32x4 b = philox4x32_10(u32x4{c0 + (uint32_t)i, c1, c2, c3}, k0, k1);
+  out[4 * i + 0] = b.x;
+  out[4 * i + 1] = b.y;
+  out[4 * i + 2] = b.z;
+  out[4 * i + 3] = b.w;
+}
+
+__global__ void rng_normal_kernel(uint32_t k0, uint32_t k1, int64_t C, int d, int n_steps, int step0,
+                                  uint32_t sid, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_steps * C * d) return;
+  const int64_t row = i / d;
+  const int j = (int)(i - row * d);
+  const int s = (int)(row / C);
+  const int64_t c = row - (int64_t)s * C;
+  float4 z = philox_normal4(k0, k1, (uint32_t)(j >> 2), (uint32_t)c, (uint32_t)(step0 + s), sid);
+  out[i] = f4c(z, j & 3);
+}
+
+__global__ void rng_log_uniform_kernel(uint32_t k0, uint32_t k1, int64_t C, int n_steps, int step0,
+                                       uint32_t sid, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_steps * C) return;
+  const int s = (int)(i / C);
+  const int64_t c = i - (int64_t)s * C;
+  out[i] = philox_log_uniform(k0, k1, (uint32_t)c, (uint32_t)(step0 + s), sid);
+}
+
+// ----------------------------------------------------------- host side ----
+static int check_target(const fmc_target* t) {
+  FMC_ARG(t != nullptr, "target is NULL");
+  FMC_ARG(t->n_dim >= 1, "target n_dim must be >= 1 (got %d)", t->n_dim);
+  FMC_ARG(t->kind >= FMC_TARGET_GAUSS_DIAG && t->kind <= FMC_TARGET_DUAL_MOON, "unknown target kind %d",
+          t->kind);
+  if (t->kind == FMC_TARGET_GMM_DIAG) FMC_ARG(t->n_modes >= 1, "GMM target needs n_modes >= 1");
+  if (t->kind == FMC_TARGET_DUAL_MOON) FMC_ARG(t->n_dim >= 2, "dual-moon target needs n_dim >= 2");
+  int64_t need = target_param_count(t->kind, t->n_dim, t->n_modes);
+  FMC_ARG(t->n_params >= need, "target kind %d with d=%d needs %lld params, got %lld", t->kind, t->n_dim,
+          (long long)need, (long long)t->n_params);
+  FMC_ARG(need == 0 || t->params != nullptr, "target params pointer is NULL");
+  return FMC_OK;
+}
+
+struct Layout {
+  int lpc, nv, block;
+};
+
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Heuristic lane layout: aim for ~4 dims per lane for long vectors (one float4
+// chunk, so the fused step's register footprint stays small), but keep whole
+// chains in one lane when d is tiny.
+static Layout choose_layout(int d, int kind, const fmc_launch_cfg* cfg) {
+  Layout L{0, 0, 256};
+  if (cfg && cfg->lanes_per_chain > 0 && cfg->vec_per_lane > 0) {
+    L.lpc = cfg->lanes_per_chain;
+    L.nv = cfg->vec_per_lane;
+  } else {
+    int chunks = (d + 3) / 4;
+    if (kind == FMC_TARGET_GAUSS_DENSE) {
+      // O(d^2) gather: keep the chain in few lanes.
+      L.lpc = 1;
+      while (L.lpc * 8 < chunks) L.lpc *= 2;
+      L.nv = (chunks + L.lpc - 1) / L.lpc;
+    } else {
+      L.lpc = 1;
+      while (L.lpc < chunks && L.lpc < 64) L.lpc *= 2;
+      L.nv = (chunks + L.lpc - 1) / L.lpc;
+    }
+    if (L.nv > 2) L.nv = (L.nv <= 4) ? 4 : 8;
+  }
+  if (cfg && cfg->block_size > 0) L.block = cfg->block_size;
+  return L;
+}
+
+static int check_layout(const Layout& L, int d) {
+  FMC_ARG(is_pow2(L.lpc) && L.lpc <= 64, "lanes_per_chain must be a power of two <= 64 (got %d)", L.lpc);
+  FMC_ARG(L.nv == 1 || L.nv == 2 || L.nv == 4 || L.nv == 8, "vec_per_lane must be 1, 2, 4 or 8 (got %d)",
+          L.nv);
+  FMC_ARG(4LL * L.nv * L.lpc >= d, "layout lanes_per_chain=%d vec_per_lane=%d holds %d dims < d=%d", L.lpc,
+          L.nv, 4 * L.nv * L.lpc, d);
+  FMC_ARG(L.block >= 64 && L.block <= 1024 && L.block % 64 == 0, "block_size must be a multiple of 64 in [64,1024]");
+  return FMC_OK;
+}
+
+static int lds_floats(const fmc_target* t) {
+  int64_t n = target_param_count(t->kind, t->n_dim, t->n_modes);
+  return (n > 0 && n * 4 <= 48 * 1024) ? (int)n : 0;
+}
+
+template <int KIND>
+static int launch_chain(hipStream_t st, const fmc_target* t, const ChainIO& io, const StepCfg& cfg,
+                        const Layout& L) {
+  const int per_block = L.block / L.lpc;
+  const int64_t grid = (io.C + per_block - 1) / per_block;
+  FMC_ARG(grid <= 0x7fffffffLL, "too many chains for one launch");
+  const int n_lds = lds_floats(t);
+  const size_t shm = (size_t)n_lds * sizeof(float);
+  dim3 g((unsigned)grid), b((unsigned)L.block);
+  switch (L.nv) {
+    case 1: hipLaunchKernelGGL((chain_kernel<1, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+    case 2: hipLaunchKernelGGL((chain_kernel<2, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+    case 4: hipLaunchKernelGGL((chain_kernel<4, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+    default: hipLaunchKernelGGL((chain_kernel<8, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+  }
+  return check_launch(KIND == KK_MALA ? "mala chain kernel" : "hmc chain kernel");
+}
+
+static int check_buffers(const float* out_pos, const float* out_logp, const float* out_acc,
+                         int64_t buf_len, int64_t cursor, int thinning, int step0, int n_steps) {
+  FMC_ARG(thinning >= 1, "thinning must be >= 1");
+  FMC_ARG(step0 >= 0 && n_steps >= 0, "step0 and n_steps must be >= 0");
+  if (out_pos || out_logp || out_acc) {
+    FMC_ARG(buf_len >= 1 && cursor >= 0, "bad buffer length / cursor");
+    if (n_steps > 0) {
+      int64_t last_t = step0 + n_steps - 1;
+      int64_t last_row = cursor + last_t / thinning;
+      FMC_ARG(last_row < buf_len, "buffer row %lld out of range (buf_len %lld)", (long long)last_row,
+              (long long)buf_len);
+    }
+  }
+  return FMC_OK;
+}
+
+}  // namespace fmc
+
+using namespace fmc;
+
+extern "C" int fmc_target_eval(void* stream, const fmc_target* target, const float* x, int64_t n_points,
+                               float* logp, float* grad, const fmc_launch_cfg* cfg) {
+  int rc = check_target(target);
+  if (rc) return rc;
+  FMC_ARG(x != nullptr && n_points >= 0, "x is NULL or n_points < 0");
+  if (n_points == 0) return FMC_OK;
+  Layout L = choose_layout(target->n_dim, target->kind, cfg);
+  if ((rc = check_layout(L, target->n_dim))) return rc;
+  const int per_block = L.block / L.lpc;
+  const int64_t grid = (n_points + per_block - 1) / per_block;
+  const int n_lds = lds_floats(target);
+  const size_t shm = (size_t)n_lds * sizeof(float);
+  hipStream_t st = (hipStream_t)stream;
+  dim3 g((unsigned)grid), b((unsigned)L.block);
+  switch (L.nv) {
+    case 1: hipLaunchKernelGGL(target_eval_kernel<1>, g, b, shm, st, *target, x, n_points, logp, grad, L.lpc, n_lds); break;
+    case 2: hipLaunchKernelGGL(target_eval_kernel<2>, g, b, shm, st, *target, x, n_points, logp, grad, L.lpc, n_lds); break;
+    case 4: hipLaunchKernelGGL(target_eval_kernel<4>, g, b, shm, st, *target, x, n_points, logp, grad, L.lpc, n_lds); break;
+    default: hipLaunchKernelGGL(target_eval_kernel<8>, g, b, shm, st, *target, x, n_points, logp, grad, L.lpc, n_lds); break;
+  }
+  return check_launch("target eval kernel");
+}
+
+extern "C" int fmc_mala_run(void* stream, const fmc_target* target, float* x, float* logp, int64_t n_chains,
+                            int32_t n_steps, int32_t step0, float step_size, uint32_t key0, uint32_t key1,
+                            const float* noise, const float* log_u, float* out_pos, float* out_logp,
+                            float* out_acc, int64_t buf_len, int64_t cursor, int32_t thinning,
+                            const fmc_launch_cfg* cfg) {
+  int rc = check_target(target);
+  if (rc) return rc;
+  FMC_ARG(x != nullptr, "x is NULL");
+  FMC_ARG(n_chains >= 0, "n_chains < 0");
+  FMC_ARG(step_size > 0.f, "step_size must be > 0");
+  FMC_ARG((noise == nullptr) == (log_u == nullptr), "noise and log_u must be both given or both NULL");
+  if ((rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, step0, n_steps))) return rc;
+  if (n_chains == 0) return FMC_OK;
+  Layout L = choose_layout(target->n_dim, target->kind, cfg);
+  if ((rc = check_layout(L, target->n_dim))) return rc;
+  ChainIO io{x, logp, nullptr, noise, log_u, out_pos, out_logp, out_acc, n_chains, buf_len, cursor, key0, key1};
+  StepCfg sc{n_steps, step0, step_size, 0, thinning};
+  return launch_chain<KK_MALA>((hipStream_t)stream, target, io, sc, L);
+}
+
+extern "C" int fmc_hmc_run(void* stream, const fmc_target* target, float* x, float* logp, int64_t n_chains,
+                           int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
+                           const float* metric, uint32_t key0, uint32_t key1, const float* noise,
+                           const float* log_u, float* out_pos, float* out_logp, float* out_acc,
+                           int64_t buf_len, int64_t cursor, int32_t thinning, const fmc_launch_cfg* cfg) {
+  int rc = check_target(target);
+  if (rc) return rc;
+  FMC_ARG(x != nullptr, "x is NULL");
+  FMC_ARG(n_chains >= 0, "n_chains < 0");
+  FMC_ARG(n_leapfrog >= 0, "n_leapfrog < 0");
+  FMC_ARG((noise == nullptr) == (log_u == nullptr), "noise and log_u must be both given or both NULL");
+  if ((rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, step0, n_steps))) return rc;
+  if (n_chains == 0) return FMC_OK;
+  Layout L = choose_layout(target->n_dim, target->kind, cfg);
+  if ((rc = check_layout(L, target->n_dim))) return rc;
+  ChainIO io{x, logp, metric, noise, log_u, out_pos, out_logp, out_acc, n_chains, buf_len, cursor, key0, key1};
+  StepCfg sc{n_steps, step0, step_size, n_leapfrog, thinning};
+  return launch_chain<KK_HMC>((hipStream_t)stream, target, io, sc, L);
+}
+
+extern "C" int fmc_hmc_run_batched(void* stream, const fmc_hmc_slot* slots, int32_t n_slots,
+                                   const fmc_work_item* items, int32_t n_items, int32_t max_dim,
+                                   int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
+                                   int32_t thinning, int32_t vec_per_lane, int32_t block_size) {
+  FMC_ARG(slots != nullptr && items != nullptr, "slots/items are NULL");
+  FMC_ARG(n_slots >= 1 && n_items >= 0, "bad n_slots / n_items");
+  FMC_ARG(n_leapfrog >= 0 && n_steps >= 0 && step0 >= 0 && thinning >= 1, "bad step arguments");
+  FMC_ARG(vec_per_lane == 1 || vec_per_lane == 2 || vec_per_lane == 4 || vec_per_lane == 8,
+          "vec_per_lane must be 1, 2, 4 or 8");
+  FMC_ARG(block_size >= 64 && block_size <= 1024 && block_size % 64 == 0, "bad block_size");
+  FMC_ARG(max_dim >= 1, "max_dim must be >= 1");
+  // slot / item tables live in device memory; the host-side builder
+  // (flowmc_amd.kernels.hmc_batched) validates lanes_per_chain, dims and tiles.
+  if (n_items == 0) return FMC_OK;
+  StepCfg sc{n_steps, step0, step_size, n_leapfrog, thinning};
+  hipStream_t st = (hipStream_t)stream;
+  dim3 g((unsigned)n_items), b((unsigned)block_size);
+  switch (vec_per_lane) {
+    case 1: hipLaunchKernelGGL(hmc_batched_kernel<1>, g, b, 0, st, slots, items, sc); break;
+    case 2: hipLaunchKernelGGL(hmc_batched_kernel<2>, g, b, 0, st, slots, items, sc); break;
+    case 4: hipLaunchKernelGGL(hmc_batched_kernel<4>, g, b, 0, st, slots, items, sc); break;
+    default: hipLaunchKernelGGL(hmc_batched_kernel<8>, g, b, 0, st, slots, items, sc); break;
+  }
+  return check_launch("batched hmc kernel");
+}
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" int fmc_mala_propose(void* stream, const float* x, const float* grad, float* prop, int64_t n_chains,
+                                int32_t n_dim, float step_size, uint32_t key0, uint32_t key1, int32_t step,
+                                const float* noise) {
+  FMC_ARG(x && grad && prop, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1 && step >= 0, "bad sizes");
+  int64_t n = n_chains * n_dim;
+  if (n == 0) return FMC_OK;
+  hipLaunchKernelGGL(mala_propose_kernel, dim3(grid1(n, 256)), dim3(256), 0, (hipStream_t)stream, x, grad,
+                     prop, n_chains, n_dim, step_size, key0, key1, step, noise);
+  return check_launch("mala propose");
+}
+
+extern "C" int fmc_mala_accept(void* stream, float* x, float* logp, float* grad, const float* prop,
+                               const float* prop_logp, const float* prop_grad, int64_t n_chains, int32_t n_dim,
+                               float step_size, uint32_t key0, uint32_t key1, int32_t step, const float* log_u,
+                               float* out_pos, float* out_logp, float* out_acc, int64_t buf_len, int64_t cursor,
+                               int32_t thinning) {
+  FMC_ARG(x && logp && grad && prop && prop_logp && prop_grad, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1, "bad sizes");
+  int rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, step, 1);
+  if (rc) return rc;
+  if (n_chains == 0) return FMC_OK;
+  hipLaunchKernelGGL(mala_accept_kernel, dim3(grid1(n_chains, 4)), dim3(256), 0, (hipStream_t)stream, x, logp,
+                     grad, prop, prop_logp, prop_grad, n_chains, n_dim, step_size, key0, key1, step, log_u,
+                     out_pos, out_logp, out_acc, buf_len, cursor, thinning);
+  return check_launch("mala accept");
+}
+
+extern "C" int fmc_hmc_momentum(void* stream, float* p, double* kinetic, int64_t n_chains, int32_t n_dim,
+                                const float* metric, uint32_t key0, uint32_t key1, int32_t step,
+                                const float* noise) {
+  FMC_ARG(p && kinetic, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1 && step >= 0, "bad sizes");
+  if (n_chains == 0) return FMC_OK;
+  hipLaunchKernelGGL(hmc_momentum_kernel, dim3(grid1(n_chains, 4)), dim3(256), 0, (hipStream_t)stream, p,
+                     kinetic, n_chains, n_dim, metric, key0, key1, step, noise);
+  return check_launch("hmc momentum");
+}
+
+extern "C" int fmc_hmc_drift(void* stream, float* x, const float* p, const float* metric, int64_t n_chains,
+                             int32_t n_dim, float coef) {
+  FMC_ARG(x && p, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1, "bad sizes");
+  int64_t n = n_chains * n_dim;
+  if (n == 0) return FMC_OK;
+  hipLaunchKernelGGL(hmc_drift_kernel, dim3(grid1(n, 256)), dim3(256), 0, (hipStream_t)stream, x, p, metric, n,
+                     n_dim, coef);
+  return check_launch("hmc drift");
+}
+
+extern "C" int fmc_hmc_kick(void* stream, float* p, const float* grad_logp, int64_t n_chains, int32_t n_dim,
+                            float coef) {
+  FMC_ARG(p && grad_logp, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1, "bad sizes");
+  int64_t n = n_chains * n_dim;
+  if (n == 0) return FMC_OK;
+  hipLaunchKernelGGL(hmc_kick_kernel, dim3(grid1(n, 256)), dim3(256), 0, (hipStream_t)stream, p, grad_logp, n,
+                     coef);
+  return check_launch("hmc kick");
+}
+
+extern "C" int fmc_hmc_accept(void* stream, float* x, float* logp, const float* x_new, const float* logp_new,
+                              const float* p_new, const double* kinetic0, const float* metric, int64_t n_chains,
+                              int32_t n_dim, uint32_t key0, uint32_t key1, int32_t step, const float* log_u,
+                              float* out_pos, float* out_logp, float* out_acc, int64_t buf_len, int64_t cursor,
+                              int32_t thinning) {
+  FMC_ARG(x && logp && x_new && logp_new && p_new && kinetic0, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1, "bad sizes");
+  int rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, step, 1);
+  if (rc) return rc;
+  if (n_chains == 0) return FMC_OK;
+  hipLaunchKernelGGL(hmc_accept_kernel, dim3(grid1(n_chains, 4)), dim3(256), 0, (hipStream_t)stream, x, logp,
+                     x_new, logp_new, p_new, kinetic0, metric, n_chains, n_dim, key0, key1, step, log_u, out_pos,
+                     out_logp, out_acc, buf_len, cursor, thinning);
+  return check_launch("hmc accept");
+}
+
+extern "C" int fmc_nf_mh(void* stream, const float* x0, const float* lp0, const float* q0, const float* prop,
+                         const float* lp_prop, const float* q_prop, int64_t n_chains, int32_t n_steps,
+                         int32_t n_dim, uint32_t key0, uint32_t key1, const float* log_u, int32_t* out_idx,
+                         float* x_last, float* lp_last, float* out_pos, float* out_logp, float* out_acc,
+                         int64_t buf_len, int64_t cursor, int32_t thinning) {
+  FMC_ARG(x0 && lp0 && q0 && prop && lp_prop && q_prop && out_idx, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_steps >= 1 && n_dim >= 1, "bad sizes");
+  int rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, 0, n_steps);
+  if (rc) return rc;
+  if (n_chains == 0) return FMC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(nf_mh_scan_kernel, dim3(grid1(n_chains, 256)), dim3(256), 0, st, lp0, q0, lp_prop, q_prop,
+                     n_chains, n_steps, key0, key1, log_u, out_idx, lp_last, out_logp, out_acc, buf_len, cursor,
+                     thinning);
+  if ((rc = check_launch("nf mh scan"))) return rc;
+  const int n_rows = (n_steps + thinning - 1) / thinning;
+  if (out_pos) {
+    hipLaunchKernelGGL(nf_mh_gather_kernel, dim3(grid1(n_chains * n_rows, 4)), dim3(256), 0, st, x0, prop,
+                       out_idx, n_chains, n_steps, n_dim, out_pos, buf_len, cursor, thinning, n_rows);
+    if ((rc = check_launch("nf mh gather"))) return rc;
+  }
+  if (x_last) {
+    const int s_last = (n_rows - 1) * thinning;
+    hipLaunchKernelGGL(nf_mh_last_kernel, dim3(grid1(n_chains * n_dim, 256)), dim3(256), 0, st, x0, prop,
+                       out_idx, n_chains, n_steps, n_dim, s_last, x_last);
+    if ((rc = check_launch("nf mh last"))) return rc;
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_philox_blocks(void* stream, uint32_t key0, uint32_t key1, uint32_t ctr0, uint32_t ctr1,
+                                 uint32_t ctr2, uint32_t ctr3, int64_t n_blocks, uint32_t* out) {
+  FMC_ARG(out != nullptr && n_blocks >= 0, "bad output");
+  if (n_blocks == 0) return FMC_OK;
+  hipLaunchKernelGGL(philox_blocks_kernel, dim3(grid1(n_blocks, 256)), dim3(256), 0, (hipStream_t)stream, key0,
+                     key1, ctr0, ctr1, ctr2, ctr3, n_blocks, out);
+  return check_launch("philox blocks");
+}
+
+extern "C" int fmc_rng_normal(void* stream, uint32_t key0, uint32_t key1, int64_t n_chains, int32_t n_dim,
+                              int32_t n_steps, int32_t step0, int32_t stream_id, float* out) {
+  FMC_ARG(out != nullptr && n_chains >= 0 && n_dim >= 1 && n_steps >= 0 && step0 >= 0, "bad arguments");
+  int64_t n = (int64_t)n_steps * n_chains * n_dim;
+  if (n == 0) return FMC_OK;
+  hipLaunchKernelGGL(rng_normal_kernel, dim3(grid1(n, 256)), dim3(256), 0, (hipStream_t)stream, key0, key1,
+                     n_chains, n_dim, n_steps, step0, (uint32_t)stream_id, out);
+  return check_launch("rng normal");
+}
+
+extern "C" int fmc_rng_log_uniform(void* stream, uint32_t key0, uint32_t key1, int64_t n_chains, int32_t n_steps,
+                                   int32_t step0, int32_t stream_id, float* out) {
+  FMC_ARG(out != nullptr && n_chains >= 0 && n_steps >= 0 && step0 >= 0, "bad arguments");
+  int64_t n = (int64_t)n_steps * n_chains;
+  if (n == 0) return FMC_OK;
+  hipLaunchKernelGGL(rng_log_uniform_kernel, dim3(grid1(n, 256)), dim3(256), 0, (hipStream_t)stream, key0, key1,
+                     n_chains, n_steps, step0, (uint32_t)stream_id, out);
+  return check_launch("rng log uniform");
+}

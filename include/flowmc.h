@@ -1,0 +1,227 @@
+/*
+ * flowmc.h — C ABI of the MI355X (gfx950) flowMC sampling hot path.
+ *
+ * Every entry point:
+ *   - takes device pointers owned by the caller (no allocation inside hot calls),
+ *   - is stream-ordered on the `stream` argument (a hipStream_t passed as void*),
+ *   - returns FMC_OK (0) or a non-zero fmc_status; the message of the last
+ *     failure on the calling thread is available from fmc_last_error().
+ * Nothing throws across this boundary.
+ *
+ * Reference citations are relative to the flowMC v0.4.5 source tree.
+ */
+#ifndef FLOWMC_H_
+#define FLOWMC_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMC_ABI_VERSION 1
+
+typedef enum fmc_status {
+  FMC_OK = 0,
+  FMC_ERR_INVALID_ARG = 1,  /* null pointer, bad size, bad enum            */
+  FMC_ERR_UNSUPPORTED = 2,  /* valid request this build cannot serve       */
+  FMC_ERR_LAUNCH = 3,       /* kernel launch failed                        */
+  FMC_ERR_HIP = 4           /* other HIP runtime error                     */
+} fmc_status;
+
+/* ------------------------------------------------------------------------ */
+/* Lifecycle / introspection                                                */
+/* ------------------------------------------------------------------------ */
+int fmc_abi_version(void);
+/* Thread-local message of the last failing call on this thread ("" if none). */
+const char* fmc_last_error(void);
+/* Device properties used by the host-side tile chooser. */
+int fmc_device_query(int device, int* n_cu, int* lds_bytes_per_block,
+                     int* warp_size, int* arch_major, int* arch_minor);
+
+/* ------------------------------------------------------------------------ */
+/* Built-in log-densities (value + hand-written gradient on device).        */
+/* Replaces the user `logpdf(x, data)` + jax.value_and_grad pair that       */
+/* MALA.kernel (src/flowMC/resource/kernel/MALA.py:59) and HMC              */
+/* (HMC.py:71-80) differentiate with autodiff.                              */
+/* ------------------------------------------------------------------------ */
+typedef enum fmc_target_kind {
+  FMC_TARGET_GAUSS_DIAG = 0,  /* params: mean[d], prec[d]:  -1/2 sum prec (x-mean)^2          */
+  FMC_TARGET_GAUSS_DENSE = 1, /* params: mean[d], P[d*d]:  -1/2 (x-mean)^T P (x-mean)         */
+  FMC_TARGET_ROSENBROCK = 2,  /* params: scale[1]: -sum_i [100(x_{i+1}-x_i^2)^2+(1-x_i)^2]*scale */
+  FMC_TARGET_FUNNEL = 3,      /* params: none; x_0 = v ~ N(0,9), x_i|v ~ N(0,e^v)              */
+  FMC_TARGET_GMM_DIAG = 4,    /* params: mean[K*d], inv_sigma[K*d], c[K] (c = log w - sum log s) */
+  FMC_TARGET_DUAL_MOON = 5    /* params: center[d]; docs/tutorials/dualmoon.ipynb target        */
+} fmc_target_kind;
+
+typedef struct fmc_target {
+  int32_t kind;        /* fmc_target_kind                                   */
+  int32_t n_dim;       /* d                                                 */
+  int32_t n_modes;     /* K for FMC_TARGET_GMM_DIAG, else 0                 */
+  int32_t reserved;
+  const float* params; /* device pointer to the parameter block (may be 0 for FUNNEL) */
+  int64_t n_params;    /* number of floats in params                        */
+} fmc_target;
+
+/* Lane layout of the chain kernels: each chain is held by `lanes_per_chain`
+ * lanes (power of two, 1..64) of one wavefront; each lane holds
+ * `vec_per_lane` float4 chunks (1,2,4,8). d <= 4*vec_per_lane*lanes_per_chain.
+ * Zero fields mean "choose by heuristic". */
+typedef struct fmc_launch_cfg {
+  int32_t lanes_per_chain;
+  int32_t vec_per_lane;
+  int32_t block_size; /* threads per workgroup (64..1024, multiple of 64) */
+  int32_t reserved;
+} fmc_launch_cfg;
+
+/* Value (and optionally gradient) of a built-in target at C points.
+ * x [C,d] -> logp [C], grad [C,d] (grad may be NULL). */
+int fmc_target_eval(void* stream, const fmc_target* target, const float* x,
+                    int64_t n_points, float* logp, float* grad,
+                    const fmc_launch_cfg* cfg);
+
+/* ------------------------------------------------------------------------ */
+/* Fused K-step MALA.                                                       */
+/* Replaces TakeSerialSteps.sample (src/flowMC/strategy/take_steps.py:164-  */
+/* 180) scanning MALA.kernel (src/flowMC/resource/kernel/MALA.py:26-89)     */
+/* vmapped over chains (take_steps.py:127-132), plus the thinned buffer     */
+/* writes of TakeSteps.__call__ (take_steps.py:134-143).                    */
+/*                                                                          */
+/* Steps step0 .. step0+n_steps-1 of one strategy call are run. The chain   */
+/* state x[C,d] / logp[C] is read at entry and written back at exit; the    */
+/* kernel evaluates (logp, grad) of the entry state itself, as the          */
+/* reference does at take_steps.py:177 and MALA.py:59.                      */
+/* Noise: if noise != NULL the Langevin noise of step s, chain c is         */
+/* noise[(s*C + c)*d + j] and the MH log-uniform is log_u[s*C + c]          */
+/* (injected-noise mode); otherwise both come from Philox4x32-10 keyed by   */
+/* (key0, key1) with counter (dim_chunk, chain, step0+s, stream_id).        */
+/* Outputs (any may be NULL): rows of the [C, buf_len, d] / [C, buf_len]    */
+/* buffers: global step t (= step0 + s) with t % thinning == 0 is written    */
+/* to row cursor + t / thinning; acceptance is stored as 1.0f / 0.0f.       */
+/* ------------------------------------------------------------------------ */
+int fmc_mala_run(void* stream, const fmc_target* target, float* x, float* logp,
+                 int64_t n_chains, int32_t n_steps, int32_t step0, float step_size,
+                 uint32_t key0, uint32_t key1, const float* noise, const float* log_u,
+                 float* out_pos, float* out_logp, float* out_acc, int64_t buf_len,
+                 int64_t cursor, int32_t thinning, const fmc_launch_cfg* cfg);
+
+/* ------------------------------------------------------------------------ */
+/* Fused K-step HMC.                                                        */
+/* Replaces HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) with its  */
+/* leapfrog scan (HMC.py:71-96) and coefficient table (HMC.py:47-50:        */
+/* n_leapfrog+1 drifts, n_leapfrog+2 kicks), scanned by TakeSerialSteps.    */
+/* metric: NULL = identity, else diag(metric[d]) (the condition matrix);    */
+/* momentum p = xi / sqrt(metric), kinetic = 1/2 sum metric p^2.            */
+/* The Hamiltonian difference H0 - H1 is formed in FP64.                    */
+/* Noise: noise[(s*C + c)*d + j] is the momentum draw, log_u[s*C + c].      */
+/* ------------------------------------------------------------------------ */
+int fmc_hmc_run(void* stream, const fmc_target* target, float* x, float* logp,
+                int64_t n_chains, int32_t n_steps, int32_t step0, float step_size,
+                int32_t n_leapfrog, const float* metric, uint32_t key0, uint32_t key1,
+                const float* noise, const float* log_u, float* out_pos, float* out_logp,
+                float* out_acc, int64_t buf_len, int64_t cursor, int32_t thinning,
+                const fmc_launch_cfg* cfg);
+
+/* Many independent HMC problems of uneven dimension in ONE launch.        */
+/* slots[n_slots] (device memory) describe each problem; items[n_items]    */
+/* (device memory) are the (slot, first chain, n chains) tiles, one per    */
+/* workgroup. Each slot carries its own lanes_per_chain.                   */
+typedef struct fmc_hmc_slot {
+  fmc_target target;
+  float* x;        /* [C_t, d_t] */
+  float* logp;     /* [C_t]      */
+  const float* metric; /* [d_t] or NULL */
+  const float* noise;  /* [n_steps, C_t, d_t] or NULL */
+  const float* log_u;  /* [n_steps, C_t] or NULL */
+  float* out_pos;  /* [C_t, buf_len, d_t] or NULL */
+  float* out_logp; /* [C_t, buf_len] or NULL */
+  float* out_acc;  /* [C_t, buf_len] or NULL */
+  int64_t n_chains;
+  int64_t buf_len;
+  int64_t cursor;
+  uint32_t key0, key1;
+  int32_t lanes_per_chain;
+  int32_t reserved;
+} fmc_hmc_slot;
+
+typedef struct fmc_work_item {
+  int32_t slot;
+  int32_t chain0;
+  int32_t n_chains;
+  int32_t reserved;
+} fmc_work_item;
+
+int fmc_hmc_run_batched(void* stream, const fmc_hmc_slot* slots, int32_t n_slots,
+                        const fmc_work_item* items, int32_t n_items, int32_t max_dim,
+                        int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
+                        int32_t thinning, int32_t vec_per_lane, int32_t block_size);
+
+/* ------------------------------------------------------------------------ */
+/* Per-step building blocks for user-supplied (non built-in) targets whose  */
+/* gradient is computed outside this library.                              */
+/* ------------------------------------------------------------------------ */
+/* MALA proposal: prop = x + dt^2/2 grad + dt * xi   (MALA.py:60-63).     */
+int fmc_mala_propose(void* stream, const float* x, const float* grad, float* prop,
+                     int64_t n_chains, int32_t n_dim, float step_size, uint32_t key0,
+                     uint32_t key1, int32_t step, const float* noise);
+/* MALA MH accept (MALA.py:75-87) and thinned buffer write (take_steps.py:134-143).
+ * x/logp/grad are updated in place to the accepted state. */
+int fmc_mala_accept(void* stream, float* x, float* logp, float* grad, const float* prop,
+                    const float* prop_logp, const float* prop_grad, int64_t n_chains,
+                    int32_t n_dim, float step_size, uint32_t key0, uint32_t key1,
+                    int32_t step, const float* log_u, float* out_pos, float* out_logp,
+                    float* out_acc, int64_t buf_len, int64_t cursor, int32_t thinning);
+/* HMC pieces (HMC.py:128-149): momentum draw + kinetic energy (FP64),
+ * drift x += a * metric * p, kick p -= b * grad, and the FP64 accept. */
+int fmc_hmc_momentum(void* stream, float* p, double* kinetic, int64_t n_chains,
+                     int32_t n_dim, const float* metric, uint32_t key0, uint32_t key1,
+                     int32_t step, const float* noise);
+int fmc_hmc_drift(void* stream, float* x, const float* p, const float* metric,
+                  int64_t n_elem_chains, int32_t n_dim, float coef);
+int fmc_hmc_kick(void* stream, float* p, const float* grad_logp, int64_t n_chains,
+                 int32_t n_dim, float coef);
+int fmc_hmc_accept(void* stream, float* x, float* logp, const float* x_new,
+                   const float* logp_new, const float* p_new, const double* kinetic0,
+                   const float* metric, int64_t n_chains, int32_t n_dim, uint32_t key0,
+                   uint32_t key1, int32_t step, const float* log_u, float* out_pos,
+                   float* out_logp, float* out_acc, int64_t buf_len, int64_t cursor,
+                   int32_t thinning);
+
+/* ------------------------------------------------------------------------ */
+/* Normalizing-flow independence Metropolis–Hastings                        */
+/* (NFProposal.kernel's scan, src/flowMC/resource/kernel/NF_proposal.py:    */
+/* 91-126). Per chain c, proposals s = 0..S-1:                             */
+/*   r = (lp_prop - lp_cur) - (q_prop - q_cur); accept iff log_u < r.       */
+/* Writes, for each step, the index of the proposal the chain sits on       */
+/* (-1 = still at the entry state) plus logp and acceptance, then a gather  */
+/* kernel fills the [C, buf_len, d] position buffer rows.                  */
+/* Proposals are laid out [C, S, d] / [C, S].                              */
+/* ------------------------------------------------------------------------ */
+int fmc_nf_mh(void* stream, const float* x0, const float* lp0, const float* q0,
+              const float* prop, const float* lp_prop, const float* q_prop,
+              int64_t n_chains, int32_t n_steps, int32_t n_dim, uint32_t key0,
+              uint32_t key1, const float* log_u, int32_t* out_idx, float* x_last,
+              float* lp_last, float* out_pos, float* out_logp, float* out_acc,
+              int64_t buf_len, int64_t cursor, int32_t thinning);
+
+/* ------------------------------------------------------------------------ */
+/* Random numbers (the exact streams the fused kernels use).                */
+/* ------------------------------------------------------------------------ */
+/* Raw Philox4x32-10 blocks: out[4*i .. 4*i+3] = philox(ctr_i, key) with    */
+/* ctr_i = (ctr0 + i, ctr1, ctr2, ctr3). */
+int fmc_philox_blocks(void* stream, uint32_t key0, uint32_t key1, uint32_t ctr0,
+                      uint32_t ctr1, uint32_t ctr2, uint32_t ctr3, int64_t n_blocks,
+                      uint32_t* out);
+/* Standard normals the chain kernels draw for (chain c, step t, dim j):    */
+/* out[(s*C + c)*d + j] for t = step0 + s, using stream id `stream_id`.    */
+int fmc_rng_normal(void* stream, uint32_t key0, uint32_t key1, int64_t n_chains,
+                   int32_t n_dim, int32_t n_steps, int32_t step0, int32_t stream_id,
+                   float* out);
+/* log-uniforms the chain kernels draw: out[s*C + c]. */
+int fmc_rng_log_uniform(void* stream, uint32_t key0, uint32_t key1, int64_t n_chains,
+                        int32_t n_steps, int32_t step0, int32_t stream_id, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLOWMC_H_ */

@@ -1,0 +1,316 @@
+"""GPU parity of the chain kernels against the FP64 oracle.
+
+Tolerances (FP32 kernels vs FP64 oracle on identical injected noise):
+  * target values / gradients: |err| <= 2e-5 * (1 + |ref|) (relative to scale)
+  * MALA / HMC trajectories over the 100-step window: positions within
+    atol 2e-4 * (1 + |x|); log-probs within 2e-4 * (1 + |lp|)
+  * accept/reject decisions identical, except on chains where some step's
+    oracle log-ratio lies within MARGIN of the log-uniform (counted; such a
+    chain is excluded from the position comparison and must be rare).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import flowmc_amd._native as N
+from flowmc_amd import random as fr
+from flowmc_amd import targets as TG
+from flowmc_amd.resource.buffers import Buffer
+from flowmc_amd.resource.kernel.base import StepOutputs
+from flowmc_amd.resource.kernel.HMC import HMC
+from flowmc_amd.resource.kernel.MALA import MALA
+from flowmc_amd.resource.logPDF import LogPDF
+from flowmc_amd.resource.states import State
+from flowmc_amd.strategy.take_steps import TakeSerialSteps
+from oracle import chains as OC
+from oracle import philox as OP
+from oracle import targets as OT
+
+pytestmark = pytest.mark.gpu
+
+MARGIN = 2e-4
+
+
+def make_target(name, d, device="cuda", seed=0):
+    g = torch.Generator().manual_seed(seed)
+    if name == "gauss_diag":
+        mean = torch.randn(d, generator=g, dtype=torch.float64)
+        prec = 0.5 + torch.rand(d, generator=g, dtype=torch.float64)
+        return TG.GaussianDiag(mean, prec, device), lambda z: OT.gauss_diag(z, mean, prec)
+    if name == "gauss_dense":
+        t = TG.correlated_gaussian_5d(seed, n_dim=d, device=device)
+        P = t.precision.to(torch.float64)
+        return t, lambda z: OT.gauss_dense(z, t.mean.to(torch.float64), P)
+    if name == "rosenbrock":
+        return TG.Rosenbrock(d, device=device), lambda z: OT.rosenbrock(z)
+    if name == "funnel":
+        return TG.Funnel(d, device=device), OT.funnel
+    if name == "gmm":
+        t = TG.gaussian_mixture_64d(seed, n_dim=d, n_modes=4, device=device)
+        return t, lambda z: OT.gmm_diag(z, t.means, t.sigmas)
+    if name == "dual_moon":
+        c = torch.arange(d, dtype=torch.float64) * 0.1
+        return TG.DualMoon(d, c, device), lambda z: OT.dual_moon(z, c)
+    raise ValueError(name)
+
+
+def test_philox_blocks_bit_exact(gpu):
+    key = (0xDEADBEEF, 0x12345678)
+    n = 4096
+    out = torch.empty(4 * n, dtype=torch.int32, device=gpu)
+    N.call("fmc_philox_blocks", N.stream_ptr(), key[0], key[1], 7, 3, 11, 2, n, out.data_ptr())
+    got = out.cpu().numpy().view(np.uint32).reshape(n, 4)
+    ctr = np.stack([7 + np.arange(n), np.full(n, 3), np.full(n, 11), np.full(n, 2)], -1).astype(np.uint32)
+    want = OP.philox4x32_10(ctr, key)
+    assert np.array_equal(got, want)
+
+
+def test_rng_streams_match_oracle(gpu):
+    key = (5, 77)
+    C_, d, S = 300, 13, 3
+    z = torch.empty(S, C_, d, device=gpu, dtype=torch.float32)
+    N.call("fmc_rng_normal", N.stream_ptr(), key[0], key[1], C_, d, S, 10, OP.RNG_MALA_NOISE, z.data_ptr())
+    want = OP.normals(key, C_, d, S, 10, OP.RNG_MALA_NOISE)
+    assert np.allclose(z.cpu().numpy(), want, atol=2e-6 * (1 + np.abs(want)).max(), rtol=2e-6)
+    lu = torch.empty(S, C_, device=gpu, dtype=torch.float32)
+    N.call("fmc_rng_log_uniform", N.stream_ptr(), key[0], key[1], C_, S, 10, OP.RNG_MALA_UNIFORM, lu.data_ptr())
+    want = OP.log_uniforms(key, C_, S, 10, OP.RNG_MALA_UNIFORM)
+    assert np.allclose(lu.cpu().numpy(), want, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("name,d", [("gauss_diag", 5), ("gauss_diag", 130), ("gauss_dense", 5), ("gauss_dense", 12),
+                                    ("rosenbrock", 128), ("rosenbrock", 7), ("funnel", 16), ("funnel", 201),
+                                    ("gmm", 64), ("gmm", 9), ("dual_moon", 5)])
+@pytest.mark.parametrize("layout", [None, (1, 8), (4, 2), (64, 1)])
+def test_target_value_and_grad(gpu, name, d, layout):
+    tgt, fn = make_target(name, d)
+    cfg = None
+    if layout is not None:
+        lpc, nv = layout
+        if 4 * lpc * nv < d:
+            pytest.skip("layout too small for d")
+        cfg = N.fmc_launch_cfg(lpc, nv, 256, 0)
+    g = torch.Generator().manual_seed(1)
+    x64 = torch.randn(257, d, generator=g, dtype=torch.float64) * 0.8
+    if name == "funnel":
+        x64[:, 0] = x64[:, 0].clamp(-2, 2)
+    x = x64.to(torch.float32).to(gpu)
+    lp = torch.empty(257, device=gpu, dtype=torch.float32)
+    gr = torch.empty_like(x)
+    desc = tgt.descriptor()
+    N.call("fmc_target_eval", N.stream_ptr(), C.byref(desc), x.data_ptr(), 257, lp.data_ptr(), gr.data_ptr(),
+           C.byref(cfg) if cfg is not None else None)
+    v_ref, g_ref = OT.value_and_grad(fn, x.cpu().double())
+    scale_v = 1 + v_ref.abs()
+    assert ((lp.cpu().double() - v_ref).abs() <= 2e-5 * scale_v).all()
+    scale_g = 1 + g_ref.abs().amax(-1, keepdim=True)
+    assert ((gr.cpu().double() - g_ref).abs() <= 2e-5 * scale_g).all()
+
+
+def _compare_traj(kind, name, d, C_, S, dt, thinning=1, n_leapfrog=5, seed=0):
+    tgt, fn = make_target(name, d, seed=seed)
+    g = torch.Generator().manual_seed(seed + 11)
+    x0 = torch.randn(C_, d, generator=g, dtype=torch.float64) * 0.5
+    noise = torch.randn(S, C_, d, generator=g, dtype=torch.float64)
+    log_u = torch.log(torch.rand(S, C_, generator=g, dtype=torch.float64))
+    metric = 0.5 + torch.rand(d, generator=g, dtype=torch.float64) if kind == "hmc" else None
+    n_rows = (S + thinning - 1) // thinning
+    pos = torch.full((C_, n_rows + 3, d), float("nan"), device="cuda", dtype=torch.float32)
+    lps = torch.full((C_, n_rows + 3), float("nan"), device="cuda", dtype=torch.float32)
+    accs = torch.full((C_, n_rows + 3), float("nan"), device="cuda", dtype=torch.float32)
+    out = StepOutputs(pos, lps, accs, 2)
+    if kind == "mala":
+        prop = MALA(dt, fused_steps=32)
+    else:
+        prop = HMC(metric.to(torch.float32), dt, n_leapfrog, fused_steps=8)
+    logpdf = LogPDF(tgt, n_dims=d)
+    last = prop.sample_steps(fr.PRNGKey(0), x0.float().cuda(), logpdf, None, S, thinning, out,
+                             noise=noise.float().cuda(), log_u=log_u.float().cuda())
+    torch.cuda.synchronize()
+    # oracle on the FP32-rounded inputs
+    kw = dict(step_size=dt)
+    if kind == "hmc":
+        kw.update(n_leapfrog=n_leapfrog, metric=metric.float().double())
+    r_pos, r_lp, r_acc, r_last, ratios = OC.take_serial_steps(kind, x0.float().double(), fn, noise.float().double(),
+                                                              log_u.float().double(), thinning, **kw)
+    close = ((ratios - log_u.float().double().T).abs() < MARGIN).any(1)  # [C]
+    n_close = int(close.sum())
+    assert n_close <= max(2, (15 * C_) // 100), f"{n_close} chains near the MH threshold"
+    ok = ~close
+    gp = pos[:, 2:2 + n_rows].cpu().double()
+    gl = lps[:, 2:2 + n_rows].cpu().double()
+    ga = accs[:, 2:2 + n_rows].cpu().double()
+    assert torch.equal(ga[ok], r_acc[ok]), "accept/reject decisions differ"
+    assert ((gp[ok] - r_pos[ok]).abs() <= 2e-4 * (1 + r_pos[ok].abs())).all()
+    assert ((gl[ok] - r_lp[ok]).abs() <= 2e-4 * (1 + r_lp[ok].abs())).all()
+    assert ((last.cpu().double()[ok] - r_last[ok]).abs() <= 2e-4 * (1 + r_last[ok].abs())).all()
+    # rows outside the written window are untouched
+    assert torch.isnan(pos[:, :2]).all() and torch.isnan(pos[:, 2 + n_rows:]).all()
+    return ga
+
+
+@pytest.mark.parametrize("name,d", [("gauss_dense", 5), ("rosenbrock", 128), ("gmm", 64), ("funnel", 20),
+                                    ("dual_moon", 5), ("gauss_diag", 33)])
+def test_mala_fused_matches_oracle(gpu, name, d):
+    dt = {"gauss_dense": 0.3, "rosenbrock": 0.01, "gmm": 0.2, "funnel": 0.2, "dual_moon": 0.05,
+          "gauss_diag": 0.4}[name]
+    acc = _compare_traj("mala", name, d, 192, 100, dt)
+    assert 0.05 < float(acc.mean()) < 1.0
+
+
+def test_mala_thinning_and_cursor(gpu):
+    _compare_traj("mala", "gauss_diag", 6, 64, 23, 0.5, thinning=4)
+
+
+@pytest.mark.parametrize("name,d", [("gauss_diag", 8), ("funnel", 16), ("rosenbrock", 12), ("gmm", 10)])
+def test_hmc_fused_matches_oracle(gpu, name, d):
+    dt = {"gauss_diag": 0.2, "funnel": 0.05, "rosenbrock": 0.02, "gmm": 0.1}[name]
+    acc = _compare_traj("hmc", name, d, 128, 40, dt, n_leapfrog=6)
+    assert float(acc.mean()) > 0.05
+
+
+def test_philox_mode_equals_injected_mode(gpu):
+    tgt, _ = make_target("rosenbrock", 64)
+    C_, S, d = 500, 40, 64
+    key = fr.PRNGKey(99)
+    k0, k1 = fr.key_words(key)
+    x0 = torch.randn(C_, d, device=gpu) * 0.3
+    noise = torch.empty(S, C_, d, device=gpu, dtype=torch.float32)
+    lu = torch.empty(S, C_, device=gpu, dtype=torch.float32)
+    N.call("fmc_rng_normal", N.stream_ptr(), k0, k1, C_, d, S, 0, OP.RNG_MALA_NOISE, noise.data_ptr())
+    N.call("fmc_rng_log_uniform", N.stream_ptr(), k0, k1, C_, S, 0, OP.RNG_MALA_UNIFORM, lu.data_ptr())
+    res = []
+    for nz, l in ((None, None), (noise, lu)):
+        pos = torch.zeros(C_, S, d, device=gpu, dtype=torch.float32)
+        lp = torch.zeros(C_, S, device=gpu, dtype=torch.float32)
+        ac = torch.zeros(C_, S, device=gpu, dtype=torch.float32)
+        MALA(0.01, fused_steps=16).sample_steps(key, x0, LogPDF(tgt, n_dims=d), None, S, 1,
+                                               StepOutputs(pos, lp, ac, 0), noise=nz, log_u=l)
+        res.append((pos, lp, ac))
+    for a, b in zip(*res):
+        assert torch.equal(a, b)
+
+
+def test_generic_target_path_matches_fused(gpu):
+    # user-supplied torch callable (torch.func gradient + propose/accept kernels)
+    d, C_, S = 5, 64, 30
+    mean = torch.arange(d, dtype=torch.float32, device=gpu)
+
+    def user_logpdf(x, data):
+        return -0.5 * torch.sum((x - data["data"]) ** 2)
+
+    native = LogPDF(TG.GaussianDiag(mean.cpu(), None), n_dims=d)
+    generic = LogPDF(user_logpdf, n_dims=d)
+    key = fr.PRNGKey(3)
+    x0 = torch.randn(C_, d, device=gpu)
+    outs = []
+    for lpdf in (native, generic):
+        pos = torch.zeros(C_, S, d, device=gpu, dtype=torch.float32)
+        lp = torch.zeros(C_, S, device=gpu, dtype=torch.float32)
+        ac = torch.zeros(C_, S, device=gpu, dtype=torch.float32)
+        MALA(0.4).sample_steps(key, x0, lpdf, {"data": mean}, S, 1, StepOutputs(pos, lp, ac, 0))
+        outs.append((pos, lp, ac))
+    (p1, l1, a1), (p2, l2, a2) = outs
+    agree = (a1 == a2).all(1)
+    assert float(agree.float().mean()) > 0.95
+    assert torch.allclose(p1[agree], p2[agree], atol=1e-4, rtol=1e-4)
+
+
+def test_nf_mh_matches_oracle(gpu):
+    C_, S, d = 300, 50, 7
+    g = torch.Generator().manual_seed(5)
+    x0 = torch.randn(C_, d, generator=g)
+    lp0, q0 = torch.randn(C_, generator=g), torch.randn(C_, generator=g)
+    prop = torch.randn(C_, S, d, generator=g)
+    lpp, qp = torch.randn(C_, S, generator=g), torch.randn(C_, S, generator=g)
+    lu = torch.log(torch.rand(C_, S, generator=g))
+    r_pos, r_lp, r_acc = OC.nf_mh_scan(x0.double(), lp0.double(), q0.double(), prop.double(), lpp.double(),
+                                       qp.double(), lu.double())
+    for thin in (1, 3):
+        n_rows = (S + thin - 1) // thin
+        pos = torch.zeros(C_, n_rows + 1, d, device=gpu, dtype=torch.float32)
+        lps = torch.zeros(C_, n_rows + 1, device=gpu, dtype=torch.float32)
+        acs = torch.zeros(C_, n_rows + 1, device=gpu, dtype=torch.float32)
+        idx = torch.empty(C_, S, dtype=torch.int32, device=gpu)
+        xl = torch.empty(C_, d, device=gpu, dtype=torch.float32)
+        ll = torch.empty(C_, device=gpu, dtype=torch.float32)
+        dev = [t.contiguous().cuda() for t in (x0, lp0, q0, prop, lpp, qp, lu)]  # keep alive across the call
+        N.call("fmc_nf_mh", N.stream_ptr(), *[t.data_ptr() for t in dev[:6]], C_, S, d, 0, 0, dev[6].data_ptr(),
+               idx.data_ptr(), xl.data_ptr(), ll.data_ptr(), pos.data_ptr(), lps.data_ptr(), acs.data_ptr(),
+               n_rows + 1, 1, thin)
+        torch.cuda.synchronize()
+        sel = slice(None, None, thin)
+        assert torch.equal(acs[:, 1:].cpu().bool(), r_acc[:, sel])
+        assert torch.equal(pos[:, 1:].cpu(), r_pos[:, sel].float())
+        assert torch.equal(lps[:, 1:].cpu(), r_lp[:, sel].float())
+        assert torch.equal(xl.cpu(), r_pos[:, sel][:, -1].float())
+
+
+# ---------------------------------------- reference-style statistical tests
+def _strategy_run(kernel, logpdf, n_chains, n_steps, d, key=42, x0=None):
+    positions = Buffer("positions", (n_chains, n_steps, d), 1)
+    log_prob = Buffer("log_prob", (n_chains, n_steps), 1)
+    acceptance = Buffer("acceptance", (n_chains, n_steps), 1)
+    state = State({"positions": "positions", "log_prob": "log_prob", "acceptance": "acceptance"},
+                  name="sampler_state")
+    resources = {"positions": positions, "log_prob": log_prob, "acceptance": acceptance, "K": kernel,
+                 "logpdf": logpdf, "sampler_state": state}
+    strat = TakeSerialSteps("logpdf", kernel_name="K", state_name="sampler_state",
+                            buffer_names=["positions", "log_prob", "acceptance"], n_steps=n_steps)
+    k, sub = fr.split(fr.PRNGKey(key))
+    x0 = fr.normal(sub, (n_chains, d)) if x0 is None else x0
+    _, res, last = strat(k, resources, x0, {})
+    return res, last, strat
+
+
+def test_mala_close_gaussian(gpu):
+    # test/unit/test_kernels.py:241-288: N(0, I_2), step 1, 1 chain x 50k steps
+    logpdf = LogPDF(TG.GaussianDiag(torch.zeros(2, dtype=torch.float32)), n_dims=2)
+    res, _, strat = _strategy_run(MALA(1.0), logpdf, 1, 50000, 2)
+    data = res["positions"].data
+    assert abs(float(data.mean())) < 3e-2 and abs(float(data.var()) - 1) < 3e-2
+    assert strat.current_position == 50000
+
+
+def test_hmc_close_gaussian(gpu):
+    # test/unit/test_kernels.py:135-184: N(0, I_2), step 0.1, 5 leapfrog, 30k steps
+    logpdf = LogPDF(TG.GaussianDiag(torch.zeros(2, dtype=torch.float32)), n_dims=2)
+    res, _, _ = _strategy_run(HMC(torch.eye(2), 0.1, 5), logpdf, 1, 30000, 2)
+    data = res["positions"].data
+    assert abs(float(data.mean())) < 3e-2 and abs(float(data.var()) - 1) < 3e-2
+
+
+def test_acceptance_goes_to_one(gpu):
+    logpdf = LogPDF(TG.GaussianDiag(torch.zeros(2, dtype=torch.float32)), n_dims=2)
+    x = fr.normal(fr.PRNGKey(1), (100, 2))
+    _, _, acc = MALA(1e-5).kernel(fr.PRNGKey(2), x, None, logpdf, None)
+    assert bool(acc.all())
+    _, _, acc = HMC(torch.eye(2), 1e-7, 5).kernel(fr.PRNGKey(2), x, logpdf(x), logpdf, None)
+    assert bool(acc.all())
+
+
+def test_kernels_deterministic(gpu):
+    logpdf = LogPDF(TG.GaussianDiag(torch.zeros(2, dtype=torch.float32)), n_dims=2)
+    x = fr.normal(fr.PRNGKey(4), (1, 2))[0]
+    for k in (MALA(1.0), HMC(torch.eye(2), 1.0, 5)):
+        r1 = k.kernel(fr.PRNGKey(7), x, logpdf(x), logpdf, None)
+        r2 = k.kernel(fr.PRNGKey(7), x, logpdf(x), logpdf, None)
+        assert torch.equal(r1[0], r2[0]) and torch.equal(r1[1], r2[1]) and bool(r1[2] == r2[2])
+
+
+def test_nonfinite_target_rejects(gpu):
+    # NaN / -inf log-densities never accept (MALA.py:84 compares log u < NaN -> False)
+    d = 3
+
+    def bad(x, data):
+        return torch.where(x[0] > 0.0, torch.tensor(float("nan"), device=x.device), -0.5 * torch.sum(x * x))
+
+    x0 = -torch.ones(64, d, device=gpu, dtype=torch.float32) * 0.1
+    pos = torch.zeros(64, 20, d, device=gpu, dtype=torch.float32)
+    lp = torch.zeros(64, 20, device=gpu, dtype=torch.float32)
+    ac = torch.zeros(64, 20, device=gpu, dtype=torch.float32)
+    MALA(0.5).sample_steps(fr.PRNGKey(0), x0, LogPDF(bad, n_dims=d), None, 20, 1, StepOutputs(pos, lp, ac, 0))
+    assert bool(torch.isfinite(pos).all()) and bool((pos[..., 0] <= 0).all())

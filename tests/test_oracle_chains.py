@@ -1,0 +1,141 @@
+"""Pins the oracle's MALA / HMC / NF-MH restatements to properties the
+mathematics fixes: detailed balance of MALA (densities from scipy), leapfrog
+reversibility and energy behaviour, gradient-call counts, acceptance -> 1."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.stats as ss
+import torch
+
+from oracle import chains, targets as T
+
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _float64_default():
+    old = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(old)
+
+
+def test_mala_detailed_balance_gaussian():
+    # pi = N(mu, Sigma); MALA transition density from x to y is
+    # q(y|x) * min(1, e^{r(x,y)}); detailed balance needs
+    # pi(x) q(y|x) a(x,y) == pi(y) q(x|y) a(y,x)
+    rng = np.random.default_rng(0)
+    d, dt = 3, 0.7
+    A = rng.normal(size=(d, d))
+    cov = A @ A.T + np.eye(d)
+    P = np.linalg.inv(cov)
+    mu = rng.normal(size=d)
+    fn = lambda z: T.gauss_dense(z, torch.tensor(mu), torch.tensor(P))
+    for _ in range(20):
+        x = torch.tensor(rng.normal(size=(1, d)))
+        xi = torch.tensor(rng.normal(size=(1, d)))
+        _, _, _, r_xy = chains.mala_step(x, fn, dt, xi, torch.tensor([-1e300]))
+        y = x + dt * dt / 2 * torch.tensor((-(x.numpy() - mu) @ P)) + dt * xi
+        xi_back = (x - y - dt * dt / 2 * torch.tensor(-(y.numpy() - mu) @ P)) / dt
+        _, _, _, r_yx = chains.mala_step(y, fn, dt, xi_back, torch.tensor([-1e300]))
+        grad = lambda z: -(z - mu) @ P
+        q_yx = ss.multivariate_normal(x.numpy()[0] + dt * dt / 2 * grad(x.numpy()[0]), dt * dt * np.eye(d)).logpdf(
+            y.numpy()[0])
+        q_xy = ss.multivariate_normal(y.numpy()[0] + dt * dt / 2 * grad(y.numpy()[0]), dt * dt * np.eye(d)).logpdf(
+            x.numpy()[0])
+        pi = ss.multivariate_normal(mu, cov)
+        lhs = pi.logpdf(x.numpy()[0]) + q_yx + min(0.0, float(r_xy))
+        rhs = pi.logpdf(y.numpy()[0]) + q_xy + min(0.0, float(r_yx))
+        assert abs(lhs - rhs) < 1e-9
+        assert abs(float(r_xy) + float(r_yx)) < 1e-9
+
+
+def test_mala_acceptance_goes_to_one():
+    x = torch.randn(100, 2)
+    xi = torch.randn(100, 2)
+    lu = torch.log(torch.rand(100))
+    fn = lambda z: -0.5 * (z * z).sum(-1)
+    _, _, acc, _ = chains.mala_step(x, fn, 1e-5, xi, lu)
+    assert bool(acc.all())
+
+
+def _leapfrog_only(x, p, fn, eps, L):
+    # the coefficient table of HMC.py:47-50 without the MH step
+    coefs = torch.ones(L + 2, 2)
+    coefs[0] = torch.tensor([0.0, 0.5])
+    coefs[-1] = torch.tensor([1.0, 0.5])
+    for i in range(L + 2):
+        x = x + eps * coefs[i, 0] * p
+        _, g = T.value_and_grad(fn, x)
+        p = p + eps * coefs[i, 1] * g
+    return x, p
+
+
+def test_hmc_leapfrog_reversible_and_accepts_small_steps():
+    fn = lambda z: -0.5 * (z * z).sum(-1)
+    x0 = torch.randn(5, 2)
+    p0 = torch.randn(5, 2)
+    x1, p1 = _leapfrog_only(x0, p0, fn, 1.0, 5)
+    x2, p2 = _leapfrog_only(x1, -p1, fn, 1.0, 5)
+    assert torch.allclose(x2, x0, atol=1e-12) and torch.allclose(-p2, p0, atol=1e-12)
+    lp = fn(x0)
+    _, _, acc, _ = chains.hmc_step(x0, lp, fn, 1e-7, 5, torch.ones(2), torch.randn(5, 2), torch.log(torch.rand(5)))
+    assert bool(acc.all())
+
+
+def test_hmc_energy_error_is_second_order():
+    # harmonic oscillator: leapfrog H error scales as eps^2 (halving eps -> ~1/4)
+    fn = lambda z: -0.5 * (z * z).sum(-1)
+    x0 = torch.tensor([[1.0, -0.5]])
+    xi = torch.tensor([[0.3, 0.8]])
+    errs = []
+    for eps in (0.1, 0.05):
+        L = int(round(1.0 / eps))
+        _, _, _, log_acc = chains.hmc_step(x0, fn(x0), fn, eps, L, torch.ones(2), xi, torch.tensor([0.0]))
+        errs.append(abs(float(log_acc)))
+    assert 3.0 < errs[0] / errs[1] < 5.0
+
+
+def test_hmc_gradient_call_count():
+    calls = []
+
+    def fn(z):
+        calls.append(1)
+        return -0.5 * (z * z).sum(-1)
+
+    x = torch.randn(1, 3)
+    lp = fn(x)
+    calls.clear()
+    chains.hmc_step(x, lp, fn, 0.1, 10, torch.ones(3), torch.randn(1, 3), torch.tensor([0.0]))
+    # n_leapfrog + 2 gradient evaluations (HMC.py:91-95) + 1 potential (HMC.py:141)
+    assert len(calls) == 10 + 2 + 1
+
+
+def test_nf_mh_accepts_everything_when_target_is_the_flow():
+    C, S, d = 4, 7, 3
+    prop = torch.randn(C, S, d)
+    q = torch.randn(C, S)
+    x0 = torch.randn(C, d)
+    q0 = torch.randn(C)
+    pos, lps, acc = chains.nf_mh_scan(x0, q0, q0, prop, q, q, torch.log(torch.rand(C, S)))
+    assert bool(acc.all())
+    assert torch.equal(pos, prop)
+
+
+def test_take_serial_steps_thinning_rows():
+    fn = lambda z: -0.5 * (z * z).sum(-1)
+    S, C, d = 6, 3, 2
+    noise = torch.randn(S, C, d)
+    lu = torch.log(torch.rand(S, C))
+    pos, lps, accs, last, _ = chains.take_serial_steps("mala", torch.zeros(C, d), fn, noise, lu, thinning=4,
+                                                       step_size=0.5)
+    full, _, _, _, _ = chains.take_serial_steps("mala", torch.zeros(C, d), fn, noise, lu, thinning=1, step_size=0.5)
+    assert pos.shape == (C, 2, d)
+    assert torch.equal(pos[:, 1], full[:, 4]) and torch.equal(last, full[:, 4])
+
+
+def test_dynamic_update_slice_clamps():
+    buf = torch.zeros(2, 5)
+    out = chains.dynamic_update_slice(buf, torch.ones(2, 3), 4, 1)
+    assert torch.equal(out[:, 2:], torch.ones(2, 3)) and float(out[:, :2].abs().sum()) == 0
