@@ -81,6 +81,24 @@ class fmc_work_item(C.Structure):
     ]
 
 
+class fmc_flow(C.Structure):
+    _fields_ = [
+        ("n_dim", C.c_int32),
+        ("n_layers", C.c_int32),
+        ("n_bins", C.c_int32),
+        ("hidden1", C.c_int32),
+        ("hidden2", C.c_int32),
+        ("precision", C.c_int32),
+        ("range_min", C.c_float),
+        ("range_max", C.c_float),
+        ("min_bin_size", C.c_float),
+        ("min_knot_slope", C.c_float),
+        ("theta", C.c_void_p),
+        ("packed", C.c_void_p),
+        ("packed_bias", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -108,6 +126,12 @@ _SIGNATURES: dict[str, list] = {
     "fmc_philox_blocks": [P, U32, U32, U32, U32, U32, U32, I64, P],
     "fmc_rng_normal": [P, U32, U32, I64, I32, I32, I32, I32, P],
     "fmc_rng_log_uniform": [P, U32, U32, I64, I32, I32, I32, P],
+    "fmc_flow_sizes": [C.POINTER(fmc_flow), P, P, P, P, P, P],
+    "fmc_flow_pack": [P, C.POINTER(fmc_flow)],
+    "fmc_flow_pass": [P, C.POINTER(fmc_flow), I32, I32, P, I64, U32, U32, I64, P, P, P, P, P],
+    "fmc_flow_backward_layer": [P, C.POINTER(fmc_flow), I32, P, I64, P, P, P, P, I64, P, I32],
+    "fmc_flow_weight_grad": [P, C.POINTER(fmc_flow), I32, P, I64, P, I32, P],
+    "fmc_adamw_clip": [P, P, P, P, P, I64, F32, F32, F32, F32, F32, F32, I32, P, I32, P],
 }
 _RESTYPES = {"fmc_last_error": C.c_char_p}
 

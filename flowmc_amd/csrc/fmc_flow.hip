@@ -1,0 +1,1301 @@
+// MaskedCouplingRQSpline flow on gfx950: packed MFMA conditioner GEMMs with
+// the rational-quadratic spline fused as the epilogue of the last GEMM.
+//
+//   flow_pass_kernel      all layers of a forward / inverse pass for a tile of
+//                         32*MB samples (activations stay in LDS)
+//   flow_bwd_kernel       adjoint of one forward layer (recomputes conditioner
+//                         and spline bin parameters, hand-derived spline VJP),
+//                         stores feature-major activations / gradients
+//   flow_wgrad_kernel     weight gradients: batch reduction on MFMA
+//   flow_rowsum_kernel    bias gradients + ScalarAffine gradients
+//   flow_pack_kernel      master FP32 parameters -> MFMA operand images
+//   adamw_* kernels       optax clip_by_global_norm(1) + adamw
+//
+// Reference: src/flowMC/resource/model/nf_model/rqSpline.py (spline :20-239,
+// RQSpline :242-361, MaskedCouplingRQSpline :364-504), common.py (MLP
+// :68-124, MaskedCouplingLayer :127-168, ScalarAffine :211-240, Gaussian
+// :243-293), nf_model/base.py:98-125 (loss / train_step),
+// src/flowMC/resource/optimizer.py:19-22.
+#include "fmc_common.h"
+#include "fmc_flow_layout.h"
+
+namespace fmc {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// --------------------------------------------------------- MFMA traits ---
+// 32x32 output block per wavefront. A operand from LDS (row-major, stride
+// lda), B operand from a packed "B-fragment image" in global memory: block
+// (nb, kb) holds 64 lanes x EPL contiguous elements, lane l carrying column
+// nb*32 + l%32 and k = kb*KSTEP + (l/32)*EPL + e.
+// Output element v of lane l is row 8*(v/4) + 4*(l/32) + v%4, column l%32.
+struct PF32 {
+  using T = float;
+  static constexpr int EPL = 1, KSTEP = 2, PAD = 1;
+  static __device__ __forceinline__ float load_a(const T* As, int lda, int row0, int kb, int lane) {
+    return As[(row0 + (lane & 31)) * lda + kb * 2 + (lane >> 5)];
+  }
+  static __device__ __forceinline__ float load_b(const T* Bp, int KB, int nb, int kb, int lane) {
+    return Bp[(int64_t)(nb * KB + kb) * 64 + lane];
+  }
+  static __device__ __forceinline__ f32x16_t mma(float a, float b, f32x16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  // fragments read straight from a feature-major global store (rows x ld)
+  static __device__ __forceinline__ float load_g(const T* base, int64_t ld, int row0, int b, int lane) {
+    return base[(int64_t)(row0 + (lane & 31)) * ld + b + (lane >> 5)];
+  }
+  static __device__ __forceinline__ T cvt(float x) { return x; }
+  static __device__ __forceinline__ float f(T x) { return x; }
+  static __device__ __forceinline__ float act(float x) { return tanhf(x); }
+};
+
+struct PBF16 {
+  using T = __bf16;
+  static constexpr int EPL = 8, KSTEP = 16, PAD = 8;
+  static __device__ __forceinline__ bf16x8_t load_a(const T* As, int lda, int row0, int kb, int lane) {
+    return *reinterpret_cast<const bf16x8_t*>(As + (row0 + (lane & 31)) * lda + kb * 16 + (lane >> 5) * 8);
+  }
+  static __device__ __forceinline__ bf16x8_t load_b(const T* Bp, int KB, int nb, int kb, int lane) {
+    return *reinterpret_cast<const bf16x8_t*>(Bp + ((int64_t)(nb * KB + kb) * 64 + lane) * 8);
+  }
+  static __device__ __forceinline__ f32x16_t mma(bf16x8_t a, bf16x8_t b, f32x16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ bf16x8_t load_g(const T* base, int64_t ld, int row0, int b, int lane) {
+    return *reinterpret_cast<const bf16x8_t*>(base + (int64_t)(row0 + (lane & 31)) * ld + b + (lane >> 5) * 8);
+  }
+  static __device__ __forceinline__ T cvt(float x) { return (__bf16)x; }
+  static __device__ __forceinline__ float f(T x) { return (float)x; }
+  static __device__ __forceinline__ float act(float x) {
+    float e = __expf(2.f * x);
+    return 1.f - 2.f / (e + 1.f);
+  }
+};
+
+__device__ __forceinline__ int out_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+// One 32x32 block: acc += A[row0:row0+32, kb0*KSTEP : kb1*KSTEP] * B-image block column nb.
+template <class PT>
+__device__ __forceinline__ void mfma_block(const typename PT::T* As, int lda, int row0, const typename PT::T* Bp,
+                                           int KBtot, int nb, int kb0, int kb1, int a_kb0, f32x16_t& acc, int lane) {
+#pragma unroll 4
+  for (int kb = kb0; kb < kb1; ++kb) {
+    auto a = PT::load_a(As, lda, row0, kb - kb0 + a_kb0, lane);
+    auto b = PT::load_b(Bp, KBtot, nb, kb, lane);
+    acc = PT::mma(a, b, acc);
+  }
+}
+
+// --------------------------------------------------------------- flow ----
+struct FlowDev {
+  FlowDims dm;
+  const float* theta;
+  const void* packed;
+  const float* pbias;
+  float lo, hi, mb, ms, off;  // spline range, min bin size, min knot slope, softplus offset
+};
+
+__device__ __forceinline__ float softplus(float v) { return fmaxf(v, 0.f) + log1pf(expf(-fabsf(v))); }
+__device__ __forceinline__ float sigmoid(float v) { return 1.f / (1.f + expf(-v)); }
+
+// Rational-quadratic spline of one scalar given its 3K+1 unnormalised
+// parameters pp (widths | heights | slopes): rqSpline.py:20-128.
+__device__ void spline_fwd(const float* pp, const FlowDev& f, float x, float& y, float& ld) {
+  const int K = f.dm.K;
+  if (x <= f.lo) {
+    float d0 = softplus(pp[2 * K] + f.off) + f.ms;
+    y = (x - f.lo) * d0 + f.lo;
+    ld = logf(d0);
+    return;
+  }
+  if (x >= f.hi) {
+    float dK = softplus(pp[3 * K] + f.off) + f.ms;
+    y = (x - f.hi) * dK + f.hi;
+    ld = logf(dK);
+    return;
+  }
+  float mw = -INFINITY, mh = -INFINITY;
+  for (int i = 0; i < K; ++i) {
+    mw = fmaxf(mw, pp[i]);
+    mh = fmaxf(mh, pp[K + i]);
+  }
+  float sw = 0.f, sh = 0.f;
+  for (int i = 0; i < K; ++i) {
+    sw += expf(pp[i] - mw);
+    sh += expf(pp[K + i] - mh);
+  }
+  const float span = (f.hi - f.lo) - K * f.mb;
+  const float cw = span / sw, ch = span / sh;
+  float xk = f.lo, yk = f.lo, xk1 = f.hi, yk1 = f.hi;
+  int k = 0;
+  for (int i = 0; i < K; ++i) {
+    float xn = (i == K - 1) ? f.hi : xk + (expf(pp[i] - mw) * cw + f.mb);
+    float yn = (i == K - 1) ? f.hi : yk + (expf(pp[K + i] - mh) * ch + f.mb);
+    if (x < xn) {
+      k = i;
+      xk1 = xn;
+      yk1 = yn;
+      break;
+    }
+    xk = xn;
+    yk = yn;
+  }
+  float dk = softplus(pp[2 * K + k] + f.off) + f.ms;
+  float dk1 = softplus(pp[2 * K + k + 1] + f.off) + f.ms;
+  float w = xk1 - xk, hh = yk1 - yk, s = hh / w;
+  float z = fminf(fmaxf((x - xk) / w, 0.f), 1.f);
+  float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
+  float st = dk1 + dk - 2.f * s;
+  float num = hh * (s * z2 + dk * zz);
+  float den = s + st * zz;
+  y = yk + num / den;
+  ld = 2.f * logf(s) + logf(dk1 * z2 + 2.f * s * zz + dk * q) - 2.f * logf(den);
+}
+
+// Inverse spline: rqSpline.py:131-239 (stable quadratic root).
+__device__ void spline_inv(const float* pp, const FlowDev& f, float yv, float& x, float& ld) {
+  const int K = f.dm.K;
+  if (yv <= f.lo) {
+    float d0 = softplus(pp[2 * K] + f.off) + f.ms;
+    x = (yv - f.lo) / d0 + f.lo;
+    ld = -logf(d0);
+    return;
+  }
+  if (yv >= f.hi) {
+    float dK = softplus(pp[3 * K] + f.off) + f.ms;
+    x = (yv - f.hi) / dK + f.hi;
+    ld = -logf(dK);
+    return;
+  }
+  float mw = -INFINITY, mh = -INFINITY;
+  for (int i = 0; i < K; ++i) {
+    mw = fmaxf(mw, pp[i]);
+    mh = fmaxf(mh, pp[K + i]);
+  }
+  float sw = 0.f, sh = 0.f;
+  for (int i = 0; i < K; ++i) {
+    sw += expf(pp[i] - mw);
+    sh += expf(pp[K + i] - mh);
+  }
+  const float span = (f.hi - f.lo) - K * f.mb;
+  const float cw = span / sw, ch = span / sh;
+  float xk = f.lo, yk = f.lo, xk1 = f.hi, yk1 = f.hi;
+  int k = 0;
+  for (int i = 0; i < K; ++i) {
+    float xn = (i == K - 1) ? f.hi : xk + (expf(pp[i] - mw) * cw + f.mb);
+    float yn = (i == K - 1) ? f.hi : yk + (expf(pp[K + i] - mh) * ch + f.mb);
+    if (yv < yn) {
+      k = i;
+      xk1 = xn;
+      yk1 = yn;
+      break;
+    }
+    xk = xn;
+    yk = yn;
+  }
+  float dk = softplus(pp[2 * K + k] + f.off) + f.ms;
+  float dk1 = softplus(pp[2 * K + k + 1] + f.off) + f.ms;
+  float w = xk1 - xk, hh = yk1 - yk, s = hh / w;
+  float wv = fminf(fmaxf((yv - yk) / hh, 0.f), 1.f);
+  float st = dk1 + dk - 2.f * s;
+  float c = -s * wv;
+  float b = dk - st * wv;
+  float a = s - b;
+  float disc = b * b - 4.f * a * c;
+  float sq = disc > 0.f ? sqrtf(fmaxf(disc, 1.17549435e-38f)) : 0.f;
+  float z = (b >= 0.f) ? (2.f * c) / (-b - sq) : (-b + sq) / (2.f * a);
+  z = fminf(fmaxf(z, 0.f), 1.f);
+  x = w * z + xk;
+  float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
+  float den = s + st * zz;
+  ld = -2.f * logf(s) - logf(dk1 * z2 + 2.f * s * zz + dk * q) + 2.f * logf(den);
+}
+
+// Forward spline + vector-Jacobian product. Given upstream gy (dL/dy) and gl
+// (dL/dlogdet) writes dL/d(unnormalised params) to go[0..3K] and returns dL/dx.
+template <class T, class CV>
+__device__ float spline_vjp(const float* pp, const FlowDev& f, float x, float gy, float gl, T* go, CV cvt) {
+  const int K = f.dm.K;
+  for (int i = 0; i < 3 * K + 1; ++i) go[i] = cvt(0.f);
+  if (x <= f.lo || x >= f.hi) {
+    const int idx = (x <= f.lo) ? 2 * K : 3 * K;
+    const float edge = (x <= f.lo) ? f.lo : f.hi;
+    float u = pp[idx] + f.off;
+    float dd = softplus(u) + f.ms;
+    float gd = gy * (x - edge) + gl / dd;
+    go[idx] = cvt(gd * sigmoid(u));
+    return gy * dd;
+  }
+  float mw = -INFINITY, mh = -INFINITY;
+  for (int i = 0; i < K; ++i) {
+    mw = fmaxf(mw, pp[i]);
+    mh = fmaxf(mh, pp[K + i]);
+  }
+  float sw = 0.f, sh = 0.f;
+  for (int i = 0; i < K; ++i) {
+    sw += expf(pp[i] - mw);
+    sh += expf(pp[K + i] - mh);
+  }
+  const float span = (f.hi - f.lo) - K * f.mb;
+  const float cw = span / sw, ch = span / sh;
+  float xk = f.lo, yk = f.lo, xk1 = f.hi, yk1 = f.hi;
+  float pw_lt = 0.f, ph_lt = 0.f;  // sum of softmax probs of bins < k
+  float pwk = 0.f, phk = 0.f;
+  int k = 0;
+  for (int i = 0; i < K; ++i) {
+    float ew = expf(pp[i] - mw), eh = expf(pp[K + i] - mh);
+    float xn = (i == K - 1) ? f.hi : xk + (ew * cw + f.mb);
+    float yn = (i == K - 1) ? f.hi : yk + (eh * ch + f.mb);
+    if (x < xn) {
+      k = i;
+      xk1 = xn;
+      yk1 = yn;
+      pwk = ew / sw;
+      phk = eh / sh;
+      break;
+    }
+    pw_lt += ew / sw;
+    ph_lt += eh / sh;
+    xk = xn;
+    yk = yn;
+  }
+  const float uk = pp[2 * K + k] + f.off, uk1 = pp[2 * K + k + 1] + f.off;
+  const float dk = softplus(uk) + f.ms, dk1 = softplus(uk1) + f.ms;
+  const float w = xk1 - xk, hh = yk1 - yk, s = hh / w;
+  const float t = (x - xk) / w;
+  const float z = fminf(fmaxf(t, 0.f), 1.f);
+  const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
+  const float st = dk1 + dk - 2.f * s;
+  const float num = hh * (s * z2 + dk * zz);
+  const float den = s + st * zz;
+  const float A = dk1 * z2 + 2.f * s * zz + dk * q;
+  // reverse sweep of y = yk + num/den, ld = 2 log s + log A - 2 log den
+  float g_num = gy / den;
+  float g_den = -gy * num / (den * den) - 2.f * gl / den;
+  float g_s = 2.f * gl / s;
+  float g_A = gl / A;
+  float g_yk = gy;
+  float g_dk1 = g_A * z2, g_z2 = g_A * dk1, g_zz = g_A * 2.f * s, g_dk = g_A * q, g_q = g_A * dk;
+  g_s += g_A * 2.f * zz;
+  g_s += g_den;
+  float g_st = g_den * zz;
+  g_zz += g_den * st;
+  float g_hh = g_num * (s * z2 + dk * zz);
+  g_s += g_num * hh * z2;
+  g_z2 += g_num * hh * s;
+  g_dk += g_num * hh * zz;
+  g_zz += g_num * hh * dk;
+  g_dk1 += g_st;
+  g_dk += g_st;
+  g_s += -2.f * g_st;
+  g_z2 -= g_zz;
+  float g_z = g_zz + g_z2 * 2.f * z - g_q * 2.f * (1.f - z);
+  float g_t = (t > 0.f && t < 1.f) ? g_z : 0.f;
+  float g_x = g_t / w;
+  float g_xk = -g_t / w;
+  float g_w = -g_t * t / w;
+  g_hh += g_s / w;
+  g_w += -g_s * s / w;
+  float g_yk1 = g_hh;
+  g_yk -= g_hh;
+  float g_xk1 = g_w;
+  g_xk -= g_w;
+  // knots -> softmax logits: x_i = lo + sum_{m<i} w_m for 1 <= i <= K-1
+  const float G1x = (k >= 1) ? g_xk : 0.f, G2x = (k + 1 <= K - 1) ? g_xk1 : 0.f;
+  const float G1y = (k >= 1) ? g_yk : 0.f, G2y = (k + 1 <= K - 1) ? g_yk1 : 0.f;
+  const float dotw = span * (G1x * pw_lt + G2x * (pw_lt + pwk));
+  const float doth = span * (G1y * ph_lt + G2y * (ph_lt + phk));
+  for (int m = 0; m < K; ++m) {
+    float gwm = span * ((m < k ? G1x : 0.f) + (m <= k ? G2x : 0.f));
+    float ghm = span * ((m < k ? G1y : 0.f) + (m <= k ? G2y : 0.f));
+    float pw = expf(pp[m] - mw) / sw, ph = expf(pp[K + m] - mh) / sh;
+    go[m] = cvt(pw * (gwm - dotw));
+    go[K + m] = cvt(ph * (ghm - doth));
+  }
+  go[2 * K + k] = cvt(g_dk * sigmoid(uk));
+  go[2 * K + k + 1] = cvt(g_dk1 * sigmoid(uk1));
+  return g_x;
+}
+
+// ------------------------------------------------------- pass kernel -----
+struct PassArgs {
+  int dir;    // 0 forward, 1 inverse (reference layer order), 2 exact inverse
+  int flags;  // 1 standardize, 2 base log-prob, 4 sample base noise
+  const float* x_in;
+  int64_t n;
+  uint32_t key0, key1;
+  int64_t sample0;
+  float* x_out;
+  float* logdet;
+  float* saved;
+  const float* g_logp;
+  float* gy_out;
+};
+
+enum { FL_STD = 1, FL_BASE = 2, FL_SAMPLE = 4 };
+
+template <class PT>
+struct PassSmem {
+  // byte offsets inside dynamic LDS
+  int xs, ldr, ldp, r1, r2, total;
+  int ldx, lh1, lh2, lo_, la0;
+  __host__ __device__ PassSmem(const FlowDims& dm, int MT) {
+    ldx = dm.d + 1;
+    lh1 = dm.h1p() + PT::PAD;
+    lh2 = dm.h2p() + PT::PAD;
+    lo_ = dm.ncf() + 1;
+    la0 = rup(dm.kc(1) > dm.kc(0) ? dm.kc(1) : dm.kc(0), PT::KSTEP) + PT::PAD;
+    int o = 0;
+    xs = o;
+    o += rup(MT * ldx * 4, 16);
+    ldr = o;
+    o += rup(MT * 4, 16);
+    ldp = o;
+    o += rup(MT * dm.G() * 4, 16);
+    r1 = o;
+    int s1 = MT * lh1 * (int)sizeof(typename PT::T), s1b = MT * lo_ * 4;
+    o += rup(s1 > s1b ? s1 : s1b, 16);
+    r2 = o;
+    int s2 = MT * lh2 * (int)sizeof(typename PT::T), s2b = MT * la0 * (int)sizeof(typename PT::T);
+    o += rup(s2 > s2b ? s2 : s2b, 16);
+    total = o;
+  }
+};
+
+template <class PT, int MB>
+__global__ void __launch_bounds__(256) flow_pass_kernel(FlowDev f, PassArgs a) {
+  using T = typename PT::T;
+  constexpr int MT = 32 * MB;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const FlowDims& dm = f.dm;
+  PassSmem<PT> sm(dm, MT);
+  float* xs = reinterpret_cast<float*>(smem + sm.xs);
+  float* ldr = reinterpret_cast<float*>(smem + sm.ldr);
+  float* ldp = reinterpret_cast<float*>(smem + sm.ldp);
+  T* hb1 = reinterpret_cast<T*>(smem + sm.r1);
+  float* obuf = reinterpret_cast<float*>(smem + sm.r1);
+  T* hb2 = reinterpret_cast<T*>(smem + sm.r2);
+  T* a0 = reinterpret_cast<T*>(smem + sm.r2);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int d = dm.d, ldx = sm.ldx;
+  const int64_t m0 = (int64_t)blockIdx.x * MT;
+  const float* th = f.theta;
+  const T* pk = reinterpret_cast<const T*>(f.packed);
+
+  // ---- load tile (+ standardisation / base sampling)
+  for (int i = tid; i < MT * d; i += 256) {
+    int m = i / d, j = i - m * d;
+    int64_t row = m0 + m;
+    float v = 0.f;
+    if (row < a.n) {
+      if ((a.flags & FL_SAMPLE) && a.x_in == nullptr) {
+        float4 z = philox_normal4(a.key0, a.key1, (uint32_t)(j >> 2), (uint32_t)(a.sample0 + row), 0u, RNG_FLOW_BASE);
+        v = f4c(z, j & 3);
+      } else {
+        v = a.x_in[row * d + j];
+      }
+      if (a.dir == 0 && (a.flags & FL_STD)) {
+        v = (v - th[dm.th_data_mean() + j]) / sqrtf(th[dm.th_data_cov() + (int64_t)j * d + j]);
+      } else if (a.dir != 0 && (a.flags & FL_SAMPLE)) {
+        v = th[dm.th_base_mean() + j] + sqrtf(th[dm.th_base_cov() + (int64_t)j * d + j]) * v;
+      }
+    }
+    xs[m * ldx + j] = v;
+  }
+  for (int m = tid; m < MT; m += 256) ldr[m] = 0.f;
+  __syncthreads();
+
+  for (int step = 0; step < dm.L; ++step) {
+    const int l = (a.dir == 0) ? step : dm.L - 1 - step;
+    if (a.saved) {
+      for (int i = tid; i < MT * d; i += 256) {
+        int m = i / d, j = i - m * d;
+        if (m0 + m < a.n) a.saved[((int64_t)l * a.n + m0 + m) * d + j] = xs[m * ldx + j];
+      }
+    }
+    const float sc = th[dm.th_scale(l)], sh = th[dm.th_shift(l)];
+    const bool affine_first = (a.dir != 2);
+    // ---- ScalarAffine (forward / reference inverse apply it before the spline)
+    if (affine_first) {
+      const float e = expf(a.dir == 0 ? sc : -sc);
+      for (int i = tid; i < MT * d; i += 256) {
+        int m = i / d, j = i - m * d;
+        float v = xs[m * ldx + j];
+        xs[m * ldx + j] = (a.dir == 0) ? (v + sh) * e : v * e - sh;
+      }
+      for (int m = tid; m < MT; m += 256) ldr[m] += (a.dir == 0 ? sc : -sc) * (float)d;
+      __syncthreads();
+    }
+    // ---- conditioner input (conditioning dims only)
+    const int dc = dm.dc(l), kc = dm.kc(l);
+    for (int i = tid; i < MT * kc; i += 256) {
+      int m = i / kc, k = i - m * kc;
+      a0[m * sm.la0 + k] = PT::cvt(k < dc ? xs[m * ldx + dm.cond(l, k)] : 0.f);
+    }
+    __syncthreads();
+    const T* W = pk + dm.pk_layer(l);
+    const float* bl = f.pbias + dm.pb_layer(l);
+    const int h1p = dm.h1p(), h2p = dm.h2p();
+    // ---- GEMM1: h1 = tanh(W1c c + b1)
+    {
+      const int KB = kc / PT::KSTEP, NBt = h1p / 32;
+      for (int t = wave; t < MB * NBt; t += 4) {
+        int rb = t % MB, nb = t / MB;
+        f32x16_t acc = {};
+        mfma_block<PT>(a0, sm.la0, rb * 32, W + dm.pk_w1(l), KB, nb, 0, KB, 0, acc, lane);
+        const int col = nb * 32 + (lane & 31);
+        const float bb = bl[col];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) hb1[(rb * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(PT::act(acc[v] + bb));
+      }
+    }
+    __syncthreads();
+    // ---- GEMM2: h2 = tanh(W2 h1 + b2)
+    {
+      const int KB = h1p / PT::KSTEP, NBt = h2p / 32;
+      for (int t = wave; t < MB * NBt; t += 4) {
+        int rb = t % MB, nb = t / MB;
+        f32x16_t acc = {};
+        mfma_block<PT>(hb1, sm.lh1, rb * 32, W + dm.pk_w2(l), KB, nb, 0, KB, 0, acc, lane);
+        const int col = nb * 32 + (lane & 31);
+        const float bb = bl[h1p + col];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) hb2[(rb * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(PT::act(acc[v] + bb));
+      }
+    }
+    __syncthreads();
+    // ---- GEMM3 in chunks of G transformed dims + spline epilogue
+    const int nch = dm.n_chunks(l), P = dm.P();
+    for (int c = 0; c < nch; ++c) {
+      const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
+      const int KB = h2p / PT::KSTEP, NBt = nc / 32;
+      for (int t = wave; t < MB * NBt; t += 4) {
+        int rb = t % MB, nbl = t / MB;
+        f32x16_t acc = {};
+        mfma_block<PT>(hb2, sm.lh2, rb * 32, W + dm.pk_w3(l), KB, coff / 32 + nbl, 0, KB, 0, acc, lane);
+        const int col = nbl * 32 + (lane & 31);
+        const float bb = bl[h1p + h2p + coff + col];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) obuf[(rb * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[v] + bb;
+      }
+      __syncthreads();
+      for (int i = tid; i < MT * cnt; i += 256) {
+        int m = i % MT, jj = i / MT;
+        int j = dm.trans(l, c * dm.G() + jj);
+        const float* pp = obuf + m * sm.lo_ + jj * P;
+        float xv = xs[m * ldx + j], yv, ldv;
+        if (a.dir == 0)
+          spline_fwd(pp, f, xv, yv, ldv);
+        else
+          spline_inv(pp, f, xv, yv, ldv);
+        xs[m * ldx + j] = yv;
+        ldp[m * dm.G() + jj] = ldv;
+      }
+      __syncthreads();
+      for (int m = tid; m < MT; m += 256) {
+        float s = ldr[m];
+        for (int jj = 0; jj < cnt; ++jj) s += ldp[m * dm.G() + jj];
+        ldr[m] = s;
+      }
+      __syncthreads();
+    }
+    if (!affine_first) {  // exact inverse: affine^-1 after spline^-1
+      const float e = expf(-sc);
+      for (int i = tid; i < MT * d; i += 256) {
+        int m = i / d, j = i - m * d;
+        xs[m * ldx + j] = xs[m * ldx + j] * e - sh;
+      }
+      for (int m = tid; m < MT; m += 256) ldr[m] += -sc * (float)d;
+      __syncthreads();
+    }
+  }
+  // ---- outputs
+  for (int m = tid; m < MT; m += 256) {
+    int64_t row = m0 + m;
+    if (row >= a.n) continue;
+    float ld = ldr[m];
+    if (a.dir == 0 && (a.flags & FL_BASE)) {
+      // log N(y; base_mean, diag(base_cov)) (common.py:285-286 with a diagonal covariance)
+      float q = 0.f, lg = 0.f;
+      for (int j = 0; j < d; ++j) {
+        float var = th[dm.th_base_cov() + (int64_t)j * d + j];
+        float u = xs[m * ldx + j] - th[dm.th_base_mean() + j];
+        q += u * u / var;
+        lg += logf(var);
+        if (a.gy_out) a.gy_out[row * d + j] = a.g_logp[row] * (-u / var);
+      }
+      ld += -0.5f * q - 0.5f * lg - 0.91893853320467274f * (float)d;
+    }
+    if (a.logdet) a.logdet[row] = ld;
+  }
+  if (a.x_out) {
+    for (int i = tid; i < MT * d; i += 256) {
+      int m = i / d, j = i - m * d;
+      int64_t row = m0 + m;
+      if (row >= a.n) continue;
+      float v = xs[m * ldx + j];
+      if (a.dir != 0 && (a.flags & FL_STD))
+        v = v * sqrtf(th[dm.th_data_cov() + (int64_t)j * d + j]) + th[dm.th_data_mean() + j];
+      a.x_out[row * d + j] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------ backward kernel --
+struct BwdArgs {
+  int layer;
+  const float* x_in;     // [n, d] input of this layer (after standardisation)
+  int64_t n;
+  const float* g_logp;   // [n]
+  const float* gy;       // [n, d] dL/d(layer output)
+  float* gx;             // [n, d] dL/d(layer input)
+  void* store;           // feature-major activations [st_rows, ld]
+  int64_t ld;
+  float* aff_part;       // [n_blocks, 2]
+};
+
+template <class PT>
+struct BwdSmem {
+  int xs, gb, glp, hb1, hb2, ob, gpb, red, total;
+  int ldx, lh1, lh2, lo_, lg;
+  __host__ __device__ BwdSmem(const FlowDims& dm, int MT) {
+    ldx = dm.d + 1;
+    lh1 = dm.h1p() + PT::PAD;
+    lh2 = dm.h2p() + PT::PAD;
+    lo_ = dm.ncf() + 1;
+    int kcm = dm.kc(1) > dm.kc(0) ? dm.kc(1) : dm.kc(0);
+    lg = (dm.ncf() > kcm ? dm.ncf() : kcm) + PT::PAD;
+    const int tb = (int)sizeof(typename PT::T);
+    int o = 0;
+    xs = o;
+    o += rup(MT * ldx * 4, 16);
+    gb = o;
+    o += rup(MT * ldx * 4, 16);
+    glp = o;
+    o += rup(MT * 4, 16);
+    hb1 = o;
+    o += rup(MT * lh1 * tb, 16);
+    hb2 = o;
+    o += rup(MT * lh2 * tb, 16);
+    ob = o;
+    o += rup(MT * lo_ * 4, 16);
+    gpb = o;
+    o += rup(MT * lg * tb, 16);
+    red = o;
+    o += 64;
+    total = o;
+  }
+};
+
+// Feature-major store: dst[(row0 + f) * ld + m0 + m] = src[m * lds + f].
+template <class T>
+__device__ __forceinline__ void store_fm(T* st, int64_t ld, int row0, int nf, const T* src, int lds, int64_t m0,
+                                         int MT) {
+  for (int i = threadIdx.x; i < nf * MT; i += 256) {
+    int fidx = i / MT, m = i - fidx * MT;
+    st[(int64_t)(row0 + fidx) * ld + m0 + m] = src[m * lds + fidx];
+  }
+}
+
+template <class PT, int MB, int HMAX>
+__global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
+  using T = typename PT::T;
+  constexpr int MT = 32 * MB;
+  // 32x32 tiles per wave of an [MT x hidden] block (hidden <= HMAX); the same
+  // wave owns the same tiles in the forward recompute and in the adjoint, so
+  // the FP32 tanh derivatives 1 - h^2 stay in its registers (the LDS copy of
+  // h may be BF16, whose rounding would wreck 1 - h^2 near saturation).
+  constexpr int NTW = MB * (HMAX / 32) / 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const FlowDims& dm = f.dm;
+  BwdSmem<PT> sm(dm, MT);
+  float* xs = reinterpret_cast<float*>(smem + sm.xs);
+  float* gb = reinterpret_cast<float*>(smem + sm.gb);
+  float* glp = reinterpret_cast<float*>(smem + sm.glp);
+  T* hb1 = reinterpret_cast<T*>(smem + sm.hb1);
+  T* hb2 = reinterpret_cast<T*>(smem + sm.hb2);
+  float* obuf = reinterpret_cast<float*>(smem + sm.ob);
+  T* gpb = reinterpret_cast<T*>(smem + sm.gpb);
+  float* red = reinterpret_cast<float*>(smem + sm.red);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int d = dm.d, ldx = sm.ldx, l = a.layer;
+  const int64_t m0 = (int64_t)blockIdx.x * MT;
+  const float* th = f.theta;
+  const T* W = reinterpret_cast<const T*>(f.packed) + dm.pk_layer(l);
+  const float* bl = f.pbias + dm.pb_layer(l);
+  T* st = reinterpret_cast<T*>(a.store);
+  const int h1p = dm.h1p(), h2p = dm.h2p(), P = dm.P();
+  const float sc = th[dm.th_scale(l)], sh = th[dm.th_shift(l)];
+  const float es = expf(sc);
+
+  // ---- load x (layer input), gy, g_logp; apply the affine part
+  for (int i = tid; i < MT * d; i += 256) {
+    int m = i / d, j = i - m * d;
+    int64_t row = m0 + m;
+    bool ok = row < a.n;
+    xs[m * ldx + j] = ok ? (a.x_in[row * d + j] + sh) * es : 0.f;
+    gb[m * ldx + j] = ok ? a.gy[row * d + j] : 0.f;
+  }
+  for (int m = tid; m < MT; m += 256) glp[m] = (m0 + m < a.n) ? a.g_logp[m0 + m] : 0.f;
+  __syncthreads();
+  // ---- recompute conditioner forward
+  const int dc = dm.dc(l), kc = dm.kc(l);
+  T* a0 = gpb;
+  const int la0 = sm.lg;
+  for (int i = tid; i < MT * kc; i += 256) {
+    int m = i / kc, k = i - m * kc;
+    a0[m * la0 + k] = PT::cvt(k < dc ? xs[m * ldx + dm.cond(l, k)] : 0.f);
+  }
+  __syncthreads();
+  store_fm<T>(st, a.ld, (int)dm.st_c(l), kc, a0, la0, m0, MT);
+  f32x16_t dh1[NTW], dh2[NTW];
+  {
+    const int KB = kc / PT::KSTEP, NBt = h1p / 32;
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+      const int t = wave + 4 * i;
+      if (t < MB * NBt) {
+        int rb = t % MB, nb = t / MB;
+        f32x16_t acc = {};
+        mfma_block<PT>(a0, la0, rb * 32, W + dm.pk_w1(l), KB, nb, 0, KB, 0, acc, lane);
+        const int col = nb * 32 + (lane & 31);
+        const float bb = bl[col];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          float h = PT::act(acc[v] + bb);
+          hb1[(rb * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(h);
+          dh1[i][v] = 1.f - h * h;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  store_fm<T>(st, a.ld, (int)dm.st_h1(l), h1p, hb1, sm.lh1, m0, MT);
+  {
+    const int KB = h1p / PT::KSTEP, NBt = h2p / 32;
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+      const int t = wave + 4 * i;
+      if (t < MB * NBt) {
+        int rb = t % MB, nb = t / MB;
+        f32x16_t acc = {};
+        mfma_block<PT>(hb1, sm.lh1, rb * 32, W + dm.pk_w2(l), KB, nb, 0, KB, 0, acc, lane);
+        const int col = nb * 32 + (lane & 31);
+        const float bb = bl[h1p + col];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          float h = PT::act(acc[v] + bb);
+          hb2[(rb * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(h);
+          dh2[i][v] = 1.f - h * h;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  store_fm<T>(st, a.ld, (int)dm.st_h2(l), h2p, hb2, sm.lh2, m0, MT);
+
+  // ---- chunks: params -> spline VJP -> gp; g_h2 += gp W3c (register accumulators)
+  const int ntiles_h2 = MB * (h2p / 32);
+  f32x16_t gh2[NTW];
+#pragma unroll
+  for (int i = 0; i < NTW; ++i) gh2[i] = f32x16_t{};
+  const int nch = dm.n_chunks(l);
+  const int KB3 = h2p / PT::KSTEP;
+  const int KB3t = dm.n3p(l) / PT::KSTEP;
+  for (int c = 0; c < nch; ++c) {
+    const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
+    const int NBt = nc / 32;
+    for (int t = wave; t < MB * NBt; t += 4) {
+      int rb = t % MB, nbl = t / MB;
+      f32x16_t acc = {};
+      mfma_block<PT>(hb2, sm.lh2, rb * 32, W + dm.pk_w3(l), KB3, coff / 32 + nbl, 0, KB3, 0, acc, lane);
+      const int col = nbl * 32 + (lane & 31);
+      const float bb = bl[h1p + h2p + coff + col];
+#pragma unroll
+      for (int v = 0; v < 16; ++v) obuf[(rb * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[v] + bb;
+    }
+    __syncthreads();
+    for (int i = tid; i < MT * cnt; i += 256) {
+      int m = i % MT, jj = i / MT;
+      int j = dm.trans(l, c * dm.G() + jj);
+      const float* pp = obuf + m * sm.lo_ + jj * P;
+      float gyv = gb[m * ldx + j];
+      float gxv = spline_vjp<T>(pp, f, xs[m * ldx + j], gyv, glp[m], gpb + m * sm.lg + jj * P,
+                                [](float v) { return PT::cvt(v); });
+      gb[m * ldx + j] = gxv;
+    }
+    for (int i = tid; i < MT * (nc - cnt * P); i += 256) {
+      int w = nc - cnt * P;
+      int m = i / w, cc = cnt * P + (i - m * w);
+      gpb[m * sm.lg + cc] = PT::cvt(0.f);
+    }
+    __syncthreads();
+    store_fm<T>(st, a.ld, (int)dm.st_gp(l) + coff, nc, gpb, sm.lg, m0, MT);
+    const int kb0 = coff / PT::KSTEP, kb1 = (coff + nc) / PT::KSTEP;
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+      int t = wave + 4 * i;
+      if (t < ntiles_h2) {
+        int rb = t % MB, nb = t / MB;
+        mfma_block<PT>(gpb, sm.lg, rb * 32, W + dm.pk_w3t(l), KB3t, nb, kb0, kb1, 0, gh2[i], lane);
+      }
+    }
+    __syncthreads();
+  }
+  // ---- g_pre2 = g_h2 * (1 - h2^2)   (in place of h2)
+#pragma unroll
+  for (int i = 0; i < NTW; ++i) {
+    int t = wave + 4 * i;
+    if (t < ntiles_h2) {
+      int rb = t % MB, nb = t / MB;
+      const int col = nb * 32 + (lane & 31);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        int r = rb * 32 + out_row(v, lane);
+        hb2[r * sm.lh2 + col] = PT::cvt(gh2[i][v] * dh2[i][v]);
+      }
+    }
+  }
+  __syncthreads();
+  store_fm<T>(st, a.ld, (int)dm.st_gp2(l), h2p, hb2, sm.lh2, m0, MT);
+  // ---- g_pre1 = (g_pre2 W2) * (1 - h1^2)   (in place of h1)
+  {
+    const int KB = h2p / PT::KSTEP, NBt = h1p / 32;
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+      const int t = wave + 4 * i;
+      if (t < MB * NBt) {
+        int rb = t % MB, nb = t / MB;
+        f32x16_t acc = {};
+        mfma_block<PT>(hb2, sm.lh2, rb * 32, W + dm.pk_w2t(l), KB, nb, 0, KB, 0, acc, lane);
+        const int col = nb * 32 + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          int r = rb * 32 + out_row(v, lane);
+          hb1[r * sm.lh1 + col] = PT::cvt(acc[v] * dh1[i][v]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  store_fm<T>(st, a.ld, (int)dm.st_gp1(l), h1p, hb1, sm.lh1, m0, MT);
+  // ---- g_c = g_pre1 W1c, scattered onto the conditioning dims
+  {
+    const int KB = h1p / PT::KSTEP, NBt = dm.ncp(l) / 32;
+    for (int t = wave; t < MB * NBt; t += 4) {
+      int rb = t % MB, nb = t / MB;
+      f32x16_t acc = {};
+      mfma_block<PT>(hb1, sm.lh1, rb * 32, W + dm.pk_w1t(l), KB, nb, 0, KB, 0, acc, lane);
+      const int k = nb * 32 + (lane & 31);
+      if (k < dc) {
+        const int j = dm.cond(l, k);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) gb[(rb * 32 + out_row(v, lane)) * ldx + j] += acc[v];
+      }
+    }
+  }
+  __syncthreads();
+  // ---- affine backward: xa = (x + shift) e^s, ld += d*s
+  float ps = 0.f, pt = 0.f;
+  for (int i = tid; i < MT * d; i += 256) {
+    int m = i / d, j = i - m * d;
+    float g = gb[m * ldx + j];
+    ps += g * xs[m * ldx + j];
+    pt += g * es;
+    int64_t row = m0 + m;
+    if (row < a.n) a.gx[row * d + j] = g * es;
+  }
+  for (int m = tid; m < MT; m += 256) ps += (float)d * glp[m];
+  for (int o = 32; o > 0; o >>= 1) {
+    ps += __shfl_xor(ps, o, 64);
+    pt += __shfl_xor(pt, o, 64);
+  }
+  if (lane == 0) {
+    red[2 * wave] = ps;
+    red[2 * wave + 1] = pt;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    float s0 = 0.f, s1 = 0.f;
+    for (int w = 0; w < 4; ++w) {
+      s0 += red[2 * w];
+      s1 += red[2 * w + 1];
+    }
+    a.aff_part[2 * blockIdx.x] = s0;
+    a.aff_part[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// --------------------------------------------------- weight gradients ----
+// One wavefront per 32x32 tile of gW = G^T Act over the batch (both operands
+// feature-major in the store). Problems: 0 = W1c, 1 = W2, 2 = W3 (chunked).
+template <class PT>
+__global__ void __launch_bounds__(256) flow_wgrad_kernel(FlowDev f, int l, const void* store, int64_t ld,
+                                                         float* grad) {
+  using T = typename PT::T;
+  const FlowDims& dm = f.dm;
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int h1p = dm.h1p(), h2p = dm.h2p();
+  const int n0t = (h1p / 32) * (dm.ncp(l) / 32), n1t = (h2p / 32) * (h1p / 32), n2t = (dm.n3p(l) / 32) * (h2p / 32);
+  int prob, tt = tile;
+  if (tt < n0t) {
+    prob = 0;
+  } else if ((tt -= n0t) < n1t) {
+    prob = 1;
+  } else if ((tt -= n1t) < n2t) {
+    prob = 2;
+  } else {
+    return;
+  }
+  int gRow, aRow, ncolb;
+  if (prob == 0) {
+    gRow = (int)dm.st_gp1(l);
+    aRow = (int)dm.st_c(l);
+    ncolb = dm.ncp(l) / 32;
+  } else if (prob == 1) {
+    gRow = (int)dm.st_gp2(l);
+    aRow = (int)dm.st_h1(l);
+    ncolb = h1p / 32;
+  } else {
+    gRow = (int)dm.st_gp(l);
+    aRow = (int)dm.st_h2(l);
+    ncolb = h2p / 32;
+  }
+  const int nb = tt / ncolb, kb = tt - nb * ncolb;
+  const T* st = reinterpret_cast<const T*>(store);
+  f32x16_t acc = {};
+  for (int b = 0; b < ld; b += PT::KSTEP) {
+    auto ga = PT::load_g(st, ld, gRow + nb * 32, b, lane);
+    auto ab = PT::load_g(st, ld, aRow + kb * 32, b, lane);
+    acc = PT::mma(ga, ab, acc);
+  }
+  const int k = kb * 32 + (lane & 31);
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int n = nb * 32 + out_row(v, lane);
+    int64_t idx = -1;
+    if (prob == 0) {
+      if (n < dm.h1 && k < dm.dc(l)) idx = dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k);
+    } else if (prob == 1) {
+      if (n < dm.h2 && k < dm.h1) idx = dm.th_w2(l) + (int64_t)n * dm.h1 + k;
+    } else {
+      int row = dm.w3_row(l, n);
+      if (row >= 0 && k < dm.h2) idx = dm.th_w3(l) + (int64_t)row * dm.h2 + k;
+    }
+    if (idx >= 0) grad[idx] = acc[v];
+  }
+}
+
+// Bias gradients (row sums of the stored pre-activation gradients) and the
+// ScalarAffine gradients (fixed-order sum of per-block partials).
+template <class PT>
+__global__ void __launch_bounds__(256) flow_rowsum_kernel(FlowDev f, int l, const void* store, int64_t ld,
+                                                          const float* aff_part, int n_part, float* grad) {
+  using T = typename PT::T;
+  const FlowDims& dm = f.dm;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int h1p = dm.h1p(), h2p = dm.h2p(), n3p = dm.n3p(l);
+  const T* st = reinterpret_cast<const T*>(store);
+  if (r == h1p + h2p + n3p) {
+    if (lane == 0) {
+      float s0 = 0.f, s1 = 0.f;
+      for (int i = 0; i < n_part; ++i) {
+        s0 += aff_part[2 * i];
+        s1 += aff_part[2 * i + 1];
+      }
+      grad[dm.th_scale(l)] = s0;
+      grad[dm.th_shift(l)] = s1;
+    }
+    return;
+  }
+  if (r > h1p + h2p + n3p) return;
+  int srow;
+  int64_t idx = -1;
+  if (r < h1p) {
+    srow = (int)dm.st_gp1(l) + r;
+    if (r < dm.h1) idx = dm.th_b1(l) + r;
+  } else if (r < h1p + h2p) {
+    int q = r - h1p;
+    srow = (int)dm.st_gp2(l) + q;
+    if (q < dm.h2) idx = dm.th_b2(l) + q;
+  } else {
+    int q = r - h1p - h2p;
+    srow = (int)dm.st_gp(l) + q;
+    int row = dm.w3_row(l, q);
+    if (row >= 0) idx = dm.th_b3(l) + row;
+  }
+  float s = 0.f;
+  for (int64_t b = lane; b < ld; b += 64) s += PT::f(st[(int64_t)srow * ld + b]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0 && idx >= 0) grad[idx] = s;
+}
+
+// ------------------------------------------------------------- packing ---
+template <class PT>
+__global__ void flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias) {
+  const FlowDims& dm = f.dm;
+  const int64_t tot = dm.pk_total(), btot = dm.pb_total();
+  const float* th = f.theta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot + btot;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < tot) {
+      int l = 0;
+      while (l + 1 < dm.L && dm.pk_layer(l + 1) <= i) ++l;
+      int64_t e = i - dm.pk_layer(l);
+      int which = 0;
+      int64_t base = 0;
+      const int64_t offs[6] = {dm.pk_w1(l), dm.pk_w2(l), dm.pk_w3(l), dm.pk_w1t(l), dm.pk_w2t(l), dm.pk_w3t(l)};
+      for (int q = 5; q >= 0; --q)
+        if (e >= offs[q]) {
+          which = q;
+          base = offs[q];
+          break;
+        }
+      int64_t idx = e - base;
+      int Kdim;
+      switch (which) {
+        case 0: Kdim = dm.kc(l); break;
+        case 1: Kdim = dm.h1p(); break;
+        case 2: Kdim = dm.h2p(); break;
+        case 3: Kdim = dm.h1p(); break;
+        case 4: Kdim = dm.h2p(); break;
+        default: Kdim = dm.n3p(l); break;
+      }
+      const int KB = Kdim / PT::KSTEP;
+      int ee = (int)(idx % PT::EPL);
+      int lane = (int)((idx / PT::EPL) % 64);
+      int64_t blk = idx / (64 * PT::EPL);
+      int kb = (int)(blk % KB), nb = (int)(blk / KB);
+      int n = nb * 32 + (lane & 31);
+      int k = kb * PT::KSTEP + (lane >> 5) * PT::EPL + ee;
+      float v = 0.f;
+      switch (which) {
+        case 0:
+          if (n < dm.h1 && k < dm.dc(l)) v = th[dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k)];
+          break;
+        case 1:
+          if (n < dm.h2 && k < dm.h1) v = th[dm.th_w2(l) + (int64_t)n * dm.h1 + k];
+          break;
+        case 2: {
+          int row = dm.w3_row(l, n);
+          if (row >= 0 && k < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + k];
+        } break;
+        case 3:
+          if (n < dm.dc(l) && k < dm.h1) v = th[dm.th_w1(l) + (int64_t)k * dm.d + dm.cond(l, n)];
+          break;
+        case 4:
+          if (n < dm.h1 && k < dm.h2) v = th[dm.th_w2(l) + (int64_t)k * dm.h1 + n];
+          break;
+        default: {
+          int row = dm.w3_row(l, k);
+          if (row >= 0 && n < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + n];
+        } break;
+      }
+      packed[i] = PT::cvt(v);
+    } else {
+      int64_t bi = i - tot;
+      int l = 0;
+      while (l + 1 < dm.L && dm.pb_layer(l + 1) <= bi) ++l;
+      int64_t e = bi - dm.pb_layer(l);
+      const int h1p = dm.h1p(), h2p = dm.h2p();
+      float v = 0.f;
+      if (e < h1p) {
+        if (e < dm.h1) v = th[dm.th_b1(l) + e];
+      } else if (e < h1p + h2p) {
+        int q = (int)(e - h1p);
+        if (q < dm.h2) v = th[dm.th_b2(l) + q];
+      } else {
+        int row = dm.w3_row(l, (int)(e - h1p - h2p));
+        if (row >= 0) v = th[dm.th_b3(l) + row];
+      }
+      pbias[bi] = v;
+    }
+  }
+}
+
+// --------------------------------------------------------------- AdamW ---
+__global__ void __launch_bounds__(256) sumsq_kernel(const float* __restrict__ g, int64_t n, double* part) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double v = g[i];
+    s += v * v;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double r[4];
+  if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = r[0] + r[1] + r[2] + r[3];
+}
+
+struct AdamHyper {
+  float lr, b1, b2, eps, wd, max_norm;
+  float bc1, bc2;  // 1 - b1^t, 1 - b2^t
+};
+
+// optax.chain(clip_by_global_norm(max_norm), adamw(lr, b1, b2, eps, wd)):
+// u = g (or g / |g| * max_norm when |g| >= max_norm); m, v moments;
+// theta += -lr * (m_hat / (sqrt(v_hat) + eps) + wd * theta)
+__global__ void __launch_bounds__(256) adamw_kernel(float* theta, const float* __restrict__ g, float* m, float* v,
+                                                    int64_t n, AdamHyper h, const double* part, int n_part,
+                                                    float* norm_out) {
+  __shared__ float s_norm;
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < n_part; ++i) t += part[i];
+    s_norm = (float)sqrt(t);
+    if (blockIdx.x == 0 && norm_out) *norm_out = s_norm;
+  }
+  __syncthreads();
+  const float gn = s_norm;
+  const bool clip = !(gn < h.max_norm);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    float u = g[i];
+    if (clip) u = (u / gn) * h.max_norm;
+    float mi = (1.f - h.b1) * u + h.b1 * m[i];
+    float vi = (1.f - h.b2) * u * u + h.b2 * v[i];
+    m[i] = mi;
+    v[i] = vi;
+    float mh = mi / h.bc1, vh = vi / h.bc2;
+    float up = mh / (sqrtf(vh) + h.eps) + h.wd * theta[i];
+    theta[i] = theta[i] + (-h.lr) * up;
+  }
+}
+
+// --------------------------------------------------------------- host ----
+static int check_flow(const fmc_flow* fl, FlowDev& fd) {
+  FMC_ARG(fl != nullptr, "flow is NULL");
+  FMC_ARG(fl->n_dim >= 2 && fl->n_dim <= 512, "flow n_dim must be in [2, 512] (got %d)", fl->n_dim);
+  FMC_ARG(fl->n_layers >= 1 && fl->n_layers <= 256, "flow n_layers must be in [1, 256]");
+  FMC_ARG(fl->n_bins >= 1 && fl->n_bins <= 42, "flow n_bins must be in [1, 42]");
+  FMC_ARG(fl->hidden1 >= 1 && fl->hidden1 <= 512 && fl->hidden2 >= 1 && fl->hidden2 <= 512,
+          "hidden sizes must be in [1, 512]");
+  FMC_ARG(fl->precision == FMC_PREC_F32 || fl->precision == FMC_PREC_BF16, "unknown precision %d", fl->precision);
+  FMC_ARG(fl->theta != nullptr, "flow theta is NULL");
+  FMC_ARG(fl->range_max > fl->range_min, "spline range must be increasing");
+  FMC_ARG(fl->n_bins * fl->min_bin_size < fl->range_max - fl->range_min, "min_bin_size too large for the range");
+  FMC_ARG(fl->min_knot_slope > 0.f && fl->min_knot_slope < 1.f, "min_knot_slope must be in (0, 1)");
+  fd.dm = FlowDims{fl->n_dim, fl->n_layers, fl->n_bins, fl->hidden1, fl->hidden2,
+                   fl->precision == FMC_PREC_BF16 ? 16 : 2};
+  fd.theta = fl->theta;
+  fd.packed = fl->packed;
+  fd.pbias = fl->packed_bias;
+  fd.lo = fl->range_min;
+  fd.hi = fl->range_max;
+  fd.mb = fl->min_bin_size;
+  fd.ms = fl->min_knot_slope;
+  fd.off = (float)log(exp(1.0 - (double)fl->min_knot_slope) - 1.0);
+  return FMC_OK;
+}
+
+static int pass_mb(const FlowDims& dm, bool bf16, size_t& shm) {
+  // largest tile that fits the LDS budget
+  for (int mb = 2; mb >= 1; --mb) {
+    int tot = bf16 ? PassSmem<PBF16>(dm, 32 * mb).total : PassSmem<PF32>(dm, 32 * mb).total;
+    if (tot <= 160 * 1024) {
+      shm = (size_t)tot;
+      return mb;
+    }
+  }
+  return 0;
+}
+
+static int bwd_mb(const FlowDims& dm, bool bf16, size_t& shm) {
+  for (int mb = 2; mb >= 1; --mb) {
+    int tot = bf16 ? BwdSmem<PBF16>(dm, 32 * mb).total : BwdSmem<PF32>(dm, 32 * mb).total;
+    if (tot <= 160 * 1024) {
+      shm = (size_t)tot;
+      return mb;
+    }
+  }
+  return 0;
+}
+
+template <class K>
+static void set_lds(K kernel, size_t shm) {
+  hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+}
+
+}  // namespace fmc
+
+using namespace fmc;
+
+extern "C" int fmc_flow_sizes(const fmc_flow* fl, int64_t* theta_size, int64_t* packed_size,
+                              int64_t* packed_bias_size, int64_t* store_rows, int32_t* tile_fwd, int32_t* tile_bwd) {
+  FlowDev fd;
+  int rc = check_flow(fl, fd);
+  if (rc) return rc;
+  const FlowDims& dm = fd.dm;
+  if (theta_size) *theta_size = dm.th_total();
+  if (packed_size) *packed_size = dm.pk_total();
+  if (packed_bias_size) *packed_bias_size = dm.pb_total();
+  if (store_rows) *store_rows = dm.st_rows(0) > dm.st_rows(1) ? dm.st_rows(0) : dm.st_rows(1);
+  size_t s1 = 0, s2 = 0;
+  int mbf = pass_mb(dm, fl->precision == FMC_PREC_BF16, s1);
+  int mbb = bwd_mb(dm, fl->precision == FMC_PREC_BF16, s2);
+  if (tile_fwd) *tile_fwd = 32 * mbf;
+  if (tile_bwd) *tile_bwd = 32 * mbb;
+  return FMC_OK;
+}
+
+extern "C" int fmc_flow_pack(void* stream, const fmc_flow* fl) {
+  FlowDev fd;
+  int rc = check_flow(fl, fd);
+  if (rc) return rc;
+  FMC_ARG(fl->packed != nullptr && fl->packed_bias != nullptr, "packed buffers are NULL");
+  const int64_t n = fd.dm.pk_total() + fd.dm.pb_total();
+  unsigned grid = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+  hipStream_t st = (hipStream_t)stream;
+  if (fl->precision == FMC_PREC_BF16)
+    hipLaunchKernelGGL(flow_pack_kernel<PBF16>, dim3(grid), dim3(256), 0, st, fd, (__bf16*)fl->packed,
+                       fl->packed_bias);
+  else
+    hipLaunchKernelGGL(flow_pack_kernel<PF32>, dim3(grid), dim3(256), 0, st, fd, (float*)fl->packed,
+                       fl->packed_bias);
+  return check_launch("flow pack");
+}
+
+extern "C" int fmc_flow_pass(void* stream, const fmc_flow* fl, int32_t direction, int32_t flags, const float* x_in,
+                             int64_t n, uint32_t key0, uint32_t key1, int64_t sample0, float* x_out, float* logdet,
+                             float* saved, const float* g_logp, float* gy_out) {
+  FlowDev fd;
+  int rc = check_flow(fl, fd);
+  if (rc) return rc;
+  FMC_ARG(fl->packed != nullptr && fl->packed_bias != nullptr, "packed buffers are NULL (call fmc_flow_pack)");
+  FMC_ARG(direction >= 0 && direction <= 2, "direction must be 0 (forward), 1 (inverse) or 2 (exact inverse)");
+  FMC_ARG(n >= 0, "n < 0");
+  FMC_ARG(x_in != nullptr || (direction != 0 && (flags & FL_SAMPLE)), "x_in is NULL");
+  FMC_ARG(gy_out == nullptr || (g_logp != nullptr && direction == 0 && (flags & FL_BASE)),
+          "gy_out needs g_logp, the forward direction and the base log-prob flag");
+  FMC_ARG(saved == nullptr || direction == 0, "saved layer inputs are only produced by the forward direction");
+  FMC_ARG(sample0 >= 0 && sample0 + n <= 0xFFFFFFFFLL, "sample index range exceeds 32 bits");
+  if (n == 0) return FMC_OK;
+  size_t shm = 0;
+  const bool bf = fl->precision == FMC_PREC_BF16;
+  int mb = pass_mb(fd.dm, bf, shm);
+  FMC_REQUIRE(mb > 0, FMC_ERR_UNSUPPORTED, "flow dimensions need more than 160 KB of LDS per tile");
+  PassArgs a{direction, flags, x_in, n, key0, key1, sample0, x_out, logdet, saved, g_logp, gy_out};
+  const int MT = 32 * mb;
+  dim3 g((unsigned)((n + MT - 1) / MT)), b(256);
+  hipStream_t st = (hipStream_t)stream;
+  if (bf) {
+    if (mb == 2) {
+      set_lds(flow_pass_kernel<PBF16, 2>, shm);
+      hipLaunchKernelGGL((flow_pass_kernel<PBF16, 2>), g, b, shm, st, fd, a);
+    } else {
+      set_lds(flow_pass_kernel<PBF16, 1>, shm);
+      hipLaunchKernelGGL((flow_pass_kernel<PBF16, 1>), g, b, shm, st, fd, a);
+    }
+  } else {
+    if (mb == 2) {
+      set_lds(flow_pass_kernel<PF32, 2>, shm);
+      hipLaunchKernelGGL((flow_pass_kernel<PF32, 2>), g, b, shm, st, fd, a);
+    } else {
+      set_lds(flow_pass_kernel<PF32, 1>, shm);
+      hipLaunchKernelGGL((flow_pass_kernel<PF32, 1>), g, b, shm, st, fd, a);
+    }
+  }
+  return check_launch("flow pass");
+}
+
+extern "C" int fmc_flow_backward_layer(void* stream, const fmc_flow* fl, int32_t layer, const float* x_layer,
+                                       int64_t n, const float* g_logp, const float* gy, float* gx, void* store,
+                                       int64_t store_ld, float* affine_partials, int32_t n_partials) {
+  FlowDev fd;
+  int rc = check_flow(fl, fd);
+  if (rc) return rc;
+  FMC_ARG(fl->packed != nullptr && fl->packed_bias != nullptr, "packed buffers are NULL (call fmc_flow_pack)");
+  FMC_ARG(layer >= 0 && layer < fl->n_layers, "layer out of range");
+  FMC_ARG(x_layer && g_logp && gy && gx && store && affine_partials, "NULL pointer");
+  FMC_ARG(n >= 1, "n must be >= 1");
+  FMC_ARG(store_ld >= n && store_ld % 64 == 0, "store_ld must be a multiple of 64 and >= n");
+  size_t shm = 0;
+  const bool bf = fl->precision == FMC_PREC_BF16;
+  int mb = bwd_mb(fd.dm, bf, shm);
+  FMC_REQUIRE(mb > 0, FMC_ERR_UNSUPPORTED, "flow dimensions need more than 160 KB of LDS per tile");
+  const int hmax = fd.dm.h1p() > fd.dm.h2p() ? fd.dm.h1p() : fd.dm.h2p();
+  FMC_REQUIRE(hmax <= 512, FMC_ERR_UNSUPPORTED, "hidden sizes > 512 not supported by the backward kernel");
+  const int MT = 32 * mb;
+  // every store column [0, store_ld) is rewritten (rows >= n carry zero gradients)
+  const int64_t nblk = store_ld / MT;
+  FMC_ARG(n_partials >= nblk, "affine_partials holds %d blocks, need %lld", n_partials, (long long)nblk);
+  BwdArgs a{layer, x_layer, n, g_logp, gy, gx, store, store_ld, affine_partials};
+  dim3 g((unsigned)nblk), b(256);
+  hipStream_t st = (hipStream_t)stream;
+#define FMC_BWD(PT_, MB_, H_)                                                   \
+  do {                                                                          \
+    set_lds(flow_bwd_kernel<PT_, MB_, H_>, shm);                                \
+    hipLaunchKernelGGL((flow_bwd_kernel<PT_, MB_, H_>), g, b, shm, st, fd, a); \
+  } while (0)
+  const bool small = hmax <= 256;
+  if (bf) {
+    if (mb == 2) {
+      if (small) FMC_BWD(PBF16, 2, 256); else FMC_BWD(PBF16, 2, 512);
+    } else {
+      if (small) FMC_BWD(PBF16, 1, 256); else FMC_BWD(PBF16, 1, 512);
+    }
+  } else {
+    if (mb == 2) {
+      if (small) FMC_BWD(PF32, 2, 256); else FMC_BWD(PF32, 2, 512);
+    } else {
+      if (small) FMC_BWD(PF32, 1, 256); else FMC_BWD(PF32, 1, 512);
+    }
+  }
+#undef FMC_BWD
+  return check_launch("flow backward");
+}
+
+extern "C" int fmc_flow_weight_grad(void* stream, const fmc_flow* fl, int32_t layer, const void* store,
+                                    int64_t store_ld, const float* affine_partials, int32_t n_partials, float* grad) {
+  FlowDev fd;
+  int rc = check_flow(fl, fd);
+  if (rc) return rc;
+  FMC_ARG(layer >= 0 && layer < fl->n_layers, "layer out of range");
+  FMC_ARG(store && affine_partials && grad, "NULL pointer");
+  FMC_ARG(store_ld >= 64 && store_ld % 64 == 0, "store_ld must be a positive multiple of 64");
+  const FlowDims& dm = fd.dm;
+  const int h1p = dm.h1p(), h2p = dm.h2p();
+  const int ntiles = (h1p / 32) * (dm.ncp(layer) / 32) + (h2p / 32) * (h1p / 32) + (dm.n3p(layer) / 32) * (h2p / 32);
+  hipStream_t st = (hipStream_t)stream;
+  const bool bf = fl->precision == FMC_PREC_BF16;
+  dim3 g((unsigned)((ntiles + 3) / 4)), b(256);
+  if (bf)
+    hipLaunchKernelGGL(flow_wgrad_kernel<PBF16>, g, b, 0, st, fd, layer, store, store_ld, grad);
+  else
+    hipLaunchKernelGGL(flow_wgrad_kernel<PF32>, g, b, 0, st, fd, layer, store, store_ld, grad);
+  if ((rc = check_launch("flow weight grad"))) return rc;
+  const int nrows = h1p + h2p + dm.n3p(layer) + 1;
+  dim3 g2((unsigned)((nrows + 3) / 4));
+  if (bf)
+    hipLaunchKernelGGL(flow_rowsum_kernel<PBF16>, g2, b, 0, st, fd, layer, store, store_ld, affine_partials,
+                       n_partials, grad);
+  else
+    hipLaunchKernelGGL(flow_rowsum_kernel<PF32>, g2, b, 0, st, fd, layer, store, store_ld, affine_partials,
+                       n_partials, grad);
+  return check_launch("flow bias grad");
+}
+
+extern "C" int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n, float lr,
+                              float b1, float b2, float eps, float weight_decay, float max_norm, int32_t step,
+                              double* workspace, int32_t workspace_len, float* grad_norm) {
+  FMC_ARG(theta && grad && m && v && workspace, "NULL pointer");
+  FMC_ARG(n >= 0 && step >= 1, "n must be >= 0 and step >= 1");
+  FMC_ARG(workspace_len >= 1, "workspace_len must be >= 1");
+  if (n == 0) return FMC_OK;
+  int nb = (int)((n + 255) / 256);
+  if (nb > workspace_len) nb = workspace_len;
+  if (nb > 1024) nb = 1024;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(sumsq_kernel, dim3(nb), dim3(256), 0, st, grad, n, workspace);
+  int rc = check_launch("adamw sumsq");
+  if (rc) return rc;
+  AdamHyper h{lr, b1, b2, eps, weight_decay, max_norm, (float)(1.0 - pow((double)b1, step)),
+              (float)(1.0 - pow((double)b2, step))};
+  int nb2 = (int)((n + 255) / 256);
+  if (nb2 > 4096) nb2 = 4096;
+  hipLaunchKernelGGL(adamw_kernel, dim3(nb2), dim3(256), 0, st, theta, grad, m, v, n, h, workspace, nb, grad_norm);
+  return check_launch("adamw update");
+}

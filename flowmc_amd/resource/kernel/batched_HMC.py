@@ -1,0 +1,136 @@
+"""Many independent HMC problems of uneven dimension advanced in ONE launch.
+
+Each problem ("slot") is a built-in target with its own dimension, chains,
+position / log-prob / acceptance buffers and lanes-per-chain; the work
+items (slot, first chain, n chains) map one workgroup each. Per chain the
+step is HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) exactly as
+in the single-target fused kernel (fmc_hmc_run); this is the batched
+form of TakeSerialSteps(HMC) over several LogPDFs.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import torch
+
+from ... import _native
+from ... import autotune
+from ... import random as frandom
+from ...targets import Target
+
+
+def _pow2ceil(v: int) -> int:
+    p = 1
+    while p < v:
+        p *= 2
+    return p
+
+
+class BatchedHMC:
+    def __init__(self, targets: Sequence[Target], n_chains, step_size: float = 0.05, n_leapfrog: int = 10,
+                 metrics: Optional[Sequence[Optional[torch.Tensor]]] = None, device="cuda"):
+        _native.require_gpu()
+        self.targets = list(targets)
+        T = len(self.targets)
+        self.n_chains = [int(n_chains)] * T if isinstance(n_chains, int) else [int(c) for c in n_chains]
+        self.dims = [t.n_dim for t in self.targets]
+        self.step_size = float(step_size)
+        self.n_leapfrog = int(n_leapfrog)
+        self.device = torch.device(device)
+        self.metrics = list(metrics) if metrics is not None else [None] * T
+        self.x = [torch.zeros((c, d), dtype=torch.float32, device=self.device) for c, d in zip(self.n_chains, self.dims)]
+        self.logp = [torch.zeros(c, dtype=torch.float32, device=self.device) for c in self.n_chains]
+        self.layout = None  # (vec_per_lane, block_size, lanes_per_chain per slot)
+
+    def init_positions(self, key):
+        for i, x in enumerate(self.x):
+            x.copy_(frandom.normal(frandom.fold_in(key, i), tuple(x.shape), device=self.device))
+
+    # ------------------------------------------------------------- tables --
+    def default_layout(self, nv: int = 1, block: int = 256, spread: int = 1):
+        lpcs = [min(64, _pow2ceil(max(1, -(-d // (4 * nv)))) * spread) for d in self.dims]
+        for d, l in zip(self.dims, lpcs):
+            if 4 * nv * l < d:
+                raise ValueError(f"vec_per_lane={nv} cannot hold d={d} in 64 lanes")
+        return nv, block, lpcs
+
+    def _tables(self, layout, key, n_steps_noise, outputs, noise, log_u):
+        nv, block, lpcs = layout
+        slots = (_native.fmc_hmc_slot * len(self.targets))()
+        items = []
+        self._keep = []
+        for s, t in enumerate(self.targets):
+            k0, k1 = frandom.key_words(frandom.fold_in(key, s))
+            out = outputs[s] if outputs is not None else (None, None, None, 0)
+            m = self.metrics[s]
+            nz = noise[s] if noise is not None else None
+            lu = log_u[s] if log_u is not None else None
+            self._keep += [m, nz, lu]
+            buf_len = int(out[0].shape[1]) if out[0] is not None else 1
+            slots[s].target = t.descriptor()
+            slots[s].x = self.x[s].data_ptr()
+            slots[s].logp = self.logp[s].data_ptr()
+            slots[s].metric = _native.dptr(m, name="metric")
+            slots[s].noise = _native.dptr(nz, name="noise")
+            slots[s].log_u = _native.dptr(lu, name="log_u")
+            slots[s].out_pos = _native.dptr(out[0], name="position buffer")
+            slots[s].out_logp = _native.dptr(out[1], name="log_prob buffer")
+            slots[s].out_acc = _native.dptr(out[2], name="acceptance buffer")
+            slots[s].n_chains = self.n_chains[s]
+            slots[s].buf_len = buf_len
+            slots[s].cursor = int(out[3])
+            slots[s].key0, slots[s].key1 = k0, k1
+            slots[s].lanes_per_chain = lpcs[s]
+            cpb = block // lpcs[s]
+            for c0 in range(0, self.n_chains[s], cpb):
+                items.append((s, c0, min(cpb, self.n_chains[s] - c0), 0))
+        # order work items by cost (dim) so long rows start first
+        items.sort(key=lambda it: -self.dims[it[0]])
+        sb = bytes(slots)
+        slot_dev = torch.frombuffer(bytearray(sb), dtype=torch.uint8).to(self.device)
+        it_dev = torch.tensor(items, dtype=torch.int32).reshape(-1).to(self.device)
+        return slot_dev, it_dev, len(items)
+
+    # --------------------------------------------------------------- run ---
+    def run(self, key, n_steps: int, outputs=None, thinning: int = 1, noise=None, log_u=None, fused_steps: int = 16,
+            layout=None):
+        """Advance every problem by ``n_steps`` HMC steps. ``outputs[s]`` is
+        (positions [C,T,d], log_probs [C,T], accepts [C,T], cursor) or None;
+        ``noise[s]`` [n_steps, C, d] / ``log_u[s]`` [n_steps, C] inject randomness."""
+        layout = layout or self.layout or self.default_layout()
+        nv, block, lpcs = layout
+        st = _native.stream_ptr()
+        maxd = max(self.dims)
+        for s0 in range(0, n_steps, fused_steps):
+            ns = min(fused_steps, n_steps - s0)
+            nz = [n[s0:s0 + ns] for n in noise] if noise is not None else None
+            lu = [u[s0:s0 + ns] for u in log_u] if log_u is not None else None
+            slot_dev, it_dev, n_items = self._tables(layout, key, ns, outputs, nz, lu)
+            _native.call("fmc_hmc_run_batched", st, slot_dev.data_ptr(), len(self.targets), it_dev.data_ptr(), n_items,
+                         maxd, ns, s0, self.step_size, self.n_leapfrog, thinning, nv, block)
+            self._last_tables = (slot_dev, it_dev)  # keep alive until the stream passes them
+
+    def tune(self, key, n_steps: int = 8):
+        """Pick (vec_per_lane, block size, lane spread) by timing short runs."""
+        sig = ("hmc_batched", tuple(self.dims), tuple(self.n_chains), self.n_leapfrog)
+        saved = [x.clone() for x in self.x]
+        cands = []
+        for nv in (1, 2):
+            for block in (128, 256, 512):
+                for spread in (1, 2):
+                    try:
+                        cands.append(self.default_layout(nv, block, spread))
+                    except ValueError:
+                        pass
+
+        def go(layout):
+            for x, s in zip(self.x, saved):
+                x.copy_(s)
+            self.run(key, n_steps, layout=layout, fused_steps=n_steps)
+
+        self.layout = autotune.best_config(sig, cands, go)
+        for x, s in zip(self.x, saved):
+            x.copy_(s)
+        return self.layout

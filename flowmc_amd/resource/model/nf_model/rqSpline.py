@@ -1,0 +1,381 @@
+"""MaskedCouplingRQSpline — src/flowMC/resource/model/nf_model/rqSpline.py:364-507.
+
+All arithmetic runs in the flow kernels of libflowmc_hip (fmc_flow_pass,
+fmc_flow_backward_layer, fmc_flow_weight_grad, fmc_adamw_clip). Parameters
+live in one flat FP32 master vector ``theta`` (layout documented in
+flowmc_amd/csrc/fmc_flow_layout.h); the conditioner GEMMs read an MFMA
+operand image packed from it in ``precision`` ("bfloat16" default, or
+"float32") after every update.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+from typing import Optional
+
+import numpy as np
+import torch
+
+from .... import _native
+from .... import random as frandom
+from ..common import MLP, Gaussian, Linear, MaskedCouplingLayer, ScalarAffine
+from .base import NFModel
+
+FORWARD, INVERSE, INVERSE_EXACT = 0, 1, 2
+STANDARDIZE, BASE_LOGP, SAMPLE_BASE = 1, 2, 4
+_PREC = {"float32": 0, "fp32": 0, "bfloat16": 1, "bf16": 1}
+
+
+class _Workspace:
+    """Device scratch of one training batch size (allocated once, reused)."""
+
+    def __init__(self, model: "MaskedCouplingRQSpline", B: int):
+        dev = model.theta.device
+        d, L = model.n_features, model.n_layers
+        self.B = B
+        self.ld = (B + 63) // 64 * 64
+        tdt = torch.bfloat16 if model.precision == "bfloat16" else torch.float32
+        self.saved = torch.empty((L, B, d), dtype=torch.float32, device=dev)
+        self.logp = torch.empty(B, dtype=torch.float32, device=dev)
+        self.g_logp = torch.full((B,), -1.0 / B, dtype=torch.float32, device=dev)
+        self.gy = torch.empty((B, d), dtype=torch.float32, device=dev)
+        self.gx = torch.empty((B, d), dtype=torch.float32, device=dev)
+        self.store = torch.empty((model._store_rows, self.ld), dtype=tdt, device=dev)
+        self.n_part = self.ld // model._tile_bwd
+        self.aff = torch.empty((self.n_part, 2), dtype=torch.float32, device=dev)
+        self.grad = torch.zeros(model.theta.numel(), dtype=torch.float32, device=dev)
+        self.norm_ws = torch.empty(1024, dtype=torch.float64, device=dev)
+        self.gnorm = torch.empty(1, dtype=torch.float32, device=dev)
+
+
+class MaskedCouplingRQSpline(NFModel):
+    def __repr__(self):
+        return "MaskedCouplingRQSpline with n_features=" + str(self._n_features) + ", n_layers=" + str(
+            self.n_layers)
+
+    def __init__(self, n_features: int, n_layers: int, hidden_size: list, num_bins: int, key,
+                 spline_range=(-10.0, 10.0), precision: str = "bfloat16", device=None, **kwargs):
+        if len(hidden_size) != 2:
+            raise NotImplementedError("the flow kernels implement conditioners with two hidden layers")
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {list(_PREC)}")
+        _native.require_gpu()
+        self.device = torch.device("cuda") if device is None else torch.device(device)
+        self._n_features = int(n_features)
+        self.n_layers = int(n_layers)
+        self.hidden_size = [int(h) for h in hidden_size]
+        self.num_bins = int(num_bins)
+        self.spline_range = (float(spline_range[0]), float(spline_range[1]))
+        self.min_bin_size = 1e-4
+        self.min_knot_slope = 1e-4
+        self.precision = "bfloat16" if _PREC[precision] == 1 else "float32"
+        self.exact_inverse = bool(kwargs.get("exact_inverse", False))
+        self._configure()
+        theta = kwargs.get("_theta")
+        if theta is None:
+            theta = self._init_theta(key, kwargs)
+        self.theta = theta.to(self.device, torch.float32).contiguous()
+        self._dirty = True
+        self._ws: dict[int, _Workspace] = {}
+
+    # ------------------------------------------------------------ layout --
+    def _configure(self):
+        probe = self._struct(theta_ptr=1)
+        th, pk, pb, rows = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        tf, tb = C.c_int32(), C.c_int32()
+        _native.call("fmc_flow_sizes", C.byref(probe), C.byref(th), C.byref(pk), C.byref(pb), C.byref(rows),
+                     C.byref(tf), C.byref(tb))
+        self._theta_size, self._packed_size, self._pbias_size = th.value, pk.value, pb.value
+        self._store_rows, self._tile_fwd, self._tile_bwd = rows.value, tf.value, tb.value
+        if self._tile_fwd == 0 or self._tile_bwd == 0:
+            raise NotImplementedError("flow dimensions exceed the kernels' LDS budget")
+        tdt = torch.bfloat16 if self.precision == "bfloat16" else torch.float32
+        self.packed = torch.empty(self._packed_size, dtype=tdt, device=self.device)
+        self.packed_bias = torch.empty(self._pbias_size, dtype=torch.float32, device=self.device)
+        d, (h1, h2), P = self._n_features, self.hidden_size, 3 * self.num_bins + 1
+        self._layer_size = 2 + h1 * d + h1 + h2 * h1 + h2 + d * P * h2 + d * P
+        off = self.n_layers * self._layer_size
+        self._off = {"data_mean": off, "data_cov": off + d, "base_mean": off + d + d * d,
+                     "base_cov": off + 2 * d + d * d}
+        assert off + 2 * d + 2 * d * d == self._theta_size
+
+    def _struct(self, theta_ptr=None) -> _native.fmc_flow:
+        h1, h2 = self.hidden_size
+        tp = theta_ptr if theta_ptr is not None else _native.dptr(self.theta, name="theta")
+        pk = getattr(self, "packed", None)
+        pb = getattr(self, "packed_bias", None)
+        return _native.fmc_flow(self._n_features, self.n_layers, self.num_bins, h1, h2, _PREC[self.precision],
+                                self.spline_range[0], self.spline_range[1], self.min_bin_size, self.min_knot_slope,
+                                tp, pk.data_ptr() if (pk is not None and theta_ptr is None) else None,
+                                pb.data_ptr() if (pb is not None and theta_ptr is None) else None)
+
+    def view(self, layer: int, name: str) -> torch.Tensor:
+        """View of one leaf of layer ``layer`` inside theta."""
+        d, (h1, h2), P = self._n_features, self.hidden_size, 3 * self.num_bins + 1
+        shapes = [("scale", ()), ("shift", ()), ("W1", (h1, d)), ("b1", (h1,)), ("W2", (h2, h1)), ("b2", (h2,)),
+                  ("W3", (d * P, h2)), ("b3", (d * P,))]
+        o = layer * self._layer_size
+        for k, shp in shapes:
+            n = int(np.prod(shp)) if shp else 1
+            if k == name:
+                return self.theta[o:o + n].view(shp)
+            o += n
+        raise KeyError(name)
+
+    def _stat(self, name):
+        d = self._n_features
+        o = self._off[name]
+        n = d if name.endswith("mean") else d * d
+        t = self.theta[o:o + n]
+        return t.view(d) if name.endswith("mean") else t.view(d, d)
+
+    @property
+    def data_mean(self):
+        return self._stat("data_mean")
+
+    @property
+    def data_cov(self):
+        return self._stat("data_cov")
+
+    @property
+    def base_dist(self):
+        return Gaussian(self._stat("base_mean"), self._stat("base_cov"), learnable=False)
+
+    @property
+    def layers(self):
+        """Reference-shaped views: [MaskedCouplingLayer(ScalarAffine), MaskedCouplingLayer(RQSpline MLP)] per layer."""
+        out = []
+        d = self._n_features
+        for i in range(self.n_layers):
+            aff = MaskedCouplingLayer(ScalarAffine(self.view(i, "scale"), self.view(i, "shift")),
+                                      torch.zeros(d, dtype=torch.bool))
+            mlp = MLP([Linear(self.view(i, "W1"), self.view(i, "b1")), Linear(self.view(i, "W2"), self.view(i, "b2")),
+                       Linear(self.view(i, "W3"), self.view(i, "b3"))])
+            mask = ((torch.arange(d) + i) % 2).to(torch.bool)
+            out.append([aff, MaskedCouplingLayer(mlp, mask)])
+        return out
+
+    # -------------------------------------------------------------- init --
+    def _init_theta(self, key, kwargs) -> torch.Tensor:
+        """Initialisation with the distributions of rqSpline.py:427-443 / common.py:84-107:
+        hidden weights N(0, 1e-2 / fan_in), hidden biases and the last layer
+        U(-1/sqrt(fan_in), 1/sqrt(fan_in)), ScalarAffine (0, 0). The draws come
+        from a torch generator seeded from ``key`` (not JAX's streams)."""
+        d, (h1, h2), P = self._n_features, self.hidden_size, 3 * self.num_bins + 1
+        k0, k1 = frandom.key_words(key)
+        g = torch.Generator().manual_seed((k0 << 32) | k1)
+        parts = []
+        for _ in range(self.n_layers):
+            parts += [torch.zeros(2)]
+            parts += [torch.randn(h1, d, generator=g) * math.sqrt(1e-2 / d),
+                      (torch.rand(h1, generator=g) * 2 - 1) / math.sqrt(d)]
+            parts += [torch.randn(h2, h1, generator=g) * math.sqrt(1e-2 / h1),
+                      (torch.rand(h2, generator=g) * 2 - 1) / math.sqrt(h1)]
+            parts += [(torch.rand(d * P, h2, generator=g) * 2 - 1) / math.sqrt(h2),
+                      (torch.rand(d * P, generator=g) * 2 - 1) / math.sqrt(h2)]
+        dm = kwargs.get("data_mean")
+        dc = kwargs.get("data_cov")
+        bd = kwargs.get("base_dist")
+        dm = torch.zeros(d) if dm is None else torch.as_tensor(dm, dtype=torch.float32).reshape(d)
+        dc = torch.eye(d) if dc is None else torch.as_tensor(dc, dtype=torch.float32).reshape(d, d)
+        if bd is None:
+            bm, bc = torch.zeros(d), torch.eye(d)
+        else:
+            bm = torch.as_tensor(bd.mean, dtype=torch.float32).reshape(d).cpu()
+            bc = torch.as_tensor(bd.cov, dtype=torch.float32).reshape(d, d).cpu()
+            if torch.count_nonzero(bc - torch.diag(torch.diagonal(bc))) != 0:
+                raise NotImplementedError("the flow kernels need a diagonal base covariance")
+        parts += [dm.reshape(-1).cpu(), dc.reshape(-1).cpu(), bm.reshape(-1), bc.reshape(-1)]
+        return torch.cat([p.reshape(-1).to(torch.float32) for p in parts])
+
+    def clone(self, theta: Optional[torch.Tensor] = None) -> "MaskedCouplingRQSpline":
+        m = MaskedCouplingRQSpline.__new__(MaskedCouplingRQSpline)
+        m.__dict__.update({k: v for k, v in self.__dict__.items() if k not in ("theta", "packed", "packed_bias", "_ws")})
+        m._configure()
+        m.theta = (self.theta if theta is None else theta).clone()
+        m._dirty = True
+        m._ws = {}
+        return m
+
+    def set_data_stats(self, mean, cov):
+        self.data_mean.copy_(mean)
+        self.data_cov.copy_(cov)
+        self._dirty = True
+
+    def to_precision(self, precision: str = "float32") -> "MaskedCouplingRQSpline":
+        """nf_model/base.py:212-242: precision of the conditioner GEMMs (theta stays FP32)."""
+        p = precision.lower()
+        if p not in _PREC:
+            raise ValueError(f"Precision {precision} not supported. Choose from {list(_PREC)}")
+        m = self.clone()
+        m.precision = "bfloat16" if _PREC[p] == 1 else "float32"
+        m._configure()
+        return m
+
+    # -------------------------------------------------------------- kernels
+    def _ensure_packed(self):
+        if self._dirty:
+            s = self._struct()
+            _native.call("fmc_flow_pack", _native.stream_ptr(), C.byref(s))
+            self._dirty = False
+
+    def mark_updated(self):
+        """Call after editing theta (or a view of it) in place."""
+        self._dirty = True
+
+    def _pass(self, direction, flags, x, n, key=(0, 0), saved=None, g_logp=None, gy_out=None, want_x=True):
+        self._ensure_packed()
+        d = self._n_features
+        out = torch.empty((n, d), dtype=torch.float32, device=self.device) if want_x else None
+        ld = torch.empty(n, dtype=torch.float32, device=self.device)
+        s = self._struct()
+        _native.call("fmc_flow_pass", _native.stream_ptr(), C.byref(s), direction, flags,
+                     _native.dptr(x, name="x"), n, key[0], key[1], 0, _native.dptr(out, name="x_out"),
+                     _native.dptr(ld, name="logdet"), _native.dptr(saved, name="saved"),
+                     _native.dptr(g_logp, name="g_logp"), _native.dptr(gy_out, name="gy_out"))
+        return out, ld
+
+    def _as_batch(self, x):
+        x = torch.as_tensor(x, dtype=torch.float32)
+        if not x.is_cuda:
+            x = x.to(self.device)
+        lead = x.shape[:-1]
+        return x.reshape(-1, self._n_features).contiguous(), lead
+
+    def forward(self, x, key=None, condition=None):
+        """rqSpline.py:450-468: (y, log_det) with no standardisation."""
+        xb, lead = self._as_batch(x)
+        y, ld = self._pass(FORWARD, 0, xb, xb.shape[0])
+        return y.reshape(*lead, -1), ld.reshape(lead)
+
+    def inverse(self, x, condition=None):
+        """rqSpline.py:470-488 (reference layer order unless exact_inverse)."""
+        xb, lead = self._as_batch(x)
+        y, ld = self._pass(INVERSE_EXACT if self.exact_inverse else INVERSE, 0, xb, xb.shape[0])
+        return y.reshape(*lead, -1), ld.reshape(lead)
+
+    def log_prob(self, x):
+        """rqSpline.py:498-504. Differentiable in x: the backward pass runs the
+        hand-written adjoint kernels (fmc_flow_backward_layer)."""
+        if isinstance(x, torch.Tensor) and x.requires_grad and torch.is_grad_enabled():
+            return _FlowLogProb.apply(x, self)
+        return self._log_prob_value(x)
+
+    def _log_prob_value(self, x):
+        xb, lead = self._as_batch(x)
+        _, lp = self._pass(FORWARD, STANDARDIZE | BASE_LOGP, xb, xb.shape[0], want_x=False)
+        return lp.reshape(lead)
+
+    def sample(self, rng_key, n_samples: int):
+        """rqSpline.py:490-496, base noise from the device Philox stream of ``rng_key``."""
+        k = frandom.key_words(rng_key)
+        x, _ = self._pass(INVERSE_EXACT if self.exact_inverse else INVERSE, SAMPLE_BASE | STANDARDIZE, None,
+                          int(n_samples), key=k)
+        return x
+
+    def sample_from_noise(self, noise):
+        """``sample`` with injected standard-normal base noise [n, d]."""
+        xb, lead = self._as_batch(noise)
+        x, _ = self._pass(INVERSE_EXACT if self.exact_inverse else INVERSE, SAMPLE_BASE | STANDARDIZE, xb,
+                          xb.shape[0])
+        return x.reshape(*lead, -1)
+
+    # ------------------------------------------------------------ training
+    def _workspace(self, B: int) -> _Workspace:
+        ws = self._ws.get(B)
+        if ws is None:
+            ws = self._ws[B] = _Workspace(self, B)
+        return ws
+
+    def grad_log_prob_sum(self, x, g_logp=None):
+        """Hand-written adjoint: d/dtheta of sum_i g_logp[i] * log_prob(x_i) (flat,
+        theta layout) and d/dx (standardised-space gradient divided by sigma)."""
+        xb, _ = self._as_batch(x)
+        B = xb.shape[0]
+        ws = self._workspace(B)
+        if g_logp is not None:
+            ws.g_logp.copy_(torch.as_tensor(g_logp, dtype=torch.float32, device=self.device).reshape(B))
+        else:
+            ws.g_logp.fill_(1.0)
+        self._ensure_packed()
+        self._pass(FORWARD, STANDARDIZE | BASE_LOGP, xb, B, saved=ws.saved, g_logp=ws.g_logp, gy_out=ws.gy,
+                   want_x=False)
+        self._backward(ws, B)
+        gx = ws.gy if self.n_layers % 2 == 0 else ws.gx
+        gx = gx / torch.sqrt(torch.diagonal(self.data_cov))
+        out = ws.grad.clone(), gx.clone()
+        ws.g_logp.fill_(-1.0 / B)
+        return out
+
+    def _backward(self, ws: _Workspace, B: int):
+        s = self._struct()
+        st = _native.stream_ptr()
+        ws.grad.zero_()
+        gy, gx = ws.gy, ws.gx
+        for layer in reversed(range(self.n_layers)):
+            _native.call("fmc_flow_backward_layer", st, C.byref(s), layer, ws.saved[layer].data_ptr(), B,
+                         ws.g_logp.data_ptr(), gy.data_ptr(), gx.data_ptr(), ws.store.data_ptr(), ws.ld,
+                         ws.aff.data_ptr(), ws.n_part)
+            _native.call("fmc_flow_weight_grad", st, C.byref(s), layer, ws.store.data_ptr(), ws.ld,
+                         ws.aff.data_ptr(), ws.n_part, ws.grad.data_ptr())
+            gy, gx = gx, gy
+
+    def train_step(self, x, optim, state):
+        """nf_model/base.py:102-125: loss, hand-written gradient, clip + AdamW (in place)."""
+        xb, _ = self._as_batch(x)
+        B = xb.shape[0]
+        ws = self._workspace(B)
+        self._ensure_packed()
+        s = self._struct()
+        _native.call("fmc_flow_pass", _native.stream_ptr(), C.byref(s), FORWARD, STANDARDIZE | BASE_LOGP,
+                     xb.data_ptr(), B, 0, 0, 0, None, ws.logp.data_ptr(), ws.saved.data_ptr(), ws.g_logp.data_ptr(),
+                     ws.gy.data_ptr())
+        loss = -ws.logp.mean()
+        self._backward(ws, B)
+        state.count += 1
+        mx = optim.max_norm if optim.max_norm is not None else 3.0e38
+        _native.call("fmc_adamw_clip", _native.stream_ptr(), self.theta.data_ptr(), ws.grad.data_ptr(),
+                     state.mu.data_ptr(), state.nu.data_ptr(), self.theta.numel(), float(optim.learning_rate),
+                     float(optim.b1), float(optim.b2), float(optim.eps), float(optim.weight_decay), float(mx),
+                     state.count, ws.norm_ws.data_ptr(), 1024, ws.gnorm.data_ptr())
+        self._dirty = True
+        return loss, self, state
+
+    # --------------------------------------------------------------- I/O --
+    def _config(self):
+        return {"n_features": self._n_features, "n_layers": self.n_layers, "hidden_size": self.hidden_size,
+                "num_bins": self.num_bins, "spline_range": list(self.spline_range), "precision": self.precision}
+
+    def save_model(self, path: str):
+        np.savez(path + ".npz", theta=self.theta.detach().cpu().numpy(), config=json.dumps(self._config()))
+
+    def load_model(self, path: str) -> "MaskedCouplingRQSpline":
+        p = path if path.endswith(".npz") else path + ".npz"
+        f = np.load(p)
+        cfg = json.loads(str(f["config"]))
+        if cfg["n_features"] != self._n_features or cfg["n_layers"] != self.n_layers:
+            raise ValueError("saved flow has a different shape")
+        return self.clone(torch.as_tensor(f["theta"], device=self.device))
+
+    def print_parameters(self):
+        print("RQSpline parameters:")
+        print(self._config())
+
+
+class _FlowLogProb(torch.autograd.Function):
+    """log_prob(x) with d/dx from the adjoint kernels."""
+
+    @staticmethod
+    def forward(ctx, x, model):
+        ctx.model = model
+        ctx.save_for_backward(x)
+        return model._log_prob_value(x.detach()).to(x.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        (x,) = ctx.saved_tensors
+        model = ctx.model
+        _, gx = model.grad_log_prob_sum(x.detach().reshape(-1, model.n_features), g.reshape(-1).float())
+        return gx.reshape(x.shape).to(g.dtype), None

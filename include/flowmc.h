@@ -221,6 +221,80 @@ int fmc_rng_normal(void* stream, uint32_t key0, uint32_t key1, int64_t n_chains,
 int fmc_rng_log_uniform(void* stream, uint32_t key0, uint32_t key1, int64_t n_chains,
                         int32_t n_steps, int32_t step0, int32_t stream_id, float* out);
 
+
+/* ------------------------------------------------------------------------ */
+/* MaskedCouplingRQSpline flow                                              */
+/* (src/flowMC/resource/model/nf_model/rqSpline.py:364-504 with the spline  */
+/* of rqSpline.py:20-361 and the blocks of common.py:68-293).               */
+/* The master parameters `theta` are FP32 in the leaf order documented in   */
+/* flowmc_amd/csrc/fmc_flow_layout.h; `packed` / `packed_bias` hold the     */
+/* MFMA operand images rebuilt from theta by fmc_flow_pack.                 */
+/* ------------------------------------------------------------------------ */
+typedef enum fmc_precision { FMC_PREC_F32 = 0, FMC_PREC_BF16 = 1 } fmc_precision;
+
+typedef struct fmc_flow {
+  int32_t n_dim, n_layers, n_bins, hidden1, hidden2;
+  int32_t precision;          /* fmc_precision of the conditioner GEMMs */
+  float range_min, range_max; /* spline range (rqSpline.py:399)        */
+  float min_bin_size, min_knot_slope; /* rqSpline.py:299-300           */
+  const float* theta;
+  void* packed;
+  float* packed_bias;
+} fmc_flow;
+
+/* Sizes (elements) of theta, packed (of the precision's type), packed_bias,
+ * the rows of one backward layer's activation store, and the sample tiles
+ * of the pass / backward kernels. */
+int fmc_flow_sizes(const fmc_flow* flow, int64_t* theta_size, int64_t* packed_size,
+                   int64_t* packed_bias_size, int64_t* store_rows, int32_t* tile_fwd,
+                   int32_t* tile_bwd);
+/* theta -> packed MFMA operands + biases (after every parameter update). */
+int fmc_flow_pack(void* stream, const fmc_flow* flow);
+
+enum { FMC_FLOW_FORWARD = 0, FMC_FLOW_INVERSE = 1, FMC_FLOW_INVERSE_EXACT = 2 };
+enum { FMC_FLOW_STANDARDIZE = 1, FMC_FLOW_BASE_LOGP = 2, FMC_FLOW_SAMPLE_BASE = 4 };
+
+/* One pass over n samples [n, d].
+ * FORWARD  (MaskedCouplingRQSpline.forward :450-468): per layer ScalarAffine
+ *   then RQ-spline coupling; with STANDARDIZE the input is first mapped by
+ *   (x - data_mean) / sqrt(diag data_cov); with BASE_LOGP `logdet` receives
+ *   log_prob = logdet + log N(y; base_mean, diag base_cov) (log_prob :498-504).
+ * INVERSE  (inverse :470-488): layers in reverse, within a layer
+ *   ScalarAffine^-1 then spline^-1 (the reference's order);
+ *   INVERSE_EXACT applies spline^-1 then ScalarAffine^-1.
+ *   With SAMPLE_BASE the input is standard-normal noise (x_in, or Philox
+ *   stream 5 keyed by (key0,key1) with counter (dim/4, sample0+i, 0, 5) when
+ *   x_in is NULL) mapped to N(base_mean, diag base_cov); with STANDARDIZE
+ *   the output is y * sqrt(diag data_cov) + data_mean (sample :490-496).
+ * saved (FORWARD only, may be NULL): [n_layers, n, d] input of every layer.
+ * gy_out (FORWARD + BASE_LOGP): g_logp[i] * d log N / dy. */
+int fmc_flow_pass(void* stream, const fmc_flow* flow, int32_t direction, int32_t flags,
+                  const float* x_in, int64_t n, uint32_t key0, uint32_t key1, int64_t sample0,
+                  float* x_out, float* logdet, float* saved, const float* g_logp, float* gy_out);
+
+/* Adjoint of forward layer `layer` (autodiff of NFModel.loss_fn,
+ * nf_model/base.py:98-100, done by hand): given the layer input x_layer,
+ * dL/dlog_prob per sample (g_logp) and dL/d(layer output) (gy), writes
+ * dL/d(layer input) (gx), the feature-major activation store
+ * [store_rows, store_ld] (store_ld = n rounded up to 64) and per-block
+ * ScalarAffine partial gradients (store_ld / tile_bwd blocks x 2). */
+int fmc_flow_backward_layer(void* stream, const fmc_flow* flow, int32_t layer,
+                            const float* x_layer, int64_t n, const float* g_logp, const float* gy,
+                            float* gx, void* store, int64_t store_ld, float* affine_partials,
+                            int32_t n_partials);
+/* Weight / bias / ScalarAffine gradients of `layer` into grad (theta layout). */
+int fmc_flow_weight_grad(void* stream, const fmc_flow* flow, int32_t layer, const void* store,
+                         int64_t store_ld, const float* affine_partials, int32_t n_partials,
+                         float* grad);
+
+/* optax.chain(clip_by_global_norm(max_norm), adamw(lr, b1, b2, eps, weight_decay))
+ * applied to the flat vector theta (src/flowMC/resource/optimizer.py:19-22);
+ * step counts from 1. workspace: >= min(1024, ceil(n/256)) doubles.
+ * grad_norm (may be NULL) receives the global gradient norm. */
+int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n,
+                   float lr, float b1, float b2, float eps, float weight_decay, float max_norm,
+                   int32_t step, double* workspace, int32_t workspace_len, float* grad_norm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,47 @@
+"""R-hat / bulk-ESS on processes with known answers (CPU)."""
+
+import math
+
+import torch
+
+from flowmc_amd import diagnostics as D
+
+
+def ar1(C, N, Dm, phi, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    e = torch.randn(C, N, Dm, generator=g, dtype=torch.float64)
+    x = torch.empty_like(e)
+    x[:, 0] = e[:, 0] / math.sqrt(1 - phi * phi)
+    for t in range(1, N):
+        x[:, t] = phi * x[:, t - 1] + e[:, t]
+    return x
+
+
+def test_ess_of_iid_draws_is_the_sample_size():
+    x = torch.randn(8, 2000, 3, generator=torch.Generator().manual_seed(1), dtype=torch.float64)
+    e = D.ess_bulk(x)
+    assert bool(((e > 0.85 * 16000) & (e < 1.15 * 16000)).all()), e
+
+
+def test_ess_of_ar1_matches_closed_form():
+    # integrated autocorrelation time of AR(1): (1 + phi) / (1 - phi)
+    phi = 0.8
+    x = ar1(16, 4000, 2, phi)
+    want = 16 * 4000 * (1 - phi) / (1 + phi)
+    e = D.ess_bulk(x)
+    assert bool(((e > 0.85 * want) & (e < 1.15 * want)).all()), (e, want)
+
+
+def test_rhat_detects_unmixed_chains():
+    x = torch.randn(4, 1000, 2, generator=torch.Generator().manual_seed(2), dtype=torch.float64)
+    assert float(D.rhat(x).max()) < 1.01
+    x[:2] += 3.0  # two chains stuck elsewhere
+    assert float(D.rhat(x).max()) > 1.5
+
+
+def test_rank_normalize_is_monotone_and_standard():
+    x = torch.rand(3, 500, 1, generator=torch.Generator().manual_seed(3), dtype=torch.float64) ** 3
+    z = D.rank_normalize(x)
+    order_x = torch.argsort(x.reshape(-1))
+    assert bool((torch.diff(z.reshape(-1)[order_x]) > 0).all())
+    assert abs(float(z.mean())) < 1e-3 and abs(float(z.std()) - 1) < 0.02

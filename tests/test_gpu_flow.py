@@ -1,0 +1,300 @@
+"""GPU parity of the RQ-spline flow kernels against the FP64 oracle.
+
+Tolerances (stated per precision of the conditioner GEMMs):
+  FP32 conditioner: log_prob / log-det / positions within 2e-4 * (1 + |ref|);
+                    parameter gradients within 2e-3 * max|grad| (FP32
+                    accumulation over the batch vs FP64).
+  BF16 conditioner: log_prob within 3e-2 * (1 + |ref|) and positions within
+                    3e-2 * (1 + |ref|) (8-bit mantissa operands, FP32
+                    accumulation, FP32 spline); gradients: cosine similarity
+                    with the FP64 gradient >= 0.999 and relative L2 error <= 5e-2.
+"""
+
+import ctypes as C
+
+import pytest
+import torch
+
+import flowmc_amd._native as N
+from flowmc_amd import random as fr
+from flowmc_amd.resource.buffers import Buffer
+from flowmc_amd.resource.kernel.NF_proposal import NFProposal
+from flowmc_amd.resource.logPDF import LogPDF
+from flowmc_amd.resource.model.nf_model.rqSpline import MaskedCouplingRQSpline
+from flowmc_amd.resource.optimizer import AdamState, Optimizer, adamw
+from flowmc_amd.resource.states import State
+from flowmc_amd.strategy.take_steps import TakeGroupSteps
+from flowmc_amd.strategy.train_model import TrainModel
+from oracle import flow as F
+from oracle import optim as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"float32": 2e-4, "bfloat16": 2e-3}
+CASES = [  # d, L, K, h1, h2
+    (5, 4, 8, 32, 32),
+    (32, 3, 10, 128, 128),
+    (64, 2, 8, 256, 256),
+    (7, 3, 4, 16, 48),
+]
+
+
+def make(d, L, K, h1, h2, prec, seed=0, scale=0.3, hidden_var=1.0):
+    p = F.init_params(d, L, K, h1, h2, seed=seed, scale=scale, hidden_var=hidden_var)
+    g = torch.Generator().manual_seed(seed + 100)
+    p["data_mean"] = torch.randn(d, generator=g, dtype=torch.float64) * 0.5
+    A = torch.rand(d, generator=g, dtype=torch.float64) + 0.5
+    p["data_cov"] = torch.diag(A) + 0.01 * torch.ones(d, d, dtype=torch.float64)  # off-diagonals are ignored
+    p["base_cov"] = torch.eye(d, dtype=torch.float64) * 0.9
+    p["base_mean"] = torch.full((d,), 0.1, dtype=torch.float64)
+    theta = F.to_flat(p).float()
+    p = F.from_flat(theta.double(), d, L, K, h1, h2)  # oracle on the FP32-rounded parameters
+    m = MaskedCouplingRQSpline(d, L, [h1, h2], K, fr.PRNGKey(0), precision=prec, _theta=theta)
+    return m, p
+
+
+def within(a, b, tol, what="", quantile=1.0):
+    """scaled error |a-b|/(1+|b|) <= tol for the given quantile of elements."""
+    a = a.detach().cpu().double()
+    err = (a - b).abs() / (1 + b.abs())
+    ok = bool(torch.quantile(err.reshape(-1), quantile) <= tol) if quantile < 1.0 else bool((err <= tol).all())
+    if not ok:
+        print(f"{what}: max scaled err {float(err.max()):.3e} (tol {tol:.1e}), "
+              f"median {float(err.median()):.3e}, n>tol {int((err > tol).sum())}/{err.numel()}")
+    return ok
+
+
+@pytest.mark.parametrize("prec", ["float32", "bfloat16"])
+@pytest.mark.parametrize("case", CASES)
+def test_flow_passes_match_oracle(gpu, prec, case):
+    """FP32: every element within 2e-4 of the FP64 oracle. BF16: against the
+    oracle that rounds the GEMM operands to BF16 as the kernels do, 99% of
+    elements within 2e-3 (the rest are points where a rounding tie or a
+    steep spline segment amplifies a last-bit difference)."""
+    d, L, K, h1, h2 = case
+    m, p = make(d, L, K, h1, h2, prec)
+    g = torch.Generator().manual_seed(1)
+    x = torch.randn(300, d, generator=g, dtype=torch.float64) * 3  # some samples land in the linear tails
+    xf = x.float()
+    tol = TOL[prec]
+    q = 1.0 if prec == "float32" else 0.99
+    y, ld = m.forward(xf.cuda())
+    xi, ldi = m.inverse(xf.cuda())
+    m.exact_inverse = True
+    xe, _ = m.inverse(y.detach())
+    m.exact_inverse = False
+    z = torch.randn(200, d, generator=g)
+    lp = m.log_prob(xf.cuda())
+    smp = m.sample_from_noise(z.cuda())
+    with (F.emulate_bf16() if prec == "bfloat16" else _null()):
+        ry, rld = F.forward(p, xf.double(), K)
+        rxi, rldi = F.inverse(p, xf.double(), K)
+        rlp = F.log_prob(p, xf.double(), K)
+        rsmp = F.sample_from_noise(p, z.double(), K)
+    checks = [
+        within(lp, rlp, tol, "log_prob", q),
+        within(y, ry, tol, "forward y", q), within(ld, rld, tol, "forward logdet", q),
+        within(xi, rxi, tol, "inverse x", q), within(ldi, rldi, tol, "inverse logdet", q),
+        within(xe, xf.double(), 10 * tol, "exact inverse round trip", q),
+        within(smp, rsmp, tol, "sample", q),
+    ]
+    assert all(checks), checks
+
+
+class _null:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *a):
+        return False
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_flow_bf16_error_vs_fp64_reference_init(gpu, case):
+    """BF16 conditioner against the exact FP64 flow for parameters drawn like
+    the reference's initialisation (hidden weights N(0, 1e-2/fan_in)): every
+    log_prob within 1e-2 * (1 + |ref|)."""
+    d, L, K, h1, h2 = case
+    m, p = make(d, L, K, h1, h2, "bfloat16", scale=0.05, hidden_var=1e-2)
+    g = torch.Generator().manual_seed(11)
+    x = (torch.randn(500, d, generator=g) * 2).float()
+    assert within(m.log_prob(x.cuda()), F.log_prob(p, x.double(), K), 1e-2, "bf16 log_prob vs fp64")
+
+
+@pytest.mark.parametrize("prec", ["float32", "bfloat16"])
+@pytest.mark.parametrize("case", CASES[:3])
+def test_flow_adjoint_matches_oracle_autograd(gpu, prec, case):
+    d, L, K, h1, h2 = case
+    m, p = make(d, L, K, h1, h2, prec, seed=2)
+    g = torch.Generator().manual_seed(3)
+    B = 200
+    x = (torch.randn(B, d, generator=g) * 2).float()
+    w = torch.randn(B, generator=g).float()
+    gt, gx = m.grad_log_prob_sum(x.cuda(), w.cuda())
+    theta = F.to_flat(p).clone().requires_grad_(True)
+    xr = x.double().clone().requires_grad_(True)
+    with (F.emulate_bf16() if prec == "bfloat16" else _null()):
+        obj = (w.double() * F.log_prob(F.from_flat(theta, d, L, K, h1, h2), xr, K)).sum()
+        rg, rgx = torch.autograd.grad(obj, [theta, xr])
+    n_layers_part = m.n_layers * m._layer_size  # stat leaves get no gradient
+    gt = gt.cpu().double()
+    assert float(gt[n_layers_part:].abs().max()) == 0.0
+    a, b = gt[:n_layers_part], rg[:n_layers_part]
+    if prec == "float32":
+        assert float((a - b).abs().max()) <= 2e-3 * float(b.abs().max())
+        assert within(gx, rgx, 2e-3)
+    else:
+        # against the BF16-emulating oracle: relative L2 error of the whole gradient
+        rel = float((a - b).norm() / b.norm())
+        relx = float((gx.cpu().double() - rgx).norm() / rgx.norm())
+        print(f"bf16 adjoint: rel L2 err theta {rel:.3e}, x {relx:.3e}")
+        assert rel <= 1e-2 and relx <= 1e-2, (rel, relx)
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_flow_bf16_gradient_vs_fp64_reference_init(gpu, case):
+    """BF16 adjoint against the exact FP64 gradient at reference-like init."""
+    d, L, K, h1, h2 = case
+    m, p = make(d, L, K, h1, h2, "bfloat16", seed=2, scale=0.05, hidden_var=1e-2)
+    g = torch.Generator().manual_seed(3)
+    x = (torch.randn(400, d, generator=g) * 2).float()
+    gt, _ = m.grad_log_prob_sum(x.cuda(), torch.full((400,), -1.0 / 400).cuda())
+    theta = F.to_flat(p).clone().requires_grad_(True)
+    (rg,) = torch.autograd.grad(F.loss(F.from_flat(theta, d, L, K, h1, h2), x.double(), K), theta)
+    n = m.n_layers * m._layer_size
+    a, b = gt.cpu().double()[:n], rg[:n]
+    cos = float((a * b).sum() / (a.norm() * b.norm()))
+    rel = float((a - b).norm() / b.norm())
+    print(f"bf16 gradient vs fp64: cos {cos:.6f}, rel L2 {rel:.3e}")
+    assert cos >= 0.999 and rel <= 5e-2, (cos, rel)
+
+
+@pytest.mark.parametrize("clip", [True, False])
+def test_adamw_kernel_matches_oracle(gpu, clip):
+    n = 100003
+    g = torch.Generator().manual_seed(4)
+    theta = torch.randn(n, generator=g)
+    th, mm, vv = theta.double(), torch.zeros(n, dtype=torch.float64), torch.zeros(n, dtype=torch.float64)
+    dt, dm, dv = theta.cuda(), torch.zeros(n, device=gpu), torch.zeros(n, device=gpu)
+    ws = torch.empty(1024, dtype=torch.float64, device=gpu)
+    gn = torch.empty(1, device=gpu)
+    for t in range(1, 4):
+        grad = torch.randn(n, generator=g) * (0.05 if clip else 1e-4)
+        th, mm, vv = O.adamw_clip_step(th, grad.double(), mm, vv, t, 1e-3, max_norm=1.0)
+        gd = grad.cuda()
+        N.call("fmc_adamw_clip", N.stream_ptr(), dt.data_ptr(), gd.data_ptr(), dm.data_ptr(), dv.data_ptr(), n, 1e-3,
+               0.9, 0.999, 1e-8, 1e-4, 1.0, t, ws.data_ptr(), 1024, gn.data_ptr())
+        assert abs(float(gn) - float(grad.double().norm())) <= 1e-5 * float(grad.double().norm())
+    assert float((dt.cpu().double() - th).abs().max()) <= 1e-6
+
+
+def test_train_step_matches_oracle_fp32(gpu):
+    d, L, K, h1, h2 = 6, 2, 5, 32, 32
+    m, p = make(d, L, K, h1, h2, "float32", seed=5)
+    g = torch.Generator().manual_seed(6)
+    x = (torch.randn(256, d, generator=g) * 1.5).float()
+    opt = adamw(1e-3, weight_decay=1e-4, max_norm=1.0)
+    state = opt.init(m)
+    loss, m2, state = m.train_step(x.cuda(), opt, state)
+    theta = F.to_flat(p).clone().requires_grad_(True)
+    rl = F.loss(F.from_flat(theta, d, L, K, h1, h2), x.double(), K)
+    (rg,) = torch.autograd.grad(rl, theta)
+    th, _, _ = O.adamw_clip_step(theta.detach(), rg, torch.zeros_like(rg), torch.zeros_like(rg), 1, 1e-3,
+                                 max_norm=1.0)
+    assert abs(float(loss) - float(rl.detach())) <= 2e-4 * (1 + abs(float(rl.detach())))
+    # the first Adam step moves every parameter by ~lr*sign(g): compare the update where |g| is not tiny
+    upd = m2.theta.cpu().double() - F.to_flat(p)
+    rupd = th - F.to_flat(p)
+    big = rg.abs() > 1e-3 * rg.abs().max()
+    assert float((upd[big] - rupd[big]).abs().max()) <= 2e-5
+    assert state.count == 1
+
+
+def test_training_reduces_loss_and_best_model(gpu):
+    d = 4
+    g = torch.Generator().manual_seed(7)
+    means = torch.tensor([[2.0, -1.0, 0.0, 1.0], [-2.0, 1.0, 1.0, -1.0]])
+    lab = torch.randint(0, 2, (4000,), generator=g)
+    data = (means[lab] + 0.5 * torch.randn(4000, d, generator=g)).cuda()
+    m = MaskedCouplingRQSpline(d, 4, [32, 32], 8, fr.PRNGKey(1))
+    opt = Optimizer(m, learning_rate=1e-2)
+    rng, best, state, losses = m.train(fr.PRNGKey(2), data, opt.optim, opt.optim_state, 8, 500, verbose=False)
+    assert losses.shape == (8,)
+    assert float(losses[-1]) < float(losses[0])
+    assert float(losses.min()) == pytest.approx(float(best.loss_fn(data)), abs=0.5)
+    # the caller's model is unchanged (functional train)
+    assert float(m.data_mean.abs().max()) == 0.0 and float(best.data_mean.abs().max()) > 0.0
+
+
+def test_nf_proposal_accepts_all_when_target_is_the_flow(gpu):
+    # test/unit/test_strategies.py:251-319 with the assertion the reference leaves out
+    n_chains, n_steps, d = 5, 25, 2
+    model = MaskedCouplingRQSpline(d, 3, [16, 16], 8, fr.PRNGKey(10))
+    resources = {
+        "test_position": Buffer("test_position", (n_chains, n_steps, d), 1),
+        "test_log_prob": Buffer("test_log_prob", (n_chains, n_steps), 1),
+        "test_acceptance": Buffer("test_acceptance", (n_chains, n_steps), 1),
+        "NFProposal": NFProposal(model, n_NFproposal_batch_size=5),
+        "logpdf": LogPDF(lambda x, data: model.log_prob(x), n_dims=d),
+        "sampler_state": State({"test_position": "test_position", "test_log_prob": "test_log_prob",
+                                "test_acceptance": "test_acceptance"}, name="sampler_state"),
+    }
+    x0 = torch.zeros(n_chains, d, device=gpu)
+    for S in (11, 5):
+        strat = TakeGroupSteps("logpdf", "NFProposal", "sampler_state",
+                               ["test_position", "test_log_prob", "test_acceptance"], n_steps=S)
+        strat(fr.PRNGKey(42), resources, x0, {"data": torch.arange(d)})
+        acc = resources["test_acceptance"].data[:, :S]
+        assert bool((acc == 1).all())
+        pos = resources["test_position"].data[:, :S]
+        assert torch.allclose(resources["test_log_prob"].data[:, :S], model.log_prob(pos), atol=1e-5)
+
+
+def test_train_model_strategy(gpu):
+    # test/unit/test_strategies.py:224-249 plus checks of the data selection
+    n_chains, n_steps, d = 5, 25, 2
+    model = MaskedCouplingRQSpline(d, 3, [16, 16], 8, fr.PRNGKey(10))
+    buf = Buffer("test_data", (n_chains, n_steps, d), 1)
+    buf.update_buffer(fr.normal(fr.PRNGKey(0), (n_chains, n_steps - 5, d)))  # last 5 rows stay -inf
+    opt = Optimizer(model)
+    res = {"test_data": buf, "optimizer": opt, "model": model,
+           "loss": Buffer("loss", (10,), 0)}
+    strat = TrainModel("model", "test_data", "optimizer", loss_buffer_name="loss", n_epochs=10,
+                       batch_size=n_chains * n_steps, n_max_examples=10000, history_window=1)
+    _, res, _ = strat(fr.PRNGKey(42), res, torch.zeros(n_chains, d, device=gpu), {})
+    new = res["model"]
+    assert isinstance(new, MaskedCouplingRQSpline) and new is not model
+    # history_window=1: training data are the last finite row of each chain
+    last = buf.data[:, n_steps - 6]
+    assert torch.allclose(new.data_mean, last.mean(0), atol=5e-2)
+    assert bool(torch.isfinite(res["loss"].data).all()) and res["loss"].cursor == 10
+    assert opt.optim_state.count >= 1
+
+
+def test_flow_log_prob_autograd_function(gpu):
+    d, L, K, h1, h2 = 5, 3, 6, 32, 32
+    m, p = make(d, L, K, h1, h2, "float32", seed=8)
+    x = (torch.randn(40, d) * 1.5).cuda().requires_grad_(True)
+    lp = m.log_prob(x)
+    (gx,) = torch.autograd.grad(lp.sum(), x)
+    xr = x.detach().cpu().double().requires_grad_(True)
+    (rgx,) = torch.autograd.grad(F.log_prob(p, xr, K).sum(), xr)
+    assert within(gx, rgx, 2e-3, "d log_prob / dx")
+
+
+def test_mala_on_a_flow_target(gpu):
+    # a user target that calls the flow: LogPDF falls back to the batched path
+    from flowmc_amd.resource.kernel.base import StepOutputs
+    from flowmc_amd.resource.kernel.MALA import MALA
+
+    d = 4
+    m = MaskedCouplingRQSpline(d, 2, [32, 32], 8, fr.PRNGKey(3), precision="float32")
+    lpdf = LogPDF(lambda x, data: m.log_prob(x), n_dims=d)
+    C_, S = 64, 20
+    pos = torch.zeros(C_, S, d, device=gpu)
+    lp = torch.zeros(C_, S, device=gpu)
+    ac = torch.zeros(C_, S, device=gpu)
+    MALA(0.5).sample_steps(fr.PRNGKey(1), torch.zeros(C_, d, device=gpu), lpdf, None, S, 1,
+                           StepOutputs(pos, lp, ac, 0))
+    assert bool(torch.isfinite(pos).all()) and 0.2 < float(ac.mean()) <= 1.0
+    assert torch.allclose(lp, m.log_prob(pos), atol=1e-4)
