@@ -1,13 +1,26 @@
-"""Benchmark of the flowMC hot path on one MI355X.
+"""Benchmarks of the flowMC hot path on one MI355X. Prints ONE JSON line.
 
-Default workload (chain-steps/s): fused MALA on the 128-D Rosenbrock target
-log p = -sum_i [100 (x_{i+1} - x_i^2)^2 + (1 - x_i)^2] / 20, 16384 chains,
-step 0.01, 64 steps fused per launch, thinning 1, every step's position /
-log-prob / acceptance row written to device buffers of the full run length
-(as TakeSerialSteps does). One timed "step" = one MALA step of all chains.
-
-Prints ONE JSON line (see the keys below). ``--gpus N`` with N > 1 exits: the
-build targets a single GPU.
+Workloads (``--workload``):
+  nfmcmc_gmm64  (default) full NF-MCMC loop (RQSpline_MALA bundle through
+                Sampler) on the 64-D 8-mode Gaussian mixture: 4096 chains,
+                MALA step 0.05, RQ-spline flow 10 layers / 8 bins / [256, 256]
+                with BF16 conditioner GEMMs, ``--steps`` training rounds then
+                ``--steps`` production rounds of 100 local + 100 global steps,
+                5 epochs per round, batch 10000. value = min bulk-ESS over dims
+                of the production samples / end-to-end wall time (training
+                included); valid when max R-hat < 1.01. The same workload
+                run by the plain FP32 per-step GPU path (bench_plain.py) is
+                reported beside it.
+  mala_rosenbrock_128d  fused MALA on the 128-D Rosenbrock target, 16384
+                chains, step 0.01, 64 steps per launch; one step = one MALA
+                step of all chains (chain-steps/s).
+  flow_train_d32  flow training only: d=32, 8 layers, 10 bins, [128, 128],
+                BF16 conditioner, 100k samples of a 4-component mixture,
+                batch 10k, Adam lr 1e-3 (training samples/s).
+  hmc_funnels   32 Neal's funnels of log-uniform dimension in [8, 256], 256
+                chains each, 10 leapfrog steps of 0.05, one batched launch per
+                16 steps (chain-steps/s).
+``--gpus N`` with N > 1 exits: the build runs on a single GPU.
 """
 
 from __future__ import annotations
@@ -15,6 +28,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import platform
 import sys
@@ -25,6 +39,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -39,6 +54,171 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def flow_flops_per_sample(d, L, K, h1, h2):
+    """Minimal conditioner FLOPs of one pass: sum over layers of
+    2 * (d_c*h1 + h1*h2 + h2*d_t*(3K+1))."""
+    tot = 0
+    for l in range(L):
+        dc = (d + 1) // 2 if l % 2 else d // 2
+        dt = d - dc
+        tot += 2 * (dc * h1 + h1 * h2 + h2 * dt * (3 * K + 1))
+    return tot
+
+
+def time_kernel(fn, reps=5):
+    """Average time (ms) of fn() measured with HIP events on the current stream."""
+    import torch
+
+    fn()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+# --------------------------------------------------------------- C5 -------
+def bench_nfmcmc(args):
+    import torch
+
+    from flowmc_amd import diagnostics as D
+    from flowmc_amd import random as fr
+    from flowmc_amd import targets as TG
+    from flowmc_amd.resource.logPDF import LogPDF
+    from flowmc_amd.resource_strategy_bundle.RQSpline_MALA import RQSpline_MALA_Bundle
+    from flowmc_amd.Sampler import Sampler
+
+    dev = torch.device("cuda")
+    C_, d = args.chains or 4096, 64
+    L, K, H = 10, 8, [256, 256]
+    cfg = dict(n_local=100, n_global=100, n_loops=args.steps, n_epochs=args.epochs, batch_size=10000,
+               n_max_examples=10000,
+               step_size=0.05, lr=1e-3, n_layers=L, n_bins=K, hidden=H)
+    target = TG.gaussian_mixture_64d(seed=0, device=dev)
+
+    def make(n_chains, loops, key):
+        k1, k2 = fr.split(key)
+        b = RQSpline_MALA_Bundle(k1, n_chains, d, LogPDF(target, n_dims=d), cfg["n_local"], cfg["n_global"], loops,
+                                 loops, cfg["n_epochs"], mala_step_size=cfg["step_size"], rq_spline_hidden_units=H,
+                                 rq_spline_n_bins=K, rq_spline_n_layers=L, learning_rate=cfg["lr"],
+                                 batch_size=cfg["batch_size"], n_max_examples=cfg["n_max_examples"],
+                                 precision=args.precision)
+        return b, Sampler(d, n_chains, k2, resource_strategy_bundles=b)
+
+    # warm-up: a short run of the same shapes (kernels, allocator, workspaces)
+    if args.warmup > 0:
+        _, s = make(C_, args.warmup, fr.PRNGKey(7))
+        s.sample(fr.normal(fr.PRNGKey(8), (C_, d)), {})
+        torch.cuda.synchronize()
+    bundle, sampler = make(C_, args.steps, fr.PRNGKey(0))
+    x0 = fr.normal(fr.PRNGKey(1), (C_, d))
+    theta0 = bundle.resources["model"].theta.detach().clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sampler.sample(x0, {})
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res = sampler.resources
+    pos = res["positions_production"].data
+    summ = D.summary(pos)
+    mean_err = float((summ["mean"] - target.mean().to(dev)).abs().max())
+    var_err = float(((summ["var"] - target.variance().to(dev)) / target.variance().to(dev)).abs().max())
+    local_acc = float(res["local_accs_production"].data[torch.isfinite(res["local_accs_production"].data)].mean())
+    global_acc = float(res["global_accs_production"].data[torch.isfinite(res["global_accs_production"].data)].mean())
+    value = summ["min_ess_bulk"] / wall
+    # roofline of the dominant kernel: one flow inverse pass over the C*S proposals
+    model = res["model"]
+    S = cfg["n_global"]
+    ms = time_kernel(lambda: model.sample(fr.PRNGKey(3), C_ * S), reps=3)
+    fl = flow_flops_per_sample(d, L, K, H[0], H[1]) * C_ * S
+    achieved = fl / (ms * 1e-3) / 1e12
+    out = {
+        "value": value,
+        "ms_per_step": wall * 1e3 / (2 * args.steps),
+        "wall_s": wall,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "flow_pass_kernel (inverse, BF16)", "kernel_ms": ms,
+                     "flops_per_launch": fl},
+        "extra": {"min_ess_bulk": summ["min_ess_bulk"], "max_rhat": summ["max_rhat"],
+                  "max_abs_mean_err": mean_err, "max_rel_var_err": var_err, "local_acceptance": local_acc,
+                  "global_acceptance": global_acc, "valid": bool(summ["max_rhat"] < 1.01)},
+        "config": {"workload": "nfmcmc_gmm64", "chains": C_, "dim": d, "modes": 8, "mala_step": 0.05,
+                   "flow": {"layers": L, "bins": K, "hidden": H, "precision": args.precision},
+                   "training_rounds": args.steps, "production_rounds": args.steps, "local_steps": 100,
+                   "global_steps": 100, "epochs": args.epochs, "batch": 10000, "n_max_examples": 10000},
+    }
+    del res, sampler, bundle, pos
+    torch.cuda.empty_cache()
+    if not args.no_plain:
+        import bench_plain as BP
+
+        t0 = time.perf_counter()
+        ppos, tr, pr = BP.run_plain_nfmcmc(theta0, cfg, target, x0, time_budget_s=args.plain_budget)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t0
+        psum = D.summary(ppos)
+        pval = psum["min_ess_bulk"] / pwall
+        out["extra"]["plain_gpu"] = {"value": pval, "wall_s": pwall, "min_ess_bulk": psum["min_ess_bulk"],
+                                     "max_rhat": psum["max_rhat"], "training_rounds": tr, "production_rounds": pr,
+                                     "speedup_vs_plain": value / pval if pval > 0 else None}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_nfmcmc(args, cfg, wall, summ["min_ess_bulk"], C_)
+    return out
+
+
+def cpu_baseline_nfmcmc(args, cfg, gpu_wall, min_ess, C_):
+    """Oracle (FP32 torch-CPU restatement of the reference, full-width
+    conditioner, 2 value_and_grad per MALA step) timed on a bounded sample:
+    one training round and one production round for a subset of chains; the
+    full run's CPU time is extrapolated (chain-proportional parts scaled to
+    all chains, training cost kept) and the fused run's min-ESS is divided
+    by it."""
+    import torch
+
+    from oracle import chains as OC
+    from oracle import flow as F
+    from oracle import targets as OT
+    from flowmc_amd import targets as TG
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    d, L, K, H = 64, cfg["n_layers"], cfg["n_bins"], cfg["hidden"]
+    tgt = TG.gaussian_mixture_64d(seed=0, device="cpu")
+    fn = lambda z: OT.gmm_diag(z, tgt.means.float(), tgt.sigmas.float())
+    p = F.init_params(d, L, K, H[0], H[1], dtype=torch.float32, hidden_var=1e-2)
+    cs = args.cpu_chains
+    x = torch.randn(cs, d)
+    t0 = time.perf_counter()
+    for _ in range(cfg["n_local"]):
+        x, _, _, _ = OC.mala_step(x, fn, cfg["step_size"], torch.randn(cs, d), torch.log(torch.rand(cs)))
+    t_local = time.perf_counter() - t0
+    # one training step on 10k examples (of 5 per round)
+    theta = F.to_flat(p).clone().requires_grad_(True)
+    data = torch.randn(cfg["batch_size"], d)
+    t0 = time.perf_counter()
+    loss = F.loss(F.from_flat(theta, d, L, K, H[0], H[1]), data, K)
+    loss.backward()
+    t_train_step = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        z = torch.randn(cs * 4, d)
+        prop = F.sample_from_noise(p, z, K)
+        F.log_prob(p, prop, K)
+        fn(prop)
+    t_global = (time.perf_counter() - t0) * cfg["n_global"] / 4
+    per_chain = (t_local + t_global) / cs
+    total = cfg["n_loops"] * 2 * per_chain * C_ + cfg["n_loops"] * cfg["n_epochs"] * t_train_step
+    return {"value": min_ess / total, "unit": "min-ESS/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"FP32 torch-CPU oracle restatement ({cpu_model()}): {cs} chains x (100 MALA steps + "
+                      f"4 of 100 NF proposals per chain) and one 10k-sample training step timed; full run of "
+                      f"{C_} chains x {2 * cfg['n_loops']} rounds extrapolated to {total:.0f} s; ESS taken "
+                      f"from the fused run (same algorithm)"}
+
+
+# --------------------------------------------------------------- C3 -------
 def bench_mala_rosenbrock(args):
     import torch
 
@@ -46,7 +226,7 @@ def bench_mala_rosenbrock(args):
     from flowmc_amd import random as fr
     from flowmc_amd import targets as TG
 
-    C_, d, dt, K = args.chains, args.dim, args.step_size, args.fused
+    C_, d, dt, K = args.chains or 16384, 128, 0.01, 64
     dev = torch.device("cuda")
     tgt = TG.Rosenbrock(d, device=dev)
     desc = tgt.descriptor()
@@ -81,35 +261,34 @@ def bench_mala_rosenbrock(args):
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
     n_launch = (steps + K - 1) // K
-    # algorithmic HBM bytes: per chain-step the position row (4d), log-prob (4)
-    # and acceptance (4); per launch and chain the state read + write (8d + 8)
     bytes_total = C_ * (steps * (4 * d + 8) + n_launch * (8 * d + 8))
     achieved = bytes_total / (gpu_ms * 1e-3) / 1e9
-    accept = float(acc[:, :steps].mean())
-    # FLOP-side count per chain-step: Rosenbrock value+grad ~17d, MALA update ~12d
     flops = C_ * steps * (17 * d + 12 * d)
-    return {
+    out = {
         "value": C_ * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
-        "kernel_ms_per_launch": gpu_ms / n_launch,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-        "fp32_tflops": flops / (gpu_ms * 1e-3) / 1e12,
-        "acceptance": accept,
-        "bytes_per_launch": bytes_total / n_launch,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "chain_kernel<1,0> (fused MALA)",
+                     "kernel_ms": gpu_ms / n_launch, "bytes_per_launch": bytes_total / n_launch},
+        "extra": {"fp32_tflops": flops / (gpu_ms * 1e-3) / 1e12, "acceptance": float(acc[:, :steps].mean())},
+        "config": {"workload": "mala_rosenbrock_128d", "chains": C_, "dim": d, "step_size": dt,
+                   "fused_steps_per_launch": K, "thinning": 1},
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_mala_rosenbrock(args)
+    return out
 
 
 def cpu_baseline_mala_rosenbrock(args):
-    """The FP32 CPU restatement (oracle, 2 value_and_grad per step as MALA.py:59,71-73)
-    on a bounded sample of chains and steps, timed on the host cores."""
+    """Oracle MALA (2 value_and_grad per step as MALA.py:59,71-73), FP32, on a
+    bounded sample of chains and steps, timed on the host cores."""
     import torch
 
     from oracle import chains as OC
     from oracle import targets as OT
 
     torch.set_num_threads(os.cpu_count() or 1)
-    C_, d, dt = args.cpu_chains, args.dim, args.step_size
+    C_, d, dt = args.cpu_chains, 128, 0.01
     g = torch.Generator().manual_seed(0)
     x = torch.randn(C_, d, generator=g)
     fn = lambda z: OT.rosenbrock(z)
@@ -123,25 +302,152 @@ def cpu_baseline_mala_rosenbrock(args):
         steps += 1
     el = time.perf_counter() - t0
     return {"value": C_ * steps / el, "unit": "chain-steps/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{C_} chains x {steps} MALA steps, d={d}, FP32 torch-CPU oracle restatement "
-                      f"({cpu_model()})"}
+            "sample": f"{C_} chains x {steps} MALA steps, d={d}, FP32 torch-CPU oracle restatement ({cpu_model()})"}
+
+
+# --------------------------------------------------------------- C2 -------
+def bench_flow_train(args):
+    import torch
+
+    from flowmc_amd import random as fr
+    from flowmc_amd.resource.model.nf_model.rqSpline import MaskedCouplingRQSpline
+    from flowmc_amd.resource.optimizer import adam
+
+    dev = torch.device("cuda")
+    d, L, K, H, N, B = 32, 8, 10, [128, 128], 100000, 10000
+    g = torch.Generator().manual_seed(0)
+    means = torch.rand(4, d, generator=g) * 10 - 5
+    lab = torch.randint(0, 4, (N,), generator=g)
+    data = (means[lab] + torch.randn(N, d, generator=g)).to(dev)
+    m = MaskedCouplingRQSpline(d, L, H, K, fr.PRNGKey(0), precision=args.precision)
+    opt = adam(1e-3)
+    state = opt.init(m)
+    m.set_data_stats(data.mean(0), torch.cov(data.T))
+    perm = torch.randperm(N, device=dev)
+    batches = [data[perm[i * B:(i + 1) * B]].contiguous() for i in range(N // B)]
+    for i in range(max(1, args.warmup)):
+        m.train_step(batches[i % len(batches)], opt, state)
+    torch.cuda.synchronize()
+    l0 = float(-m.log_prob(data[:20000]).mean())
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        loss, m, state = m.train_step(batches[i % len(batches)], opt, state)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    l1 = float(-m.log_prob(data[:20000]).mean())
+    # dominant kernel: the per-layer adjoint + weight-gradient; report the whole step on MFMA
+    fl = 3 * flow_flops_per_sample(d, L, K, H[0], H[1]) * B
+    achieved = fl * args.steps / wall / 1e12
+    ms_fwd = time_kernel(lambda: m.log_prob(batches[0]), reps=5)
+    out = {
+        "value": args.steps * B / wall,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "train step (forward + adjoint + weight grad + AdamW)",
+                     "forward_pass_ms": ms_fwd,
+                     "forward_pass_tflops": flow_flops_per_sample(d, L, K, H[0], H[1]) * B / (ms_fwd * 1e-3) / 1e12},
+        "extra": {"loss_before": l0, "loss_after": l1},
+        "config": {"workload": "flow_train_d32", "dim": d, "layers": L, "bins": K, "hidden": H, "batch": B,
+                   "examples": N, "optimizer": "adam(1e-3)", "precision": args.precision},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_flow_train(args, d, L, K, H, B)
+    return out
+
+
+def cpu_baseline_flow_train(args, d, L, K, H, B):
+    import torch
+
+    from oracle import flow as F
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    p = F.init_params(d, L, K, H[0], H[1], dtype=torch.float32, hidden_var=1e-2)
+    theta = F.to_flat(p).clone().requires_grad_(True)
+    nb = min(B, args.cpu_chains)
+    x = torch.randn(nb, d)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        loss = F.loss(F.from_flat(theta, d, L, K, H[0], H[1]), x, K)
+        (gt,) = torch.autograd.grad(loss, theta)
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": steps * nb / el, "unit": "samples/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"{steps} forward+backward steps on {nb}-sample batches, FP32 torch-CPU oracle ({cpu_model()})"}
+
+
+# --------------------------------------------------------------- C4 -------
+def bench_hmc_funnels(args):
+    import torch
+
+    from flowmc_amd import random as fr
+    from flowmc_amd import targets as TG
+    from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
+
+    dims = TG.funnel_dims(seed=0, n_targets=32, lo=8, hi=256)
+    C_ = args.chains or 256
+    bh = BatchedHMC([TG.Funnel(d) for d in dims], C_, step_size=0.05, n_leapfrog=10)
+    bh.init_positions(fr.PRNGKey(0))
+    T = args.steps
+    outs = [(torch.empty((C_, T, d), dtype=torch.float32, device="cuda"),
+             torch.empty((C_, T), dtype=torch.float32, device="cuda"),
+             torch.empty((C_, T), dtype=torch.float32, device="cuda"), 0) for d in dims]
+    layout = bh.tune(fr.PRNGKey(9))
+    bh.run(fr.PRNGKey(1), min(T, 16), outputs=outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bh.run(fr.PRNGKey(2), T, outputs=outs, fused_steps=16)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_steps = C_ * len(dims) * T
+    bytes_total = sum(C_ * T * (4 * d + 8) + C_ * (T // 16 + 1) * (8 * d + 8) for d in dims)
+    achieved = bytes_total / wall / 1e9
+    v_all = torch.cat([o[0][..., 0].reshape(-1) for o in outs])
+    out = {
+        "value": total_steps / wall,
+        "ms_per_step": wall * 1e3 / T,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "hmc_batched_kernel"},
+        "extra": {"acceptance": float(torch.cat([o[2].reshape(-1) for o in outs]).mean()),
+                  "v_mean": float(v_all.mean()), "v_var": float(v_all.var()), "layout": [layout[0], layout[1]],
+                  "dims": dims},
+        "config": {"workload": "hmc_funnels", "targets": 32, "chains_per_target": C_, "n_leapfrog": 10,
+                   "step_size": 0.05, "fused_steps_per_launch": 16},
+    }
+    return out
+
+
+WORKLOADS = {
+    "nfmcmc_gmm64": (bench_nfmcmc, "min-ESS/s at R-hat<1.01 (fused NF-MCMC, 64-D 8-mode mixture)", "min-ESS/s",
+                     "bf16/f32"),
+    "mala_rosenbrock_128d": (bench_mala_rosenbrock, "chain-steps/s (fused MALA, 128-D Rosenbrock)", "chain-steps/s",
+                             "f32"),
+    "flow_train_d32": (bench_flow_train, "flow train samples/s (RQ-spline d=32, 8 layers)", "samples/s", "bf16/f32"),
+    "hmc_funnels": (bench_hmc_funnels, "chain-steps/s (batched HMC, 32 funnels d in [8,256])", "chain-steps/s",
+                    "f32/f64"),
+}
+DEFAULT_STEPS = {"nfmcmc_gmm64": 20, "mala_rosenbrock_128d": 10048, "flow_train_d32": 50, "hmc_funnels": 1000}
+DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d32": 3, "hmc_funnels": 16}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10048)
-    ap.add_argument("--warmup", type=int, default=640)
-    ap.add_argument("--chains", type=int, default=16384)
-    ap.add_argument("--dim", type=int, default=128)
-    ap.add_argument("--step-size", type=float, default=0.01)
-    ap.add_argument("--fused", type=int, default=64)
+    ap.add_argument("--workload", default="nfmcmc_gmm64", choices=list(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--chains", type=int, default=0)
+    ap.add_argument("--precision", default="bfloat16")
+    ap.add_argument("--epochs", type=int, default=5)
     ap.add_argument("--lpc", type=int, default=0)
     ap.add_argument("--nv", type=int, default=0)
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--cpu-chains", type=int, default=16384)
+    ap.add_argument("--cpu-chains", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--plain-budget", type=float, default=240.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -151,12 +457,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
-    res = bench_mala_rosenbrock(args)
-    cpu = None if args.no_cpu_baseline else cpu_baseline_mala_rosenbrock(args)
+    fn, metric, unit, dtype = WORKLOADS[args.workload]
+    args.steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
+    args.warmup = args.warmup if args.warmup is not None else DEFAULT_WARMUP[args.workload]
+    res = fn(args)
     line = {
-        "metric": "chain-steps/s (fused MALA, 128-D Rosenbrock)",
+        "metric": metric,
         "value": res["value"],
-        "unit": "chain-steps/s",
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -164,16 +472,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (x0 ~ N(0, I), in-kernel Philox noise)",
-        "config": {"workload": "mala_rosenbrock_128d", "chains": args.chains, "dim": args.dim,
-                   "step_size": args.step_size, "fused_steps_per_launch": args.fused, "thinning": 1},
+        "dtype": dtype,
+        "data": "synthetic (seeded targets / mixtures, Philox noise)",
+        "config": res["config"],
         "roofline": res["roofline"],
-        "cpu_baseline": cpu,
-        "extra": {"kernel_ms_per_launch": res["kernel_ms_per_launch"], "fp32_tflops": res["fp32_tflops"],
-                  "acceptance": res["acceptance"], "bytes_per_launch": res["bytes_per_launch"]},
+        "cpu_baseline": res.get("cpu_baseline"),
+        "extra": res.get("extra", {}),
     }
-    print(json.dumps(line))
+    print(json.dumps(line, default=float))
 
 
 if __name__ == "__main__":
