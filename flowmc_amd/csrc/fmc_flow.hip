@@ -88,6 +88,45 @@ __device__ __forceinline__ void mfma_block(const typename PT::T* As, int lda, in
   }
 }
 
+// Panel of MB row blocks x up to CB column blocks (nb0, nb0 + nbstep, ...):
+// every A fragment feeds CB MFMAs, every B fragment MB MFMAs, and the B
+// fragments (global / L2) are prefetched P k-steps ahead in a register ring
+// so their latency hides behind the MFMAs of the previous steps.
+template <class PT, int MB, int CB, int P>
+__device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, const typename PT::T* Bp, int KBtot,
+                                           int nb0, int nbstep, int ncb, int kb0, int kb1, int a_kb0,
+                                           f32x16_t (&acc)[CB][MB], int lane) {
+  using BF = decltype(PT::load_b(Bp, 0, 0, 0, 0));
+  using AF = decltype(PT::load_a(As, 0, 0, 0, 0));
+  BF ring[P][CB];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int j = 0; j < CB; ++j)
+      if (kb0 + p < kb1 && j < ncb) ring[p][j] = PT::load_b(Bp, KBtot, nb0 + j * nbstep, kb0 + p, lane);
+  for (int kb = kb0; kb < kb1; kb += P) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int k = kb + p;
+      if (k < kb1) {
+        AF a[MB];
+#pragma unroll
+        for (int r = 0; r < MB; ++r) a[r] = PT::load_a(As, lda, r * 32, k - kb0 + a_kb0, lane);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+          if (j < ncb) {
+#pragma unroll
+            for (int r = 0; r < MB; ++r) acc[j][r] = PT::mma(a[r], ring[p][j], acc[j][r]);
+            if (k + P < kb1) ring[p][j] = PT::load_b(Bp, KBtot, nb0 + j * nbstep, k + P, lane);
+          }
+        }
+      }
+    }
+  }
+}
+
+constexpr int PF_DEPTH = 8;  // B-fragment prefetch depth (k-steps)
+
 // --------------------------------------------------------------- flow ----
 struct FlowDev {
   FlowDims dm;
@@ -441,28 +480,44 @@ __global__ void __launch_bounds__(256) flow_pass_kernel(FlowDev f, PassArgs a) {
     // ---- GEMM1: h1 = tanh(W1c c + b1)
     {
       const int KB = kc / PT::KSTEP, NBt = h1p / 32;
-      for (int t = wave; t < MB * NBt; t += 4) {
-        int rb = t % MB, nb = t / MB;
-        f32x16_t acc = {};
-        mfma_block<PT>(a0, sm.la0, rb * 32, W + dm.pk_w1(l), KB, nb, 0, KB, 0, acc, lane);
-        const int col = nb * 32 + (lane & 31);
-        const float bb = bl[col];
+      for (int c0 = wave; c0 < NBt; c0 += 8) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH>(a0, sm.la0, W + dm.pk_w1(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) hb1[(rb * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(PT::act(acc[v] + bb));
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+            const float bb = bl[col];
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v)
+                hb1[(r * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(PT::act(acc[j][r][v] + bb));
+          }
+        }
       }
     }
     __syncthreads();
     // ---- GEMM2: h2 = tanh(W2 h1 + b2)
     {
       const int KB = h1p / PT::KSTEP, NBt = h2p / 32;
-      for (int t = wave; t < MB * NBt; t += 4) {
-        int rb = t % MB, nb = t / MB;
-        f32x16_t acc = {};
-        mfma_block<PT>(hb1, sm.lh1, rb * 32, W + dm.pk_w2(l), KB, nb, 0, KB, 0, acc, lane);
-        const int col = nb * 32 + (lane & 31);
-        const float bb = bl[h1p + col];
+      for (int c0 = wave; c0 < NBt; c0 += 8) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH>(hb1, sm.lh1, W + dm.pk_w2(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) hb2[(rb * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(PT::act(acc[v] + bb));
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+            const float bb = bl[h1p + col];
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v)
+                hb2[(r * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(PT::act(acc[j][r][v] + bb));
+          }
+        }
       }
     }
     __syncthreads();
@@ -471,14 +526,22 @@ __global__ void __launch_bounds__(256) flow_pass_kernel(FlowDev f, PassArgs a) {
     for (int c = 0; c < nch; ++c) {
       const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
       const int KB = h2p / PT::KSTEP, NBt = nc / 32;
-      for (int t = wave; t < MB * NBt; t += 4) {
-        int rb = t % MB, nbl = t / MB;
-        f32x16_t acc = {};
-        mfma_block<PT>(hb2, sm.lh2, rb * 32, W + dm.pk_w3(l), KB, coff / 32 + nbl, 0, KB, 0, acc, lane);
-        const int col = nbl * 32 + (lane & 31);
-        const float bb = bl[h1p + h2p + coff + col];
+      for (int c0 = wave; c0 < NBt; c0 += 8) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH>(hb2, sm.lh2, W + dm.pk_w3(l), KB, coff / 32 + c0, 4, ncb, 0, KB, 0, acc,
+                                        lane);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) obuf[(rb * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[v] + bb;
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+            const float bb = bl[h1p + h2p + coff + col];
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v) obuf[(r * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[j][r][v] + bb;
+          }
+        }
       }
       __syncthreads();
       for (int i = tid; i < MT * cnt; i += 256) {
@@ -607,7 +670,7 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   // wave owns the same tiles in the forward recompute and in the adjoint, so
   // the FP32 tanh derivatives 1 - h^2 stay in its registers (the LDS copy of
   // h may be BF16, whose rounding would wreck 1 - h^2 near saturation).
-  constexpr int NTW = MB * (HMAX / 32) / 4;
+  constexpr int NCW = (HMAX / 32) / 4;  // column blocks per wave (even)
   extern __shared__ __align__(16) unsigned char smem[];
   const FlowDims& dm = f.dm;
   BwdSmem<PT> sm(dm, MT);
@@ -650,23 +713,33 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   }
   __syncthreads();
   store_fm<T>(st, a.ld, (int)dm.st_c(l), kc, a0, la0, m0, MT);
-  f32x16_t dh1[NTW], dh2[NTW];
+  // wave w owns column blocks w, w+4, w+8, ... (all MB row blocks) of every
+  // [MT x hidden] product below; index ci of the register arrays is the
+  // wave-local column block.
+  f32x16_t dh1[NCW][MB], dh2[NCW][MB];
   {
     const int KB = kc / PT::KSTEP, NBt = h1p / 32;
 #pragma unroll
-    for (int i = 0; i < NTW; ++i) {
-      const int t = wave + 4 * i;
-      if (t < MB * NBt) {
-        int rb = t % MB, nb = t / MB;
-        f32x16_t acc = {};
-        mfma_block<PT>(a0, la0, rb * 32, W + dm.pk_w1(l), KB, nb, 0, KB, 0, acc, lane);
-        const int col = nb * 32 + (lane & 31);
-        const float bb = bl[col];
+    for (int ci = 0; ci < NCW; ci += 2) {
+      const int c0 = wave + 4 * ci;
+      if (c0 < NBt) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH>(a0, la0, W + dm.pk_w1(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          float h = PT::act(acc[v] + bb);
-          hb1[(rb * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(h);
-          dh1[i][v] = 1.f - h * h;
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+            const float bb = bl[col];
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v) {
+                float h = PT::act(acc[j][r][v] + bb);
+                hb1[(r * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(h);
+                dh1[ci + j][r][v] = 1.f - h * h;
+              }
+          }
         }
       }
     }
@@ -676,19 +749,26 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   {
     const int KB = h1p / PT::KSTEP, NBt = h2p / 32;
 #pragma unroll
-    for (int i = 0; i < NTW; ++i) {
-      const int t = wave + 4 * i;
-      if (t < MB * NBt) {
-        int rb = t % MB, nb = t / MB;
-        f32x16_t acc = {};
-        mfma_block<PT>(hb1, sm.lh1, rb * 32, W + dm.pk_w2(l), KB, nb, 0, KB, 0, acc, lane);
-        const int col = nb * 32 + (lane & 31);
-        const float bb = bl[h1p + col];
+    for (int ci = 0; ci < NCW; ci += 2) {
+      const int c0 = wave + 4 * ci;
+      if (c0 < NBt) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH>(hb1, sm.lh1, W + dm.pk_w2(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          float h = PT::act(acc[v] + bb);
-          hb2[(rb * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(h);
-          dh2[i][v] = 1.f - h * h;
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+            const float bb = bl[h1p + col];
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v) {
+                float h = PT::act(acc[j][r][v] + bb);
+                hb2[(r * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(h);
+                dh2[ci + j][r][v] = 1.f - h * h;
+              }
+          }
         }
       }
     }
@@ -697,24 +777,34 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   store_fm<T>(st, a.ld, (int)dm.st_h2(l), h2p, hb2, sm.lh2, m0, MT);
 
   // ---- chunks: params -> spline VJP -> gp; g_h2 += gp W3c (register accumulators)
-  const int ntiles_h2 = MB * (h2p / 32);
-  f32x16_t gh2[NTW];
+  f32x16_t gh2[NCW][MB];
 #pragma unroll
-  for (int i = 0; i < NTW; ++i) gh2[i] = f32x16_t{};
+  for (int ci = 0; ci < NCW; ++ci)
+#pragma unroll
+    for (int r = 0; r < MB; ++r) gh2[ci][r] = f32x16_t{};
   const int nch = dm.n_chunks(l);
   const int KB3 = h2p / PT::KSTEP;
   const int KB3t = dm.n3p(l) / PT::KSTEP;
+  const int NB2 = h2p / 32;
   for (int c = 0; c < nch; ++c) {
     const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
     const int NBt = nc / 32;
-    for (int t = wave; t < MB * NBt; t += 4) {
-      int rb = t % MB, nbl = t / MB;
-      f32x16_t acc = {};
-      mfma_block<PT>(hb2, sm.lh2, rb * 32, W + dm.pk_w3(l), KB3, coff / 32 + nbl, 0, KB3, 0, acc, lane);
-      const int col = nbl * 32 + (lane & 31);
-      const float bb = bl[h1p + h2p + coff + col];
+    for (int c0 = wave; c0 < NBt; c0 += 8) {
+      const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+      f32x16_t acc[2][MB] = {};
+      mfma_panel<PT, MB, 2, PF_DEPTH>(hb2, sm.lh2, W + dm.pk_w3(l), KB3, coff / 32 + c0, 4, ncb, 0, KB3, 0, acc,
+                                      lane);
 #pragma unroll
-      for (int v = 0; v < 16; ++v) obuf[(rb * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[v] + bb;
+      for (int j = 0; j < 2; ++j) {
+        if (j < ncb) {
+          const int col = (c0 + 4 * j) * 32 + (lane & 31);
+          const float bb = bl[h1p + h2p + coff + col];
+#pragma unroll
+          for (int r = 0; r < MB; ++r)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) obuf[(r * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[j][r][v] + bb;
+        }
+      }
     }
     __syncthreads();
     for (int i = tid; i < MT * cnt; i += 256) {
@@ -735,27 +825,35 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
     store_fm<T>(st, a.ld, (int)dm.st_gp(l) + coff, nc, gpb, sm.lg, m0, MT);
     const int kb0 = coff / PT::KSTEP, kb1 = (coff + nc) / PT::KSTEP;
 #pragma unroll
-    for (int i = 0; i < NTW; ++i) {
-      int t = wave + 4 * i;
-      if (t < ntiles_h2) {
-        int rb = t % MB, nb = t / MB;
-        mfma_block<PT>(gpb, sm.lg, rb * 32, W + dm.pk_w3t(l), KB3t, nb, kb0, kb1, 0, gh2[i], lane);
+    for (int ci = 0; ci < NCW; ci += 2) {
+      const int c0 = wave + 4 * ci;
+      if (c0 < NB2) {
+        const int ncb = (c0 + 4 < NB2) ? 2 : 1;
+        f32x16_t acc[2][MB];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < MB; ++r) acc[j][r] = gh2[ci + j][r];
+        mfma_panel<PT, MB, 2, PF_DEPTH>(gpb, sm.lg, W + dm.pk_w3t(l), KB3t, c0, 4, ncb, kb0, kb1, 0, acc, lane);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < MB; ++r) gh2[ci + j][r] = acc[j][r];
       }
     }
     __syncthreads();
   }
   // ---- g_pre2 = g_h2 * (1 - h2^2)   (in place of h2)
 #pragma unroll
-  for (int i = 0; i < NTW; ++i) {
-    int t = wave + 4 * i;
-    if (t < ntiles_h2) {
-      int rb = t % MB, nb = t / MB;
+  for (int ci = 0; ci < NCW; ++ci) {
+    const int nb = wave + 4 * ci;
+    if (nb < NB2) {
       const int col = nb * 32 + (lane & 31);
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        int r = rb * 32 + out_row(v, lane);
-        hb2[r * sm.lh2 + col] = PT::cvt(gh2[i][v] * dh2[i][v]);
-      }
+      for (int r = 0; r < MB; ++r)
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+          hb2[(r * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(gh2[ci][r][v] * dh2[ci][r][v]);
     }
   }
   __syncthreads();
@@ -764,17 +862,22 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   {
     const int KB = h2p / PT::KSTEP, NBt = h1p / 32;
 #pragma unroll
-    for (int i = 0; i < NTW; ++i) {
-      const int t = wave + 4 * i;
-      if (t < MB * NBt) {
-        int rb = t % MB, nb = t / MB;
-        f32x16_t acc = {};
-        mfma_block<PT>(hb2, sm.lh2, rb * 32, W + dm.pk_w2t(l), KB, nb, 0, KB, 0, acc, lane);
-        const int col = nb * 32 + (lane & 31);
+    for (int ci = 0; ci < NCW; ci += 2) {
+      const int c0 = wave + 4 * ci;
+      if (c0 < NBt) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH>(hb2, sm.lh2, W + dm.pk_w2t(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          int r = rb * 32 + out_row(v, lane);
-          hb1[r * sm.lh1 + col] = PT::cvt(acc[v] * dh1[i][v]);
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v)
+                hb1[(r * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(acc[j][r][v] * dh1[ci + j][r][v]);
+          }
         }
       }
     }
@@ -784,15 +887,22 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   // ---- g_c = g_pre1 W1c, scattered onto the conditioning dims
   {
     const int KB = h1p / PT::KSTEP, NBt = dm.ncp(l) / 32;
-    for (int t = wave; t < MB * NBt; t += 4) {
-      int rb = t % MB, nb = t / MB;
-      f32x16_t acc = {};
-      mfma_block<PT>(hb1, sm.lh1, rb * 32, W + dm.pk_w1t(l), KB, nb, 0, KB, 0, acc, lane);
-      const int k = nb * 32 + (lane & 31);
-      if (k < dc) {
-        const int j = dm.cond(l, k);
+    for (int c0 = wave; c0 < NBt; c0 += 8) {
+      const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+      f32x16_t acc[2][MB] = {};
+      mfma_panel<PT, MB, 2, PF_DEPTH>(hb1, sm.lh1, W + dm.pk_w1t(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) gb[(rb * 32 + out_row(v, lane)) * ldx + j] += acc[v];
+      for (int j = 0; j < 2; ++j) {
+        if (j < ncb) {
+          const int k = (c0 + 4 * j) * 32 + (lane & 31);
+          if (k < dc) {
+            const int jd = dm.cond(l, k);
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v) gb[(r * 32 + out_row(v, lane)) * ldx + jd] += acc[j][r][v];
+          }
+        }
       }
     }
   }
@@ -867,10 +977,29 @@ __global__ void __launch_bounds__(256) flow_wgrad_kernel(FlowDev f, int l, const
   const int nb = tt / ncolb, kb = tt - nb * ncolb;
   const T* st = reinterpret_cast<const T*>(store);
   f32x16_t acc = {};
-  for (int b = 0; b < ld; b += PT::KSTEP) {
-    auto ga = PT::load_g(st, ld, gRow + nb * 32, b, lane);
-    auto ab = PT::load_g(st, ld, aRow + kb * 32, b, lane);
-    acc = PT::mma(ga, ab, acc);
+  // both fragments stream from the (L2-resident) store: prefetch P steps ahead
+  constexpr int P = PF_DEPTH;
+  using GF = decltype(PT::load_g(st, ld, 0, 0, 0));
+  GF ga[P], ab[P];
+  const int nsteps = (int)(ld / PT::KSTEP);
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+    if (p < nsteps) {
+      ga[p] = PT::load_g(st, ld, gRow + nb * 32, p * PT::KSTEP, lane);
+      ab[p] = PT::load_g(st, ld, aRow + kb * 32, p * PT::KSTEP, lane);
+    }
+  for (int s0 = 0; s0 < nsteps; s0 += P) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int sidx = s0 + p;
+      if (sidx < nsteps) {
+        acc = PT::mma(ga[p], ab[p], acc);
+        if (sidx + P < nsteps) {
+          ga[p] = PT::load_g(st, ld, gRow + nb * 32, (sidx + P) * PT::KSTEP, lane);
+          ab[p] = PT::load_g(st, ld, aRow + kb * 32, (sidx + P) * PT::KSTEP, lane);
+        }
+      }
+    }
   }
   const int k = kb * 32 + (lane & 31);
 #pragma unroll
