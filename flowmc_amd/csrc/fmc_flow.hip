@@ -6,8 +6,8 @@
 //   flow_bwd_kernel       adjoint of one forward layer (recomputes conditioner
 //                         and spline bin parameters, hand-derived spline VJP),
 //                         stores feature-major activations / gradients
-//   flow_wgrad_kernel     weight gradients: batch reduction on MFMA
-//   flow_rowsum_kernel    bias gradients + ScalarAffine gradients
+//   flow_wgrad_kernel     weight, bias and ScalarAffine gradients: split batch
+//                         reduction on MFMA
 //   flow_pack_kernel      master FP32 parameters -> MFMA operand images
 //   adamw_* kernels       optax clip_by_global_norm(1) + adamw
 //
@@ -48,6 +48,7 @@ struct PF32 {
   }
   static __device__ __forceinline__ T cvt(float x) { return x; }
   static __device__ __forceinline__ float f(T x) { return x; }
+  static __device__ __forceinline__ float hsum(float x) { return x; }
   static __device__ __forceinline__ float act(float x) { return tanhf(x); }
 };
 
@@ -68,6 +69,10 @@ struct PBF16 {
   }
   static __device__ __forceinline__ T cvt(float x) { return (__bf16)x; }
   static __device__ __forceinline__ float f(T x) { return (float)x; }
+  static __device__ __forceinline__ float hsum(bf16x8_t x) {
+    return (((float)x[0] + (float)x[1]) + ((float)x[2] + (float)x[3])) +
+           (((float)x[4] + (float)x[5]) + ((float)x[6] + (float)x[7]));
+  }
   static __device__ __forceinline__ float act(float x) {
     float e = __expf(2.f * x);
     return 1.f - 2.f / (e + 1.f);
@@ -92,7 +97,9 @@ __device__ __forceinline__ void mfma_block(const typename PT::T* As, int lda, in
 // every A fragment feeds CB MFMAs, every B fragment MB MFMAs, and the B
 // fragments (global / L2) are prefetched P k-steps ahead in a register ring
 // so their latency hides behind the MFMAs of the previous steps.
-template <class PT, int MB, int CB, int P>
+// TR = true swaps the operands: the packed image is the A operand (output
+// rows = its rows), the LDS rows are the B operand (output columns).
+template <class PT, int MB, int CB, int P, bool TR = false>
 __device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, const typename PT::T* Bp, int KBtot,
                                            int nb0, int nbstep, int ncb, int kb0, int kb1, int a_kb0,
                                            f32x16_t (&acc)[CB][MB], int lane) {
@@ -116,7 +123,8 @@ __device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, co
         for (int j = 0; j < CB; ++j) {
           if (j < ncb) {
 #pragma unroll
-            for (int r = 0; r < MB; ++r) acc[j][r] = PT::mma(a[r], ring[p][j], acc[j][r]);
+            for (int r = 0; r < MB; ++r)
+              acc[j][r] = TR ? PT::mma(ring[p][j], a[r], acc[j][r]) : PT::mma(a[r], ring[p][j], acc[j][r]);
             if (k + P < kb1) ring[p][j] = PT::load_b(Bp, KBtot, nb0 + j * nbstep, k + P, lane);
           }
         }
@@ -358,6 +366,242 @@ __device__ float spline_vjp(const float* pp, const FlowDev& f, float x, float gy
   return g_x;
 }
 
+// ------------------------------------------------ register spline -----
+// The same spline with its 3K+1 parameters in registers, in the slot order of
+// the register-spline W3 layout (widths at 0..9, heights at 10..19, slopes of
+// knots 0..K-1 at 20..29, slope of knot K at 30; K <= 10). Every loop is
+// unrolled over the 10 slots and predicated on K, so all register indices are
+// static. The bin search keeps the first bin whose right knot exceeds the
+// argument, as spline_fwd / spline_inv do.
+struct RsBin {
+  float xk, xk1, yk, yk1, uk, uk1;
+};
+
+__device__ __forceinline__ float rs_softplus(float v) { return fmaxf(v, 0.f) + __logf(1.f + __expf(-fabsf(v))); }
+
+// knots of the bin holding t (t along x when INV is false, along y otherwise)
+template <bool INV>
+__device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int K, const FlowDev& f, float t) {
+  float mw = -INFINITY, mh = -INFINITY;
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+    if (i < K) {
+      mw = fmaxf(mw, pp[i]);
+      mh = fmaxf(mh, pp[10 + i]);
+    }
+  float ew[10], eh[10], sw = 0.f, sh = 0.f;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    ew[i] = (i < K) ? __expf(pp[i] - mw) : 0.f;
+    eh[i] = (i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+    sw += ew[i];
+    sh += eh[i];
+  }
+  const float span = (f.hi - f.lo) - K * f.mb;
+  const float cw = span / sw, ch = span / sh;
+  RsBin b{f.lo, f.hi, f.lo, f.hi, pp[20], pp[21]};
+  float xc = f.lo, yc = f.lo;
+  bool found = false;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    if (i < K) {
+      const float xn = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
+      const float yn = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
+      const bool hit = !found && (INV ? (t < yn) : (t < xn));
+      if (hit) {
+        b.xk = xc;
+        b.yk = yc;
+        b.xk1 = xn;
+        b.yk1 = yn;
+        b.uk = pp[20 + i];
+        b.uk1 = (i == K - 1) ? pp[30] : pp[21 + i];
+      }
+      found = found || hit;
+      xc = xn;
+      yc = yn;
+    }
+  }
+  return b;
+}
+
+__device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowDev& f, float x, float& y, float& ld) {
+  if (x <= f.lo || x >= f.hi) {
+    const float u = (x <= f.lo) ? pp[20] : pp[30];
+    const float e = (x <= f.lo) ? f.lo : f.hi;
+    const float dd = rs_softplus(u + f.off) + f.ms;
+    y = (x - e) * dd + e;
+    ld = __logf(dd);
+    return;
+  }
+  const RsBin b = rs_bin<false>(pp, K, f, x);
+  const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
+  const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = hh / w;
+  const float z = fminf(fmaxf((x - b.xk) / w, 0.f), 1.f);
+  const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
+  const float st = dk1 + dk - 2.f * s;
+  const float num = hh * (s * z2 + dk * zz);
+  const float den = s + st * zz;
+  y = b.yk + num / den;
+  ld = 2.f * __logf(s) + __logf(dk1 * z2 + 2.f * s * zz + dk * q) - 2.f * __logf(den);
+}
+
+__device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowDev& f, float yv, float& x, float& ld) {
+  if (yv <= f.lo || yv >= f.hi) {
+    const float u = (yv <= f.lo) ? pp[20] : pp[30];
+    const float e = (yv <= f.lo) ? f.lo : f.hi;
+    const float dd = rs_softplus(u + f.off) + f.ms;
+    x = (yv - e) / dd + e;
+    ld = -__logf(dd);
+    return;
+  }
+  const RsBin b = rs_bin<true>(pp, K, f, yv);
+  const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
+  const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = hh / w;
+  const float wv = fminf(fmaxf((yv - b.yk) / hh, 0.f), 1.f);
+  const float st = dk1 + dk - 2.f * s;
+  const float c = -s * wv;
+  const float bq = dk - st * wv;
+  const float a = s - bq;
+  const float disc = bq * bq - 4.f * a * c;
+  const float sq = disc > 0.f ? sqrtf(fmaxf(disc, 1.17549435e-38f)) : 0.f;
+  float z = (bq >= 0.f) ? (2.f * c) / (-bq - sq) : (-bq + sq) / (2.f * a);
+  z = fminf(fmaxf(z, 0.f), 1.f);
+  x = w * z + b.xk;
+  const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
+  const float den = s + st * zz;
+  ld = -2.f * __logf(s) - __logf(dk1 * z2 + 2.f * s * zz + dk * q) + 2.f * __logf(den);
+}
+
+// Register-spline forward + vector-Jacobian product (the reverse sweep of
+// spline_vjp): given dL/dy (gy) and dL/dlogdet (gl), writes dL/d(slot
+// logits) to go (zero in padding slots) and returns dL/dx.
+__device__ __forceinline__ float rs_vjp(const float (&pp)[32], int K, const FlowDev& f, float x, float gy, float gl,
+                                        float (&go)[32]) {
+#pragma unroll
+  for (int i = 0; i < 32; ++i) go[i] = 0.f;
+  if (x <= f.lo || x >= f.hi) {
+    const bool left = x <= f.lo;
+    const float u = (left ? pp[20] : pp[30]) + f.off;
+    const float dd = rs_softplus(u) + f.ms;
+    const float gd = gy * (x - (left ? f.lo : f.hi)) + gl / dd;
+    const float gu = gd / (1.f + __expf(-u));
+    go[20] = left ? gu : 0.f;
+    go[30] = left ? 0.f : gu;
+    return gy * dd;
+  }
+  float mw = -INFINITY, mh = -INFINITY;
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+    if (i < K) {
+      mw = fmaxf(mw, pp[i]);
+      mh = fmaxf(mh, pp[10 + i]);
+    }
+  float ew[10], eh[10], sw = 0.f, sh = 0.f;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    ew[i] = (i < K) ? __expf(pp[i] - mw) : 0.f;
+    eh[i] = (i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+    sw += ew[i];
+    sh += eh[i];
+  }
+  const float span = (f.hi - f.lo) - K * f.mb;
+  const float cw = span / sw, ch = span / sh, isw = 1.f / sw, ish = 1.f / sh;
+  float xk = f.lo, yk = f.lo, xk1 = f.hi, yk1 = f.hi, uk = pp[20], uk1 = pp[21];
+  float pw_lt = 0.f, ph_lt = 0.f, pwk = 0.f, phk = 0.f;  // softmax mass of bins < k, and of bin k
+  float xc = f.lo, yc = f.lo;
+  int k = 0;
+  bool found = false;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    if (i < K) {
+      const float xn = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
+      const float yn = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
+      const bool hit = !found && (x < xn);
+      if (hit) {
+        k = i;
+        xk = xc;
+        yk = yc;
+        xk1 = xn;
+        yk1 = yn;
+        uk = pp[20 + i];
+        uk1 = (i == K - 1) ? pp[30] : pp[21 + i];
+        pwk = ew[i] * isw;
+        phk = eh[i] * ish;
+      }
+      if (!found && !hit) {
+        pw_lt += ew[i] * isw;
+        ph_lt += eh[i] * ish;
+      }
+      found = found || hit;
+      xc = xn;
+      yc = yn;
+    }
+  }
+  uk += f.off;
+  uk1 += f.off;
+  const float dk = rs_softplus(uk) + f.ms, dk1 = rs_softplus(uk1) + f.ms;
+  const float w = xk1 - xk, hh = yk1 - yk, s = hh / w;
+  const float t = (x - xk) / w;
+  const float z = fminf(fmaxf(t, 0.f), 1.f);
+  const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
+  const float st = dk1 + dk - 2.f * s;
+  const float num = hh * (s * z2 + dk * zz);
+  const float den = s + st * zz;
+  const float A = dk1 * z2 + 2.f * s * zz + dk * q;
+  // reverse sweep of y = yk + num/den, ld = 2 log s + log A - 2 log den
+  const float g_num = gy / den;
+  const float g_den = -gy * num / (den * den) - 2.f * gl / den;
+  float g_s = 2.f * gl / s;
+  const float g_A = gl / A;
+  float g_yk = gy;
+  float g_dk1 = g_A * z2, g_z2 = g_A * dk1, g_zz = g_A * 2.f * s, g_dk = g_A * q;
+  const float g_q = g_A * dk;
+  g_s += g_A * 2.f * zz;
+  g_s += g_den;
+  const float g_st = g_den * zz;
+  g_zz += g_den * st;
+  float g_hh = g_num * (s * z2 + dk * zz);
+  g_s += g_num * hh * z2;
+  g_z2 += g_num * hh * s;
+  g_dk += g_num * hh * zz;
+  g_zz += g_num * hh * dk;
+  g_dk1 += g_st;
+  g_dk += g_st;
+  g_s += -2.f * g_st;
+  g_z2 -= g_zz;
+  const float g_z = g_zz + g_z2 * 2.f * z - g_q * 2.f * (1.f - z);
+  const float g_t = (t > 0.f && t < 1.f) ? g_z : 0.f;
+  const float g_x = g_t / w;
+  float g_xk = -g_t / w;
+  float g_w = -g_t * t / w;
+  g_hh += g_s / w;
+  g_w += -g_s * s / w;
+  const float g_yk1 = g_hh;
+  g_yk -= g_hh;
+  const float g_xk1 = g_w;
+  g_xk -= g_w;
+  // knots -> softmax logits: x_i = lo + sum_{m<i} w_m for 1 <= i <= K-1
+  const float G1x = (k >= 1) ? g_xk : 0.f, G2x = (k + 1 <= K - 1) ? g_xk1 : 0.f;
+  const float G1y = (k >= 1) ? g_yk : 0.f, G2y = (k + 1 <= K - 1) ? g_yk1 : 0.f;
+  const float dotw = span * (G1x * pw_lt + G2x * (pw_lt + pwk));
+  const float doth = span * (G1y * ph_lt + G2y * (ph_lt + phk));
+#pragma unroll
+  for (int m = 0; m < 10; ++m) {
+    if (m < K) {
+      const float gwm = span * ((m < k ? G1x : 0.f) + (m <= k ? G2x : 0.f));
+      const float ghm = span * ((m < k ? G1y : 0.f) + (m <= k ? G2y : 0.f));
+      go[m] = ew[m] * isw * (gwm - dotw);
+      go[10 + m] = eh[m] * ish * (ghm - doth);
+    }
+  }
+  const float gk = g_dk / (1.f + __expf(-uk)), gk1 = g_dk1 / (1.f + __expf(-uk1));
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+    if (i < K) go[20 + i] = (i == k ? gk : 0.f) + (i == k + 1 ? gk1 : 0.f);
+  go[30] = (k + 1 == K) ? gk1 : 0.f;
+  return g_x;
+}
+
 // ------------------------------------------------------- pass kernel -----
 struct PassArgs {
   int dir;    // 0 forward, 1 inverse (reference layer order), 2 exact inverse
@@ -391,8 +635,8 @@ struct PassSmem {
     o += rup(MT * ldx * 4, 16);
     ldr = o;
     o += rup(MT * 4, 16);
-    ldp = o;
-    o += rup(MT * dm.G() * 4, 16);
+    ldp = o;  // per-dim log-dets of a chunk, or the 4 waves' partial log-dets
+    o += rup(MT * (dm.G() > 4 ? dm.G() : 4) * 4, 16);
     r1 = o;
     int s1 = MT * lh1 * (int)sizeof(typename PT::T), s1b = MT * lo_ * 4;
     o += rup(s1 > s1b ? s1 : s1b, 16);
@@ -431,7 +675,8 @@ __global__ void __launch_bounds__(256) flow_pass_kernel(FlowDev f, PassArgs a) {
     if (row < a.n) {
       if ((a.flags & FL_SAMPLE) && a.x_in == nullptr) {
         float4 z = philox_normal4(a.key0, a.key1, (uint32_t)(j >> 2), (uint32_t)(a.sample0 + row), 0u, RNG_FLOW_BASE);
-        v = f4c(z, j & 3);
+        const int e = j & 3;
+        v = (e == 0) ? z.x : (e == 1) ? z.y : (e == 2) ? z.z : z.w;
       } else {
         v = a.x_in[row * d + j];
       }
@@ -521,8 +766,63 @@ __global__ void __launch_bounds__(256) flow_pass_kernel(FlowDev f, PassArgs a) {
       }
     }
     __syncthreads();
-    // ---- GEMM3 in chunks of G transformed dims + spline epilogue
-    const int nch = dm.n_chunks(l), P = dm.P();
+    // ---- GEMM3 (transposed, register-spline order) + spline in registers
+    if (dm.reg_spline()) {
+      const int npair = dm.n_pairs(l), KB = h2p / PT::KSTEP, dtl = dm.dt(l), K = dm.K;
+      const T* W3r = W + dm.pk_w3r(l);
+      const float* b3r = bl + dm.pb_b3r(l);
+      float* ldw = ldp;  // [4 waves][MT] partial log-dets
+      float ldacc[MB];
+#pragma unroll
+      for (int r = 0; r < MB; ++r) ldacc[r] = 0.f;
+      const int hf = lane >> 5;
+      for (int q = wave; q < npair; q += 4) {
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH, true>(hb2, sm.lh2, W3r, KB, 2 * q, 1, 2, 0, KB, 0, acc, lane);
+        const int k = 2 * q + hf;
+        if (k < dtl) {
+          const int j = dm.trans(l, k);
+          const float4* bp = reinterpret_cast<const float4*>(b3r + q * 64 + hf * 32);
+          float bias[32];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const float4 b4 = bp[u];
+            bias[4 * u] = b4.x;
+            bias[4 * u + 1] = b4.y;
+            bias[4 * u + 2] = b4.z;
+            bias[4 * u + 3] = b4.w;
+          }
+#pragma unroll
+          for (int r = 0; r < MB; ++r) {
+            float pp[32];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+              pp[v] = acc[0][r][v] + bias[v];
+              pp[16 + v] = acc[1][r][v] + bias[16 + v];
+            }
+            const int m = r * 32 + (lane & 31);
+            const float xv = xs[m * ldx + j];
+            float yv, ldv;
+            if (a.dir == 0)
+              rs_fwd(pp, K, f, xv, yv, ldv);
+            else
+              rs_inv(pp, K, f, xv, yv, ldv);
+            xs[m * ldx + j] = yv;
+            ldacc[r] += ldv;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < MB; ++r) {
+        const float o = __shfl_xor(ldacc[r], 32, 64);
+        if (lane < 32) ldw[wave * MT + r * 32 + lane] = ldacc[r] + o;
+      }
+      __syncthreads();
+      for (int m = tid; m < MT; m += 256) ldr[m] += (ldw[m] + ldw[MT + m]) + (ldw[2 * MT + m] + ldw[3 * MT + m]);
+      __syncthreads();
+    }
+    // ---- GEMM3 in chunks of G transformed dims + spline epilogue (K > 10)
+    const int nch = dm.reg_spline() ? 0 : dm.n_chunks(l), P = dm.P();
     for (int c = 0; c < nch; ++c) {
       const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
       const int KB = h2p / PT::KSTEP, NBt = nc / 32;
@@ -629,7 +929,9 @@ struct BwdSmem {
     lh2 = dm.h2p() + PT::PAD;
     lo_ = dm.ncf() + 1;
     int kcm = dm.kc(1) > dm.kc(0) ? dm.kc(1) : dm.kc(0);
-    lg = (dm.ncf() > kcm ? dm.ncf() : kcm) + PT::PAD;
+    // gradient staging: one chunk (chunked path) or 4 dim pairs of 64 rows
+    const int gw = dm.reg_spline() ? 256 : dm.ncf();
+    lg = (gw > kcm ? gw : kcm) + PT::PAD;
     const int tb = (int)sizeof(typename PT::T);
     int o = 0;
     xs = o;
@@ -642,8 +944,8 @@ struct BwdSmem {
     o += rup(MT * lh1 * tb, 16);
     hb2 = o;
     o += rup(MT * lh2 * tb, 16);
-    ob = o;
-    o += rup(MT * lo_ * 4, 16);
+    ob = o;  // spline parameters of a chunk (chunked path only)
+    o += dm.reg_spline() ? 0 : rup(MT * lo_ * 4, 16);
     gpb = o;
     o += rup(MT * lg * tb, 16);
     red = o;
@@ -782,10 +1084,86 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   for (int ci = 0; ci < NCW; ++ci)
 #pragma unroll
     for (int r = 0; r < MB; ++r) gh2[ci][r] = f32x16_t{};
-  const int nch = dm.n_chunks(l);
+  const int NB2 = h2p / 32;
+  if (dm.reg_spline()) {
+    // rounds of 4 dim pairs (one per wave): transposed GEMM3 -> parameters in
+    // registers -> spline VJP in registers -> gp (gradient order) in LDS ->
+    // gradient store + g_h2 += gp W3
+    const int npair = dm.n_pairs(l), dtl = dm.dt(l), K = dm.K;
+    const T* W3r = W + dm.pk_w3r(l);
+    const T* W3rt = W + dm.pk_w3rt(l);
+    const float* b3r = bl + dm.pb_b3r(l);
+    const int KB3 = h2p / PT::KSTEP, KB3t = dm.n3r(l) / PT::KSTEP;
+    const int hf = lane >> 5;
+    for (int q0 = 0; q0 < npair; q0 += 4) {
+      const int nq = (npair - q0) < 4 ? (npair - q0) : 4;
+      if (wave < nq) {
+        const int q = q0 + wave;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH, true>(hb2, sm.lh2, W3r, KB3, 2 * q, 1, 2, 0, KB3, 0, acc, lane);
+        const int k = 2 * q + hf;
+        const bool valid = k < dtl;
+        const int j = valid ? dm.trans(l, k) : 0;
+        const float4* bp = reinterpret_cast<const float4*>(b3r + q * 64 + hf * 32);
+        float bias[32];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float4 b4 = bp[u];
+          bias[4 * u] = b4.x;
+          bias[4 * u + 1] = b4.y;
+          bias[4 * u + 2] = b4.z;
+          bias[4 * u + 3] = b4.w;
+        }
+#pragma unroll
+        for (int r = 0; r < MB; ++r) {
+          float pp[32], go[32];
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            pp[v] = acc[0][r][v] + bias[v];
+            pp[16 + v] = acc[1][r][v] + bias[16 + v];
+          }
+          const int m = r * 32 + (lane & 31);
+          if (valid) {
+            const float gxv = rs_vjp(pp, K, f, xs[m * ldx + j], gb[m * ldx + j], glp[m], go);
+            gb[m * ldx + j] = gxv;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 32; ++i) go[i] = 0.f;
+          }
+          // slot 8t+e of this lane's dim -> gradient-order column 64 wave + 16t + 8hf + e
+          T* dst = gpb + m * sm.lg + 64 * wave + 8 * hf;
+#pragma unroll
+          for (int t4 = 0; t4 < 4; ++t4)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) dst[16 * t4 + e] = PT::cvt(go[8 * t4 + e]);
+        }
+      }
+      __syncthreads();
+      store_fm<T>(st, a.ld, (int)dm.st_gp(l) + 64 * q0, 64 * nq, gpb, sm.lg, m0, MT);
+      const int kb0 = 64 * q0 / PT::KSTEP, kb1 = 64 * (q0 + nq) / PT::KSTEP;
+#pragma unroll
+      for (int ci = 0; ci < NCW; ci += 2) {
+        const int c0 = wave + 4 * ci;
+        if (c0 < NB2) {
+          const int ncb = (c0 + 4 < NB2) ? 2 : 1;
+          f32x16_t acc[2][MB];
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < MB; ++r) acc[j][r] = gh2[ci + j][r];
+          mfma_panel<PT, MB, 2, PF_DEPTH>(gpb, sm.lg, W3rt, KB3t, c0, 4, ncb, kb0, kb1, 0, acc, lane);
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < MB; ++r) gh2[ci + j][r] = acc[j][r];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int nch = dm.reg_spline() ? 0 : dm.n_chunks(l);
   const int KB3 = h2p / PT::KSTEP;
   const int KB3t = dm.n3p(l) / PT::KSTEP;
-  const int NB2 = h2p / 32;
   for (int c = 0; c < nch; ++c) {
     const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
     const int NBt = nc / 32;
@@ -933,32 +1311,59 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
       s0 += red[2 * w];
       s1 += red[2 * w + 1];
     }
-    a.aff_part[2 * blockIdx.x] = s0;
-    a.aff_part[2 * blockIdx.x + 1] = s1;
+    // partials at a fixed 32-sample granularity: slot MB*block holds the
+    // block's sums, the other MB-1 slots of the block hold zeros
+    a.aff_part[2 * MB * blockIdx.x] = s0;
+    a.aff_part[2 * MB * blockIdx.x + 1] = s1;
+#pragma unroll
+    for (int r = 1; r < MB; ++r) {
+      a.aff_part[2 * (MB * blockIdx.x + r)] = 0.f;
+      a.aff_part[2 * (MB * blockIdx.x + r) + 1] = 0.f;
+    }
   }
 }
 
 // --------------------------------------------------- weight gradients ----
-// One wavefront per 32x32 tile of gW = G^T Act over the batch (both operands
-// feature-major in the store). Problems: 0 = W1c, 1 = W2, 2 = W3 (chunked).
+// One workgroup of WG_SPLIT wavefronts per 32x32 tile of gW = G^T Act: the
+// batch reduction is split over the wavefronts (each takes an interleaved
+// share of the KSTEP-wide batch slices) and the partial tiles are summed in
+// LDS in a fixed order. Both operands are feature-major in the store.
+// Problems: 0 = W1c, 1 = W2, 2 = W3 (chunked). Tiles of the first column
+// block (kb == 0) also sum their G rows into the bias gradients. The last
+// workgroup sums the ScalarAffine partials.
+constexpr int WG_SPLIT = 8;
+
 template <class PT>
-__global__ void __launch_bounds__(256) flow_wgrad_kernel(FlowDev f, int l, const void* store, int64_t ld,
-                                                         float* grad) {
+__global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, int l, const void* store, int64_t ld,
+                                                                   const float* aff_part, int n_part, float* grad) {
   using T = typename PT::T;
+  __shared__ float red[WG_SPLIT][16][64];
+  __shared__ float rsum[WG_SPLIT][64];
   const FlowDims& dm = f.dm;
-  const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int tt = blockIdx.x;
   const int h1p = dm.h1p(), h2p = dm.h2p();
-  const int n0t = (h1p / 32) * (dm.ncp(l) / 32), n1t = (h2p / 32) * (h1p / 32), n2t = (dm.n3p(l) / 32) * (h2p / 32);
-  int prob, tt = tile;
+  const int n0t = (h1p / 32) * (dm.ncp(l) / 32), n1t = (h2p / 32) * (h1p / 32), n2t = (dm.n3(l) / 32) * (h2p / 32);
+  if (tt == n0t + n1t + n2t) {  // ScalarAffine gradients: fixed-order sum of the partials
+    if (threadIdx.x == 0) {
+      float s0 = 0.f, s1 = 0.f;
+      for (int i = 0; i < n_part; ++i) {
+        s0 += aff_part[2 * i];
+        s1 += aff_part[2 * i + 1];
+      }
+      grad[dm.th_scale(l)] = s0;
+      grad[dm.th_shift(l)] = s1;
+    }
+    return;
+  }
+  int prob;
   if (tt < n0t) {
     prob = 0;
   } else if ((tt -= n0t) < n1t) {
     prob = 1;
-  } else if ((tt -= n1t) < n2t) {
-    prob = 2;
   } else {
-    return;
+    tt -= n1t;
+    prob = 2;
   }
   int gRow, aRow, ncolb;
   if (prob == 0) {
@@ -975,92 +1380,88 @@ __global__ void __launch_bounds__(256) flow_wgrad_kernel(FlowDev f, int l, const
     ncolb = h2p / 32;
   }
   const int nb = tt / ncolb, kb = tt - nb * ncolb;
+  const bool want_bias = (kb == 0);
   const T* st = reinterpret_cast<const T*>(store);
   f32x16_t acc = {};
-  // both fragments stream from the (L2-resident) store: prefetch P steps ahead
+  float bsum = 0.f;
+  // this wave takes batch slices wave, wave + WG_SPLIT, ... ; fragments
+  // (L2-resident store) are prefetched P slices ahead
   constexpr int P = PF_DEPTH;
   using GF = decltype(PT::load_g(st, ld, 0, 0, 0));
   GF ga[P], ab[P];
   const int nsteps = (int)(ld / PT::KSTEP);
+  const int my = (nsteps - wave + WG_SPLIT - 1) / WG_SPLIT;  // slices of this wave
 #pragma unroll
   for (int p = 0; p < P; ++p)
-    if (p < nsteps) {
-      ga[p] = PT::load_g(st, ld, gRow + nb * 32, p * PT::KSTEP, lane);
-      ab[p] = PT::load_g(st, ld, aRow + kb * 32, p * PT::KSTEP, lane);
+    if (p < my) {
+      const int sl = wave + p * WG_SPLIT;
+      ga[p] = PT::load_g(st, ld, gRow + nb * 32, sl * PT::KSTEP, lane);
+      ab[p] = PT::load_g(st, ld, aRow + kb * 32, sl * PT::KSTEP, lane);
     }
-  for (int s0 = 0; s0 < nsteps; s0 += P) {
+  for (int s0 = 0; s0 < my; s0 += P) {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      const int sidx = s0 + p;
-      if (sidx < nsteps) {
+      const int i = s0 + p;
+      if (i < my) {
         acc = PT::mma(ga[p], ab[p], acc);
-        if (sidx + P < nsteps) {
-          ga[p] = PT::load_g(st, ld, gRow + nb * 32, (sidx + P) * PT::KSTEP, lane);
-          ab[p] = PT::load_g(st, ld, aRow + kb * 32, (sidx + P) * PT::KSTEP, lane);
+        if (want_bias) bsum += PT::hsum(ga[p]);
+        if (i + P < my) {
+          const int sl = wave + (i + P) * WG_SPLIT;
+          ga[p] = PT::load_g(st, ld, gRow + nb * 32, sl * PT::KSTEP, lane);
+          ab[p] = PT::load_g(st, ld, aRow + kb * 32, sl * PT::KSTEP, lane);
         }
       }
     }
   }
-  const int k = kb * 32 + (lane & 31);
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int n = nb * 32 + out_row(v, lane);
-    int64_t idx = -1;
-    if (prob == 0) {
-      if (n < dm.h1 && k < dm.dc(l)) idx = dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k);
-    } else if (prob == 1) {
-      if (n < dm.h2 && k < dm.h1) idx = dm.th_w2(l) + (int64_t)n * dm.h1 + k;
-    } else {
-      int row = dm.w3_row(l, n);
-      if (row >= 0 && k < dm.h2) idx = dm.th_w3(l) + (int64_t)row * dm.h2 + k;
+  for (int v = 0; v < 16; ++v) red[wave][v][lane] = acc[v];
+  rsum[wave][lane] = bsum;
+  __syncthreads();
+  if (wave == 0) {
+    float tot[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      float t = 0.f;
+#pragma unroll
+      for (int w = 0; w < WG_SPLIT; ++w) t += red[w][v][lane];
+      tot[v] = t;
     }
-    if (idx >= 0) grad[idx] = acc[v];
-  }
-}
-
-// Bias gradients (row sums of the stored pre-activation gradients) and the
-// ScalarAffine gradients (fixed-order sum of per-block partials).
-template <class PT>
-__global__ void __launch_bounds__(256) flow_rowsum_kernel(FlowDev f, int l, const void* store, int64_t ld,
-                                                          const float* aff_part, int n_part, float* grad) {
-  using T = typename PT::T;
-  const FlowDims& dm = f.dm;
-  const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int h1p = dm.h1p(), h2p = dm.h2p(), n3p = dm.n3p(l);
-  const T* st = reinterpret_cast<const T*>(store);
-  if (r == h1p + h2p + n3p) {
-    if (lane == 0) {
-      float s0 = 0.f, s1 = 0.f;
-      for (int i = 0; i < n_part; ++i) {
-        s0 += aff_part[2 * i];
-        s1 += aff_part[2 * i + 1];
+    const int k = kb * 32 + (lane & 31);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int n = nb * 32 + out_row(v, lane);
+      int64_t idx = -1;
+      if (prob == 0) {
+        if (n < dm.h1 && k < dm.dc(l)) idx = dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k);
+      } else if (prob == 1) {
+        if (n < dm.h2 && k < dm.h1) idx = dm.th_w2(l) + (int64_t)n * dm.h1 + k;
+      } else {
+        int row = dm.gp_row(l, n);
+        if (row >= 0 && k < dm.h2) idx = dm.th_w3(l) + (int64_t)row * dm.h2 + k;
       }
-      grad[dm.th_scale(l)] = s0;
-      grad[dm.th_shift(l)] = s1;
+      if (idx >= 0) grad[idx] = tot[v];
     }
-    return;
+    if (want_bias) {
+      // lane l and l+32 hold the two halves of G row nb*32 + l%32
+      float b = 0.f;
+#pragma unroll
+      for (int w = 0; w < WG_SPLIT; ++w) b += rsum[w][lane];
+      b += __shfl_xor(b, 32, 64);
+      const int n = nb * 32 + lane;
+      if (lane < 32) {
+        int64_t idx = -1;
+        if (prob == 0) {
+          if (n < dm.h1) idx = dm.th_b1(l) + n;
+        } else if (prob == 1) {
+          if (n < dm.h2) idx = dm.th_b2(l) + n;
+        } else {
+          int row = dm.gp_row(l, n);
+          if (row >= 0) idx = dm.th_b3(l) + row;
+        }
+        if (idx >= 0) grad[idx] = b;
+      }
+    }
   }
-  if (r > h1p + h2p + n3p) return;
-  int srow;
-  int64_t idx = -1;
-  if (r < h1p) {
-    srow = (int)dm.st_gp1(l) + r;
-    if (r < dm.h1) idx = dm.th_b1(l) + r;
-  } else if (r < h1p + h2p) {
-    int q = r - h1p;
-    srow = (int)dm.st_gp2(l) + q;
-    if (q < dm.h2) idx = dm.th_b2(l) + q;
-  } else {
-    int q = r - h1p - h2p;
-    srow = (int)dm.st_gp(l) + q;
-    int row = dm.w3_row(l, q);
-    if (row >= 0) idx = dm.th_b3(l) + row;
-  }
-  float s = 0.f;
-  for (int64_t b = lane; b < ld; b += 64) s += PT::f(st[(int64_t)srow * ld + b]);
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0 && idx >= 0) grad[idx] = s;
 }
 
 // ------------------------------------------------------------- packing ---
@@ -1077,8 +1478,9 @@ __global__ void flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias
       int64_t e = i - dm.pk_layer(l);
       int which = 0;
       int64_t base = 0;
-      const int64_t offs[6] = {dm.pk_w1(l), dm.pk_w2(l), dm.pk_w3(l), dm.pk_w1t(l), dm.pk_w2t(l), dm.pk_w3t(l)};
-      for (int q = 5; q >= 0; --q)
+      const int64_t offs[8] = {dm.pk_w1(l),  dm.pk_w2(l),  dm.pk_w3(l),  dm.pk_w1t(l),
+                               dm.pk_w2t(l), dm.pk_w3t(l), dm.pk_w3r(l), dm.pk_w3rt(l)};
+      for (int q = 7; q >= 0; --q)
         if (e >= offs[q]) {
           which = q;
           base = offs[q];
@@ -1092,7 +1494,9 @@ __global__ void flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias
         case 2: Kdim = dm.h2p(); break;
         case 3: Kdim = dm.h1p(); break;
         case 4: Kdim = dm.h2p(); break;
-        default: Kdim = dm.n3p(l); break;
+        case 5: Kdim = dm.n3p(l); break;
+        case 6: Kdim = dm.h2p(); break;
+        default: Kdim = dm.n3r(l); break;
       }
       const int KB = Kdim / PT::KSTEP;
       int ee = (int)(idx % PT::EPL);
@@ -1119,8 +1523,16 @@ __global__ void flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias
         case 4:
           if (n < dm.h1 && k < dm.h2) v = th[dm.th_w2(l) + (int64_t)k * dm.h1 + n];
           break;
-        default: {
+        case 5: {
           int row = dm.w3_row(l, k);
+          if (row >= 0 && n < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + n];
+        } break;
+        case 6: {
+          int row = dm.w3r_row(l, n);
+          if (row >= 0 && k < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + k];
+        } break;
+        default: {
+          int row = dm.w3k_row(l, k);
           if (row >= 0 && n < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + n];
         } break;
       }
@@ -1137,8 +1549,11 @@ __global__ void flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias
       } else if (e < h1p + h2p) {
         int q = (int)(e - h1p);
         if (q < dm.h2) v = th[dm.th_b2(l) + q];
-      } else {
+      } else if (e < dm.pb_b3r(l)) {
         int row = dm.w3_row(l, (int)(e - h1p - h2p));
+        if (row >= 0) v = th[dm.th_b3(l) + row];
+      } else {
+        int row = dm.b3r_row(l, (int)(e - dm.pb_b3r(l)));
         if (row >= 0) v = th[dm.th_b3(l) + row];
       }
       pbias[bi] = v;
@@ -1266,7 +1681,8 @@ extern "C" int fmc_flow_sizes(const fmc_flow* fl, int64_t* theta_size, int64_t* 
   int mbf = pass_mb(dm, fl->precision == FMC_PREC_BF16, s1);
   int mbb = bwd_mb(dm, fl->precision == FMC_PREC_BF16, s2);
   if (tile_fwd) *tile_fwd = 32 * mbf;
-  if (tile_bwd) *tile_bwd = 32 * mbb;
+  // affine partial gradients are kept per 32 samples whatever tile the backward uses
+  if (tile_bwd) *tile_bwd = mbb > 0 ? 32 : 0;
   return FMC_OK;
 }
 
@@ -1346,11 +1762,19 @@ extern "C" int fmc_flow_backward_layer(void* stream, const fmc_flow* fl, int32_t
   int mb = bwd_mb(fd.dm, bf, shm);
   FMC_REQUIRE(mb > 0, FMC_ERR_UNSUPPORTED, "flow dimensions need more than 160 KB of LDS per tile");
   const int hmax = fd.dm.h1p() > fd.dm.h2p() ? fd.dm.h1p() : fd.dm.h2p();
+  if (mb == 2 && (store_ld / 64 < 512 || hmax > 256)) {
+    // small batches: 32-sample tiles give twice the workgroups (and two per
+    // CU); hidden widths above 256 keep their accumulators in registers only
+    // with 32-sample tiles
+    mb = 1;
+    shm = (size_t)(bf ? BwdSmem<PBF16>(fd.dm, 32).total : BwdSmem<PF32>(fd.dm, 32).total);
+  }
   FMC_REQUIRE(hmax <= 512, FMC_ERR_UNSUPPORTED, "hidden sizes > 512 not supported by the backward kernel");
   const int MT = 32 * mb;
   // every store column [0, store_ld) is rewritten (rows >= n carry zero gradients)
   const int64_t nblk = store_ld / MT;
-  FMC_ARG(n_partials >= nblk, "affine_partials holds %d blocks, need %lld", n_partials, (long long)nblk);
+  FMC_ARG(n_partials >= store_ld / 32, "affine_partials holds %d slots, need %lld", n_partials,
+          (long long)(store_ld / 32));
   BwdArgs a{layer, x_layer, n, g_logp, gy, gx, store, store_ld, affine_partials};
   dim3 g((unsigned)nblk), b(256);
   hipStream_t st = (hipStream_t)stream;
@@ -1387,24 +1811,19 @@ extern "C" int fmc_flow_weight_grad(void* stream, const fmc_flow* fl, int32_t la
   FMC_ARG(store_ld >= 64 && store_ld % 64 == 0, "store_ld must be a positive multiple of 64");
   const FlowDims& dm = fd.dm;
   const int h1p = dm.h1p(), h2p = dm.h2p();
-  const int ntiles = (h1p / 32) * (dm.ncp(layer) / 32) + (h2p / 32) * (h1p / 32) + (dm.n3p(layer) / 32) * (h2p / 32);
+  const int ntiles = (h1p / 32) * (dm.ncp(layer) / 32) + (h2p / 32) * (h1p / 32) + (dm.n3(layer) / 32) * (h2p / 32);
   hipStream_t st = (hipStream_t)stream;
   const bool bf = fl->precision == FMC_PREC_BF16;
-  dim3 g((unsigned)((ntiles + 3) / 4)), b(256);
+  FMC_ARG(n_partials >= store_ld / 32, "affine_partials holds %d slots, need %lld", n_partials,
+          (long long)(store_ld / 32));
+  dim3 g((unsigned)(ntiles + 1)), b(64 * WG_SPLIT);
   if (bf)
-    hipLaunchKernelGGL(flow_wgrad_kernel<PBF16>, g, b, 0, st, fd, layer, store, store_ld, grad);
+    hipLaunchKernelGGL(flow_wgrad_kernel<PBF16>, g, b, 0, st, fd, layer, store, store_ld, affine_partials,
+                       (int)(store_ld / 32), grad);
   else
-    hipLaunchKernelGGL(flow_wgrad_kernel<PF32>, g, b, 0, st, fd, layer, store, store_ld, grad);
-  if ((rc = check_launch("flow weight grad"))) return rc;
-  const int nrows = h1p + h2p + dm.n3p(layer) + 1;
-  dim3 g2((unsigned)((nrows + 3) / 4));
-  if (bf)
-    hipLaunchKernelGGL(flow_rowsum_kernel<PBF16>, g2, b, 0, st, fd, layer, store, store_ld, affine_partials,
-                       n_partials, grad);
-  else
-    hipLaunchKernelGGL(flow_rowsum_kernel<PF32>, g2, b, 0, st, fd, layer, store, store_ld, affine_partials,
-                       n_partials, grad);
-  return check_launch("flow bias grad");
+    hipLaunchKernelGGL(flow_wgrad_kernel<PF32>, g, b, 0, st, fd, layer, store, store_ld, affine_partials,
+                       (int)(store_ld / 32), grad);
+  return check_launch("flow weight grad");
 }
 
 extern "C" int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n, float lr,

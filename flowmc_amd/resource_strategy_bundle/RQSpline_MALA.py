@@ -34,7 +34,7 @@ class RQSpline_MALA_Bundle(ResourceStrategyBundle):
                  rq_spline_n_bins: int = 8, rq_spline_n_layers: int = 4, learning_rate: float = 1e-3,
                  batch_size: int = 10000, n_max_examples: int = 10000, local_thinning: int = 1,
                  global_thinning: int = 1, n_NFproposal_batch_size: int = 10000, verbose: bool = False,
-                 precision: str = "bfloat16", fused_steps: int = 64):
+                 precision: str = "bfloat16", fused_steps: int = 64, exact_inverse: bool = False):
         n_training_steps = (n_local_steps // local_thinning * n_training_loops
                             + n_global_steps // global_thinning * n_training_loops)
         n_production_steps = (n_local_steps // local_thinning * n_production_loops
@@ -53,8 +53,11 @@ class RQSpline_MALA_Bundle(ResourceStrategyBundle):
 
         local_sampler = MALA(step_size=mala_step_size, fused_steps=fused_steps)
         rng_key, subkey = frandom.split(rng_key)
+        # exact_inverse=True samples with the true inverse of the forward map, so
+        # the proposal density equals the flow's log_prob (the reference's layer
+        # order, the default, does not once ScalarAffine leaves the identity)
         model = MaskedCouplingRQSpline(n_dims, rq_spline_n_layers, rq_spline_hidden_units, rq_spline_n_bins, subkey,
-                                       precision=precision)
+                                       precision=precision, exact_inverse=exact_inverse)
         global_sampler = NFProposal(model, n_NFproposal_batch_size=n_NFproposal_batch_size)
         optimizer = Optimizer(model=model, learning_rate=learning_rate)
         logpdf = logpdf if isinstance(logpdf, LogPDF) else LogPDF(logpdf, n_dims=n_dims)

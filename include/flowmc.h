@@ -243,8 +243,8 @@ typedef struct fmc_flow {
 } fmc_flow;
 
 /* Sizes (elements) of theta, packed (of the precision's type), packed_bias,
- * the rows of one backward layer's activation store, and the sample tiles
- * of the pass / backward kernels. */
+ * the rows of one backward layer's activation store, the sample tile of the
+ * pass kernel and the samples per ScalarAffine partial of the backward (32). */
 int fmc_flow_sizes(const fmc_flow* flow, int64_t* theta_size, int64_t* packed_size,
                    int64_t* packed_bias_size, int64_t* store_rows, int32_t* tile_fwd,
                    int32_t* tile_bwd);
@@ -276,13 +276,14 @@ int fmc_flow_pass(void* stream, const fmc_flow* flow, int32_t direction, int32_t
  * nf_model/base.py:98-100, done by hand): given the layer input x_layer,
  * dL/dlog_prob per sample (g_logp) and dL/d(layer output) (gy), writes
  * dL/d(layer input) (gx), the feature-major activation store
- * [store_rows, store_ld] (store_ld = n rounded up to 64) and per-block
- * ScalarAffine partial gradients (store_ld / tile_bwd blocks x 2). */
+ * [store_rows, store_ld] (store_ld = n rounded up to 64) and ScalarAffine
+ * partial gradients per 32 samples (n_partials >= store_ld / 32 slots of 2). */
 int fmc_flow_backward_layer(void* stream, const fmc_flow* flow, int32_t layer,
                             const float* x_layer, int64_t n, const float* g_logp, const float* gy,
                             float* gx, void* store, int64_t store_ld, float* affine_partials,
                             int32_t n_partials);
-/* Weight / bias / ScalarAffine gradients of `layer` into grad (theta layout). */
+/* Weight / bias / ScalarAffine gradients of `layer` into grad (theta layout),
+ * from the store and the store_ld / 32 affine partials of the backward. */
 int fmc_flow_weight_grad(void* stream, const fmc_flow* flow, int32_t layer, const void* store,
                          int64_t store_ld, const float* affine_partials, int32_t n_partials,
                          float* grad);

@@ -36,7 +36,9 @@ CASES = [  # d, L, K, h1, h2
     (32, 3, 10, 128, 128),
     (64, 2, 8, 256, 256),
     (7, 3, 4, 16, 48),
+    (6, 2, 12, 32, 32),  # K > 10: chunked spline path
 ]
+ADJ_CASES = CASES[:3] + CASES[-1:]
 
 
 def make(d, L, K, h1, h2, prec, seed=0, scale=0.3, hidden_var=1.0):
@@ -122,7 +124,7 @@ def test_flow_bf16_error_vs_fp64_reference_init(gpu, case):
 
 
 @pytest.mark.parametrize("prec", ["float32", "bfloat16"])
-@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("case", ADJ_CASES)
 def test_flow_adjoint_matches_oracle_autograd(gpu, prec, case):
     d, L, K, h1, h2 = case
     m, p = make(d, L, K, h1, h2, prec, seed=2)
@@ -149,6 +151,29 @@ def test_flow_adjoint_matches_oracle_autograd(gpu, prec, case):
         relx = float((gx.cpu().double() - rgx).norm() / rgx.norm())
         print(f"bf16 adjoint: rel L2 err theta {rel:.3e}, x {relx:.3e}")
         assert rel <= 1e-2 and relx <= 1e-2, (rel, relx)
+
+
+@pytest.mark.parametrize("prec", ["float32", "bfloat16"])
+def test_flow_gradient_is_additive_over_batch_tiles(gpu, prec):
+    """Batches of >= 32768 samples run the backward in 64-sample tiles,
+    smaller ones in 32-sample tiles: the gradient of a large batch equals the
+    sum of the gradients of its two halves (each below the switch)."""
+    d, L, K, h1, h2 = 64, 2, 8, 256, 256
+    m, _ = make(d, L, K, h1, h2, prec, seed=4)
+    g = torch.Generator().manual_seed(5)
+    B = 40000
+    x = (torch.randn(B, d, generator=g) * 2).float().cuda()
+    w = (torch.randn(B, generator=g) / B).float().cuda()
+    gt, gx = m.grad_log_prob_sum(x, w)
+    h = B // 2
+    g1, gx1 = m.grad_log_prob_sum(x[:h].contiguous(), w[:h].contiguous())
+    g2, gx2 = m.grad_log_prob_sum(x[h:].contiguous(), w[h:].contiguous())
+    gs = (g1 + g2).double()
+    n = m.n_layers * m._layer_size
+    err = float((gt.double()[:n] - gs[:n]).abs().max() / gs[:n].abs().max())
+    assert err <= 1e-4, err
+    gxs = torch.cat([gx1, gx2])
+    assert float((gx - gxs).abs().max()) <= 1e-6 * float(gxs.abs().max())
 
 
 @pytest.mark.parametrize("case", CASES[:3])
