@@ -121,14 +121,20 @@ class PlainFlow:
             ld = ld + l2
         return x, ld
 
-    def inverse(self, y):
+    def inverse(self, y, exact=False):
+        """Layers in reverse; inside a layer ScalarAffine^-1 then spline^-1 (the
+        reference's order, rqSpline.py:479-485) or, with exact, spline^-1 first."""
         ld = torch.zeros(y.shape[0], device=y.device)
         for i in reversed(range(self.L)):
             lp = self.layers[i]
+            if exact:
+                y, l2 = self._coupling(i, y, True)
+                ld = ld + l2
             y = y * torch.exp(-lp["scale"]) - lp["shift"]
             ld = ld - lp["scale"] * self.d
-            y, l2 = self._coupling(i, y, True)
-            ld = ld + l2
+            if not exact:
+                y, l2 = self._coupling(i, y, True)
+                ld = ld + l2
         return y, ld
 
     def log_prob(self, x):
@@ -138,10 +144,10 @@ class PlainFlow:
         u = y - self.base_mean
         return ld - 0.5 * (u * u / var).sum(-1) - 0.5 * torch.log(var).sum() - 0.5 * self.d * math.log(2 * math.pi)
 
-    def sample(self, n, gen):
+    def sample(self, n, gen, exact=False):
         z = self.base_mean + torch.randn(n, self.d, device=self.base_mean.device, generator=gen) * torch.sqrt(
             torch.diagonal(self.base_cov))
-        x, _ = self.inverse(z)
+        x, _ = self.inverse(z, exact)
         return x * torch.sqrt(torch.diagonal(self.data_cov)) + self.data_mean
 
 
@@ -189,7 +195,7 @@ def run_plain_nfmcmc(theta, cfg, target, x0, seed=0, time_budget_s=None):
         return x
 
     def train(buf, filled):
-        data = buf[:, :filled][:, -100:].reshape(-1, d)
+        data = buf[:, :filled][:, -100:].reshape(-1, d)  # history window of 100 rows (train_model.py:66-80)
         data = data[torch.randint(0, data.shape[0], (cfg["n_max_examples"],), device=dev, generator=g)]
         flow.data_mean = data.mean(0)
         flow.data_cov = torch.cov(data.T)
@@ -214,7 +220,7 @@ def run_plain_nfmcmc(theta, cfg, target, x0, seed=0, time_budget_s=None):
 
     def global_step(x, buf, cur):
         with torch.no_grad():
-            prop = flow.sample(C * n_global, g).reshape(C, n_global, d)
+            prop = flow.sample(C * n_global, g, cfg.get("exact_inverse", False)).reshape(C, n_global, d)
             qp = flow.log_prob(prop.reshape(-1, d)).reshape(C, n_global)
             qc = flow.log_prob(x)
             lpp = torch.func.vmap(f)(prop.reshape(-1, d)).reshape(C, n_global)
