@@ -192,11 +192,16 @@ def bench_nfmcmc(args):
     ms = time_kernel(lambda: model.sample(fr.PRNGKey(3), C_ * S), reps=3)
     fl = flow_flops_per_sample(d, L, K, H[0], H[1]) * C_ * S
     achieved = fl / (ms * 1e-3) / 1e12
+    traffic = None  # HBM bytes per launch from the counter runs of the same pass (flow_pass workload)
+    tf = os.path.join(ROOT, "profiles", "flow_pass_kernel_traffic.json")
+    if os.path.exists(tf) and C_ * S == 409600:
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
     out = {
         "value": value,
         "ms_per_step": wall * 1e3 / (2 * args.steps),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "flow_pass_kernel (exact inverse, BF16)", "kernel_ms": ms,
                      "flops_per_launch": fl},
         "extra": {"wall_s": wall, **an, "local_acceptance": local_acc, "global_acceptance": global_acc},
@@ -437,6 +442,48 @@ def cpu_baseline_flow_train(args, d, L, K, H, B):
             "sample": f"{steps} forward+backward steps on {nb}-sample batches, FP32 torch-CPU oracle ({cpu_model()})"}
 
 
+# ------------------------------------------------- flow pass (profiling) --
+def bench_flow_pass(args):
+    """The dominant kernel of the NF-MCMC workload alone: one exact-inverse
+    flow pass over the 409,600 proposals of a global step (d=64, 10 layers, 8
+    bins, [256, 256]), ``--steps`` launches. It also runs one 256 MiB
+    device-to-device copy (a kernel of known traffic) so that counter runs
+    (FETCH_SIZE / WRITE_SIZE) can be scaled to bytes. Traffic measured that
+    way is read from profiles/flow_pass_kernel_traffic.json when present."""
+    import torch
+
+    from flowmc_amd import random as fr
+    from flowmc_amd.resource.model.nf_model.rqSpline import MaskedCouplingRQSpline
+
+    dev = torch.device("cuda")
+    d, L, K, H, n = 64, 10, 8, [256, 256], 409600
+    m = MaskedCouplingRQSpline(d, L, H, K, fr.PRNGKey(0), precision=args.precision, device=dev, exact_inverse=True)
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)  # calibration copy: 256 MiB read + 256 MiB written
+    for i in range(max(1, args.warmup)):
+        m.sample(fr.PRNGKey(i), n)
+    ms = time_kernel(lambda: m.sample(fr.PRNGKey(1), n), reps=args.steps)
+    fl = flow_flops_per_sample(d, L, K, H[0], H[1]) * n
+    achieved = fl / (ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "flow_pass_kernel_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    return {
+        "value": n / (ms * 1e-3),
+        "ms_per_step": ms,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS if args.precision == "bfloat16"
+                     else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / (BF16_PEAK_TFLOPS if args.precision == "bfloat16" else FP32_PEAK_TFLOPS),
+                     "traffic": traffic, "kernel": "flow_pass_kernel (exact inverse)", "kernel_ms": ms,
+                     "flops_per_launch": fl, "algorithmic_bytes_per_launch": n * (d * 4 * 2)},
+        "config": {"workload": "flow_pass", "samples": n, "dim": d, "layers": L, "bins": K, "hidden": H,
+                   "precision": args.precision},
+    }
+
+
 # --------------------------------------------------------------- C4 -------
 def bench_hmc_funnels(args):
     import torch
@@ -486,9 +533,13 @@ WORKLOADS = {
     "flow_train_d32": (bench_flow_train, "flow train samples/s (RQ-spline d=32, 8 layers)", "samples/s", "bf16/f32"),
     "hmc_funnels": (bench_hmc_funnels, "chain-steps/s (batched HMC, 32 funnels d in [8,256])", "chain-steps/s",
                     "f32/f64"),
+    "flow_pass": (bench_flow_pass, "flow samples/s (inverse pass, d=64, 10 layers, [256,256])", "samples/s",
+                  "bf16/f32"),
 }
-DEFAULT_STEPS = {"nfmcmc_gmm64": 20, "mala_rosenbrock_128d": 10048, "flow_train_d32": 50, "hmc_funnels": 1000}
-DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d32": 3, "hmc_funnels": 16}
+DEFAULT_STEPS = {"nfmcmc_gmm64": 20, "mala_rosenbrock_128d": 10048, "flow_train_d32": 50, "hmc_funnels": 1000,
+                 "flow_pass": 10}
+DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d32": 3, "hmc_funnels": 16,
+                  "flow_pass": 1}
 
 
 def main():
