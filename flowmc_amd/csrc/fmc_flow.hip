@@ -133,7 +133,7 @@ __device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, co
   }
 }
 
-constexpr int PF_DEPTH = 8;  // B-fragment prefetch depth (k-steps)
+constexpr int PF_DEPTH = 4;  // B-fragment prefetch depth (k-steps); the kernels run 2-3 waves per SIMD
 
 // --------------------------------------------------------------- flow ----
 struct FlowDev {
@@ -647,8 +647,13 @@ struct PassSmem {
   }
 };
 
+// 32-sample tiles (MB = 1) with the register budget held to three waves per
+// SIMD: three workgroups share a CU (LDS ~43 KB each at d=64, [256,256]) and
+// hide each other's LDS / L2 latency; measured faster than 64-sample tiles
+// with more B-fragment reuse per wave.
 template <class PT, int MB>
-__global__ void __launch_bounds__(256) flow_pass_kernel(FlowDev f, PassArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+flow_pass_kernel(FlowDev f, PassArgs a) {
   using T = typename PT::T;
   constexpr int MT = 32 * MB;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -965,7 +970,7 @@ __device__ __forceinline__ void store_fm(T* st, int64_t ld, int row0, int nf, co
 }
 
 template <class PT, int MB, int HMAX>
-__global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
+__device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a) {
   using T = typename PT::T;
   constexpr int MT = 32 * MB;
   // 32x32 tiles per wave of an [MT x hidden] block (hidden <= HMAX); the same
@@ -1323,6 +1328,20 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel(FlowDev f, BwdArgs a) {
   }
 }
 
+// 32-sample tiles. Hidden widths up to 256: register budget held to two
+// waves per SIMD (no spills), and with ~68 KB of LDS per tile two workgroups
+// share a CU. Wider hidden layers keep the default budget.
+template <class PT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+flow_bwd_kernel_h256(FlowDev f, BwdArgs a) {
+  flow_bwd_body<PT, 1, 256>(f, a);
+}
+
+template <class PT>
+__global__ void __launch_bounds__(256) flow_bwd_kernel_h512(FlowDev f, BwdArgs a) {
+  flow_bwd_body<PT, 1, 512>(f, a);
+}
+
 // --------------------------------------------------- weight gradients ----
 // One workgroup of WG_SPLIT wavefronts per 32x32 tile of gW = G^T Act: the
 // batch reduction is split over the wavefronts (each takes an interleaved
@@ -1636,26 +1655,19 @@ static int check_flow(const fmc_flow* fl, FlowDev& fd) {
 }
 
 static int pass_mb(const FlowDims& dm, bool bf16, size_t& shm) {
-  // largest tile that fits the LDS budget
-  for (int mb = 2; mb >= 1; --mb) {
-    int tot = bf16 ? PassSmem<PBF16>(dm, 32 * mb).total : PassSmem<PF32>(dm, 32 * mb).total;
-    if (tot <= 160 * 1024) {
-      shm = (size_t)tot;
-      return mb;
-    }
-  }
-  return 0;
+  // 32-sample tiles (several workgroups per CU), if they fit the LDS budget
+  int tot = bf16 ? PassSmem<PBF16>(dm, 32).total : PassSmem<PF32>(dm, 32).total;
+  if (tot > 160 * 1024) return 0;
+  shm = (size_t)tot;
+  return 1;
 }
 
 static int bwd_mb(const FlowDims& dm, bool bf16, size_t& shm) {
-  for (int mb = 2; mb >= 1; --mb) {
-    int tot = bf16 ? BwdSmem<PBF16>(dm, 32 * mb).total : BwdSmem<PF32>(dm, 32 * mb).total;
-    if (tot <= 160 * 1024) {
-      shm = (size_t)tot;
-      return mb;
-    }
-  }
-  return 0;
+  // 32-sample tiles, if they fit the LDS budget
+  int tot = bf16 ? BwdSmem<PBF16>(dm, 32).total : BwdSmem<PF32>(dm, 32).total;
+  if (tot > 160 * 1024) return 0;
+  shm = (size_t)tot;
+  return 1;
 }
 
 template <class K>
@@ -1727,21 +1739,11 @@ extern "C" int fmc_flow_pass(void* stream, const fmc_flow* fl, int32_t direction
   dim3 g((unsigned)((n + MT - 1) / MT)), b(256);
   hipStream_t st = (hipStream_t)stream;
   if (bf) {
-    if (mb == 2) {
-      set_lds(flow_pass_kernel<PBF16, 2>, shm);
-      hipLaunchKernelGGL((flow_pass_kernel<PBF16, 2>), g, b, shm, st, fd, a);
-    } else {
-      set_lds(flow_pass_kernel<PBF16, 1>, shm);
-      hipLaunchKernelGGL((flow_pass_kernel<PBF16, 1>), g, b, shm, st, fd, a);
-    }
+    set_lds(flow_pass_kernel<PBF16, 1>, shm);
+    hipLaunchKernelGGL((flow_pass_kernel<PBF16, 1>), g, b, shm, st, fd, a);
   } else {
-    if (mb == 2) {
-      set_lds(flow_pass_kernel<PF32, 2>, shm);
-      hipLaunchKernelGGL((flow_pass_kernel<PF32, 2>), g, b, shm, st, fd, a);
-    } else {
-      set_lds(flow_pass_kernel<PF32, 1>, shm);
-      hipLaunchKernelGGL((flow_pass_kernel<PF32, 1>), g, b, shm, st, fd, a);
-    }
+    set_lds(flow_pass_kernel<PF32, 1>, shm);
+    hipLaunchKernelGGL((flow_pass_kernel<PF32, 1>), g, b, shm, st, fd, a);
   }
   return check_launch("flow pass");
 }
@@ -1762,42 +1764,30 @@ extern "C" int fmc_flow_backward_layer(void* stream, const fmc_flow* fl, int32_t
   int mb = bwd_mb(fd.dm, bf, shm);
   FMC_REQUIRE(mb > 0, FMC_ERR_UNSUPPORTED, "flow dimensions need more than 160 KB of LDS per tile");
   const int hmax = fd.dm.h1p() > fd.dm.h2p() ? fd.dm.h1p() : fd.dm.h2p();
-  if (mb == 2 && (store_ld / 64 < 512 || hmax > 256)) {
-    // small batches: 32-sample tiles give twice the workgroups (and two per
-    // CU); hidden widths above 256 keep their accumulators in registers only
-    // with 32-sample tiles
-    mb = 1;
-    shm = (size_t)(bf ? BwdSmem<PBF16>(fd.dm, 32).total : BwdSmem<PF32>(fd.dm, 32).total);
-  }
   FMC_REQUIRE(hmax <= 512, FMC_ERR_UNSUPPORTED, "hidden sizes > 512 not supported by the backward kernel");
-  const int MT = 32 * mb;
   // every store column [0, store_ld) is rewritten (rows >= n carry zero gradients)
-  const int64_t nblk = store_ld / MT;
-  FMC_ARG(n_partials >= store_ld / 32, "affine_partials holds %d slots, need %lld", n_partials,
-          (long long)(store_ld / 32));
+  const int64_t nblk = store_ld / 32;
+  FMC_ARG(n_partials >= nblk, "affine_partials holds %d slots, need %lld", n_partials, (long long)nblk);
   BwdArgs a{layer, x_layer, n, g_logp, gy, gx, store, store_ld, affine_partials};
   dim3 g((unsigned)nblk), b(256);
   hipStream_t st = (hipStream_t)stream;
-#define FMC_BWD(PT_, MB_, H_)                                                   \
-  do {                                                                          \
-    set_lds(flow_bwd_kernel<PT_, MB_, H_>, shm);                                \
-    hipLaunchKernelGGL((flow_bwd_kernel<PT_, MB_, H_>), g, b, shm, st, fd, a); \
-  } while (0)
-  const bool small = hmax <= 256;
-  if (bf) {
-    if (mb == 2) {
-      if (small) FMC_BWD(PBF16, 2, 256); else FMC_BWD(PBF16, 2, 512);
+  if (hmax <= 256) {
+    if (bf) {
+      set_lds(flow_bwd_kernel_h256<PBF16>, shm);
+      hipLaunchKernelGGL(flow_bwd_kernel_h256<PBF16>, g, b, shm, st, fd, a);
     } else {
-      if (small) FMC_BWD(PBF16, 1, 256); else FMC_BWD(PBF16, 1, 512);
+      set_lds(flow_bwd_kernel_h256<PF32>, shm);
+      hipLaunchKernelGGL(flow_bwd_kernel_h256<PF32>, g, b, shm, st, fd, a);
     }
   } else {
-    if (mb == 2) {
-      if (small) FMC_BWD(PF32, 2, 256); else FMC_BWD(PF32, 2, 512);
+    if (bf) {
+      set_lds(flow_bwd_kernel_h512<PBF16>, shm);
+      hipLaunchKernelGGL(flow_bwd_kernel_h512<PBF16>, g, b, shm, st, fd, a);
     } else {
-      if (small) FMC_BWD(PF32, 1, 256); else FMC_BWD(PF32, 1, 512);
+      set_lds(flow_bwd_kernel_h512<PF32>, shm);
+      hipLaunchKernelGGL(flow_bwd_kernel_h512<PF32>, g, b, shm, st, fd, a);
     }
   }
-#undef FMC_BWD
   return check_launch("flow backward");
 }
 

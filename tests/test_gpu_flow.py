@@ -37,8 +37,9 @@ CASES = [  # d, L, K, h1, h2
     (64, 2, 8, 256, 256),
     (7, 3, 4, 16, 48),
     (6, 2, 12, 32, 32),  # K > 10: chunked spline path
+    (6, 2, 8, 320, 64),  # hidden > 256: the wide-hidden adjoint kernel
 ]
-ADJ_CASES = CASES[:3] + CASES[-1:]
+ADJ_CASES = CASES[:3] + CASES[-2:]
 
 
 def make(d, L, K, h1, h2, prec, seed=0, scale=0.3, hidden_var=1.0):
@@ -155,9 +156,9 @@ def test_flow_adjoint_matches_oracle_autograd(gpu, prec, case):
 
 @pytest.mark.parametrize("prec", ["float32", "bfloat16"])
 def test_flow_gradient_is_additive_over_batch_tiles(gpu, prec):
-    """Batches of >= 32768 samples run the backward in 64-sample tiles,
-    smaller ones in 32-sample tiles: the gradient of a large batch equals the
-    sum of the gradients of its two halves (each below the switch)."""
+    """The gradient of a 40k-sample batch (1250 adjoint workgroups, several
+    per CU, split-K weight gradients over 8 wavefronts) equals the sum of the
+    gradients of its two halves, and dL/dx is per sample."""
     d, L, K, h1, h2 = 64, 2, 8, 256, 256
     m, _ = make(d, L, K, h1, h2, prec, seed=4)
     g = torch.Generator().manual_seed(5)
