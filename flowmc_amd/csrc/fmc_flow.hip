@@ -5,7 +5,7 @@
 //                         32*MB samples (activations stay in LDS)
 //   flow_bwd_kernel       adjoint of one forward layer (recomputes conditioner
 //                         and spline bin parameters, hand-derived spline VJP),
-//                         stores feature-major activations / gradients
+//                         stores activations / gradients (fragment-blocked)
 //   flow_wgrad_kernel     weight, bias and ScalarAffine gradients: split batch
 //                         reduction on MFMA
 //   flow_pack_kernel      master FP32 parameters -> MFMA operand images
@@ -42,9 +42,9 @@ struct PF32 {
   static __device__ __forceinline__ f32x16_t mma(float a, float b, f32x16_t c) {
     return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
   }
-  // fragments read straight from a feature-major global store (rows x ld)
-  static __device__ __forceinline__ float load_g(const T* base, int64_t ld, int row0, int b, int lane) {
-    return base[(int64_t)(row0 + (lane & 31)) * ld + b + (lane >> 5)];
+  // fragment of row block rb, batch slice s of the fragment-blocked store (ns slices per row block)
+  static __device__ __forceinline__ float load_g(const T* base, int64_t ns, int rb, int64_t s, int lane) {
+    return base[((int64_t)rb * ns + s) * 64 + lane];
   }
   static __device__ __forceinline__ T cvt(float x) { return x; }
   static __device__ __forceinline__ float f(T x) { return x; }
@@ -64,8 +64,8 @@ struct PBF16 {
   static __device__ __forceinline__ f32x16_t mma(bf16x8_t a, bf16x8_t b, f32x16_t c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
   }
-  static __device__ __forceinline__ bf16x8_t load_g(const T* base, int64_t ld, int row0, int b, int lane) {
-    return *reinterpret_cast<const bf16x8_t*>(base + (int64_t)(row0 + (lane & 31)) * ld + b + (lane >> 5) * 8);
+  static __device__ __forceinline__ bf16x8_t load_g(const T* base, int64_t ns, int rb, int64_t s, int lane) {
+    return *reinterpret_cast<const bf16x8_t*>(base + (((int64_t)rb * ns + s) * 64 + lane) * 8);
   }
   static __device__ __forceinline__ T cvt(float x) { return (__bf16)x; }
   static __device__ __forceinline__ float f(T x) { return (float)x; }
@@ -919,7 +919,7 @@ struct BwdArgs {
   const float* g_logp;   // [n]
   const float* gy;       // [n, d] dL/d(layer output)
   float* gx;             // [n, d] dL/d(layer input)
-  void* store;           // feature-major activations [st_rows, ld]
+  void* store;           // fragment-blocked activations / gradients (st_rows x ld elements)
   int64_t ld;
   float* aff_part;       // [n_blocks, 2]
 };
@@ -959,13 +959,25 @@ struct BwdSmem {
   }
 };
 
-// Feature-major store: dst[(row0 + f) * ld + m0 + m] = src[m * lds + f].
-template <class T>
-__device__ __forceinline__ void store_fm(T* st, int64_t ld, int row0, int nf, const T* src, int lds, int64_t m0,
-                                         int MT) {
-  for (int i = threadIdx.x; i < nf * MT; i += 256) {
-    int fidx = i / MT, m = i - fidx * MT;
-    st[(int64_t)(row0 + fidx) * ld + m0 + m] = src[m * lds + fidx];
+// Tile of the fragment-blocked store: rows row0 .. row0+nf (row0 and nf
+// multiples of 32) x samples m0 .. m0+MT, from src[m * lds + f] (rows f >=
+// nvalid are written as zeros). One thread per EPL-element lane chunk, so
+// consecutive threads write consecutive chunks of a 64-lane block.
+template <class PT>
+__device__ __forceinline__ void store_fm(typename PT::T* st, int64_t ld, int row0, int nf, int nvalid,
+                                         const typename PT::T* src, int lds, int64_t m0, int MT) {
+  using T = typename PT::T;
+  constexpr int EPL = PT::EPL, KS = PT::KSTEP;
+  const int64_t ns = ld / KS;
+  const int spt = MT / KS;                 // batch slices in the tile
+  const int nchunk = (nf / 32) * spt * 64;
+  for (int c = threadIdx.x; c < nchunk; c += 256) {
+    const int lane = c & 63, sl = (c >> 6) % spt, rbl = (c >> 6) / spt;
+    const int f = rbl * 32 + (lane & 31);
+    const int mb = sl * KS + (lane >> 5) * EPL;
+    T* dst = st + (((int64_t)(row0 / 32 + rbl) * ns + m0 / KS + sl) * 64 + lane) * EPL;
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) dst[e] = (f < nvalid) ? src[(mb + e) * lds + f] : PT::cvt(0.f);
   }
 }
 
@@ -1019,7 +1031,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
     a0[m * la0 + k] = PT::cvt(k < dc ? xs[m * ldx + dm.cond(l, k)] : 0.f);
   }
   __syncthreads();
-  store_fm<T>(st, a.ld, (int)dm.st_c(l), kc, a0, la0, m0, MT);
+  store_fm<PT>(st, a.ld, (int)dm.st_c(l), dm.ncp(l), kc, a0, la0, m0, MT);
   // wave w owns column blocks w, w+4, w+8, ... (all MB row blocks) of every
   // [MT x hidden] product below; index ci of the register arrays is the
   // wave-local column block.
@@ -1052,7 +1064,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
     }
   }
   __syncthreads();
-  store_fm<T>(st, a.ld, (int)dm.st_h1(l), h1p, hb1, sm.lh1, m0, MT);
+  store_fm<PT>(st, a.ld, (int)dm.st_h1(l), h1p, h1p, hb1, sm.lh1, m0, MT);
   {
     const int KB = h1p / PT::KSTEP, NBt = h2p / 32;
 #pragma unroll
@@ -1081,7 +1093,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
     }
   }
   __syncthreads();
-  store_fm<T>(st, a.ld, (int)dm.st_h2(l), h2p, hb2, sm.lh2, m0, MT);
+  store_fm<PT>(st, a.ld, (int)dm.st_h2(l), h2p, h2p, hb2, sm.lh2, m0, MT);
 
   // ---- chunks: params -> spline VJP -> gp; g_h2 += gp W3c (register accumulators)
   f32x16_t gh2[NCW][MB];
@@ -1144,7 +1156,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
         }
       }
       __syncthreads();
-      store_fm<T>(st, a.ld, (int)dm.st_gp(l) + 64 * q0, 64 * nq, gpb, sm.lg, m0, MT);
+      store_fm<PT>(st, a.ld, (int)dm.st_gp(l) + 64 * q0, 64 * nq, 64 * nq, gpb, sm.lg, m0, MT);
       const int kb0 = 64 * q0 / PT::KSTEP, kb1 = 64 * (q0 + nq) / PT::KSTEP;
 #pragma unroll
       for (int ci = 0; ci < NCW; ci += 2) {
@@ -1205,7 +1217,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
       gpb[m * sm.lg + cc] = PT::cvt(0.f);
     }
     __syncthreads();
-    store_fm<T>(st, a.ld, (int)dm.st_gp(l) + coff, nc, gpb, sm.lg, m0, MT);
+    store_fm<PT>(st, a.ld, (int)dm.st_gp(l) + coff, nc, nc, gpb, sm.lg, m0, MT);
     const int kb0 = coff / PT::KSTEP, kb1 = (coff + nc) / PT::KSTEP;
 #pragma unroll
     for (int ci = 0; ci < NCW; ci += 2) {
@@ -1240,7 +1252,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
     }
   }
   __syncthreads();
-  store_fm<T>(st, a.ld, (int)dm.st_gp2(l), h2p, hb2, sm.lh2, m0, MT);
+  store_fm<PT>(st, a.ld, (int)dm.st_gp2(l), h2p, h2p, hb2, sm.lh2, m0, MT);
   // ---- g_pre1 = (g_pre2 W2) * (1 - h1^2)   (in place of h1)
   {
     const int KB = h2p / PT::KSTEP, NBt = h1p / 32;
@@ -1266,7 +1278,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
     }
   }
   __syncthreads();
-  store_fm<T>(st, a.ld, (int)dm.st_gp1(l), h1p, hb1, sm.lh1, m0, MT);
+  store_fm<PT>(st, a.ld, (int)dm.st_gp1(l), h1p, h1p, hb1, sm.lh1, m0, MT);
   // ---- g_c = g_pre1 W1c, scattered onto the conditioning dims
   {
     const int KB = h1p / PT::KSTEP, NBt = dm.ncp(l) / 32;
@@ -1346,11 +1358,20 @@ __global__ void __launch_bounds__(256) flow_bwd_kernel_h512(FlowDev f, BwdArgs a
 // One workgroup of WG_SPLIT wavefronts per 32x32 tile of gW = G^T Act: the
 // batch reduction is split over the wavefronts (each takes an interleaved
 // share of the KSTEP-wide batch slices) and the partial tiles are summed in
-// LDS in a fixed order. Both operands are feature-major in the store.
-// Problems: 0 = W1c, 1 = W2, 2 = W3 (chunked). Tiles of the first column
-// block (kb == 0) also sum their G rows into the bias gradients. The last
-// workgroup sums the ScalarAffine partials.
+// LDS in a fixed order. Both operands come from the fragment-blocked store:
+// each fragment is one coalesced 64-lane read.
+// Problems: 0 = W1c, 1 = W2, 2 = W3 (gradient-store order). Tiles of the
+// first column block (kb == 0) also sum their G rows into the bias
+// gradients. The last tile sums the ScalarAffine partials. Workgroup b runs
+// on XCD b % 8; tiles are dealt so that XCD x gets a contiguous range of
+// them, i.e. the tiles sharing a G row panel share one L2.
 constexpr int WG_SPLIT = 8;
+constexpr int WG_XCDS = 8;
+
+__host__ __device__ inline int wgrad_ntiles(const FlowDims& dm, int l) {
+  return (dm.h1p() / 32) * (dm.ncp(l) / 32) + (dm.h2p() / 32) * (dm.h1p() / 32) + (dm.n3(l) / 32) * (dm.h2p() / 32) +
+         1;
+}
 
 template <class PT>
 __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, int l, const void* store, int64_t ld,
@@ -1360,7 +1381,10 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
   __shared__ float rsum[WG_SPLIT][64];
   const FlowDims& dm = f.dm;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int tt = blockIdx.x;
+  const int total = wgrad_ntiles(dm, l);
+  const int per = (total + WG_XCDS - 1) / WG_XCDS;
+  int tt = (int)(blockIdx.x % WG_XCDS) * per + (int)(blockIdx.x / WG_XCDS);
+  if (tt >= total) return;
   const int h1p = dm.h1p(), h2p = dm.h2p();
   const int n0t = (h1p / 32) * (dm.ncp(l) / 32), n1t = (h2p / 32) * (h1p / 32), n2t = (dm.n3(l) / 32) * (h2p / 32);
   if (tt == n0t + n1t + n2t) {  // ScalarAffine gradients: fixed-order sum of the partials
@@ -1406,16 +1430,17 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
   // this wave takes batch slices wave, wave + WG_SPLIT, ... ; fragments
   // (L2-resident store) are prefetched P slices ahead
   constexpr int P = PF_DEPTH;
-  using GF = decltype(PT::load_g(st, ld, 0, 0, 0));
+  using GF = decltype(PT::load_g(st, 0, 0, 0, 0));
   GF ga[P], ab[P];
   const int nsteps = (int)(ld / PT::KSTEP);
+  const int gRb = gRow / 32 + nb, aRb = aRow / 32 + kb;
   const int my = (nsteps - wave + WG_SPLIT - 1) / WG_SPLIT;  // slices of this wave
 #pragma unroll
   for (int p = 0; p < P; ++p)
     if (p < my) {
       const int sl = wave + p * WG_SPLIT;
-      ga[p] = PT::load_g(st, ld, gRow + nb * 32, sl * PT::KSTEP, lane);
-      ab[p] = PT::load_g(st, ld, aRow + kb * 32, sl * PT::KSTEP, lane);
+      ga[p] = PT::load_g(st, nsteps, gRb, sl, lane);
+      ab[p] = PT::load_g(st, nsteps, aRb, sl, lane);
     }
   for (int s0 = 0; s0 < my; s0 += P) {
 #pragma unroll
@@ -1426,8 +1451,8 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
         if (want_bias) bsum += PT::hsum(ga[p]);
         if (i + P < my) {
           const int sl = wave + (i + P) * WG_SPLIT;
-          ga[p] = PT::load_g(st, ld, gRow + nb * 32, sl * PT::KSTEP, lane);
-          ab[p] = PT::load_g(st, ld, aRow + kb * 32, sl * PT::KSTEP, lane);
+          ga[p] = PT::load_g(st, nsteps, gRb, sl, lane);
+          ab[p] = PT::load_g(st, nsteps, aRb, sl, lane);
         }
       }
     }
@@ -1484,84 +1509,19 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
 }
 
 // ------------------------------------------------------------- packing ---
+// Grid: x strides over the elements of one image, y = layer * 9 + image, with
+// images 0..7 = w1, w2, w3, w1t, w2t, w3t, w3r, w3rt and 8 = the layer's
+// packed biases, so every element decodes its own (block, lane, k) only.
 template <class PT>
-__global__ void flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias) {
+__global__ void __launch_bounds__(256) flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias) {
   const FlowDims& dm = f.dm;
-  const int64_t tot = dm.pk_total(), btot = dm.pb_total();
   const float* th = f.theta;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot + btot;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (i < tot) {
-      int l = 0;
-      while (l + 1 < dm.L && dm.pk_layer(l + 1) <= i) ++l;
-      int64_t e = i - dm.pk_layer(l);
-      int which = 0;
-      int64_t base = 0;
-      const int64_t offs[8] = {dm.pk_w1(l),  dm.pk_w2(l),  dm.pk_w3(l),  dm.pk_w1t(l),
-                               dm.pk_w2t(l), dm.pk_w3t(l), dm.pk_w3r(l), dm.pk_w3rt(l)};
-      for (int q = 7; q >= 0; --q)
-        if (e >= offs[q]) {
-          which = q;
-          base = offs[q];
-          break;
-        }
-      int64_t idx = e - base;
-      int Kdim;
-      switch (which) {
-        case 0: Kdim = dm.kc(l); break;
-        case 1: Kdim = dm.h1p(); break;
-        case 2: Kdim = dm.h2p(); break;
-        case 3: Kdim = dm.h1p(); break;
-        case 4: Kdim = dm.h2p(); break;
-        case 5: Kdim = dm.n3p(l); break;
-        case 6: Kdim = dm.h2p(); break;
-        default: Kdim = dm.n3r(l); break;
-      }
-      const int KB = Kdim / PT::KSTEP;
-      int ee = (int)(idx % PT::EPL);
-      int lane = (int)((idx / PT::EPL) % 64);
-      int64_t blk = idx / (64 * PT::EPL);
-      int kb = (int)(blk % KB), nb = (int)(blk / KB);
-      int n = nb * 32 + (lane & 31);
-      int k = kb * PT::KSTEP + (lane >> 5) * PT::EPL + ee;
-      float v = 0.f;
-      switch (which) {
-        case 0:
-          if (n < dm.h1 && k < dm.dc(l)) v = th[dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k)];
-          break;
-        case 1:
-          if (n < dm.h2 && k < dm.h1) v = th[dm.th_w2(l) + (int64_t)n * dm.h1 + k];
-          break;
-        case 2: {
-          int row = dm.w3_row(l, n);
-          if (row >= 0 && k < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + k];
-        } break;
-        case 3:
-          if (n < dm.dc(l) && k < dm.h1) v = th[dm.th_w1(l) + (int64_t)k * dm.d + dm.cond(l, n)];
-          break;
-        case 4:
-          if (n < dm.h1 && k < dm.h2) v = th[dm.th_w2(l) + (int64_t)k * dm.h1 + n];
-          break;
-        case 5: {
-          int row = dm.w3_row(l, k);
-          if (row >= 0 && n < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + n];
-        } break;
-        case 6: {
-          int row = dm.w3r_row(l, n);
-          if (row >= 0 && k < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + k];
-        } break;
-        default: {
-          int row = dm.w3k_row(l, k);
-          if (row >= 0 && n < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + n];
-        } break;
-      }
-      packed[i] = PT::cvt(v);
-    } else {
-      int64_t bi = i - tot;
-      int l = 0;
-      while (l + 1 < dm.L && dm.pb_layer(l + 1) <= bi) ++l;
-      int64_t e = bi - dm.pb_layer(l);
-      const int h1p = dm.h1p(), h2p = dm.h2p();
+  const int l = blockIdx.y / 9, which = blockIdx.y - 9 * l;
+  const int h1p = dm.h1p(), h2p = dm.h2p();
+  if (which == 8) {
+    const int64_t n = dm.pb_size(l);
+    float* out = pbias + dm.pb_layer(l);
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
       float v = 0.f;
       if (e < h1p) {
         if (e < dm.h1) v = th[dm.th_b1(l) + e];
@@ -1575,8 +1535,64 @@ __global__ void flow_pack_kernel(FlowDev f, typename PT::T* packed, float* pbias
         int row = dm.b3r_row(l, (int)(e - dm.pb_b3r(l)));
         if (row >= 0) v = th[dm.th_b3(l) + row];
       }
-      pbias[bi] = v;
+      out[e] = v;
     }
+    return;
+  }
+  int64_t off;
+  int N, Kdim;
+  switch (which) {
+    case 0: off = dm.pk_w1(l); N = h1p; Kdim = dm.kc(l); break;
+    case 1: off = dm.pk_w2(l); N = h2p; Kdim = h1p; break;
+    case 2: off = dm.pk_w3(l); N = dm.n3p(l); Kdim = h2p; break;
+    case 3: off = dm.pk_w1t(l); N = dm.ncp(l); Kdim = h1p; break;
+    case 4: off = dm.pk_w2t(l); N = h1p; Kdim = h2p; break;
+    case 5: off = dm.pk_w3t(l); N = h2p; Kdim = dm.n3p(l); break;
+    case 6: off = dm.pk_w3r(l); N = dm.n3r(l); Kdim = h2p; break;
+    default: off = dm.pk_w3rt(l); N = h2p; Kdim = dm.n3r(l); break;
+  }
+  const int64_t size = (int64_t)N * Kdim;
+  typename PT::T* out = packed + dm.pk_layer(l) + off;
+  const int KB = Kdim / PT::KSTEP;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < size; idx += (int64_t)gridDim.x * 256) {
+    const int ee = (int)(idx % PT::EPL);
+    const int lane = (int)((idx / PT::EPL) % 64);
+    const int64_t blk = idx / (64 * PT::EPL);
+    const int kb = (int)(blk % KB), nb = (int)(blk / KB);
+    const int n = nb * 32 + (lane & 31);
+    const int k = kb * PT::KSTEP + (lane >> 5) * PT::EPL + ee;
+    float v = 0.f;
+    switch (which) {
+      case 0:
+        if (n < dm.h1 && k < dm.dc(l)) v = th[dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k)];
+        break;
+      case 1:
+        if (n < dm.h2 && k < dm.h1) v = th[dm.th_w2(l) + (int64_t)n * dm.h1 + k];
+        break;
+      case 2: {
+        int row = dm.w3_row(l, n);
+        if (row >= 0 && k < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + k];
+      } break;
+      case 3:
+        if (n < dm.dc(l) && k < dm.h1) v = th[dm.th_w1(l) + (int64_t)k * dm.d + dm.cond(l, n)];
+        break;
+      case 4:
+        if (n < dm.h1 && k < dm.h2) v = th[dm.th_w2(l) + (int64_t)k * dm.h1 + n];
+        break;
+      case 5: {
+        int row = dm.w3_row(l, k);
+        if (row >= 0 && n < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + n];
+      } break;
+      case 6: {
+        int row = dm.w3r_row(l, n);
+        if (row >= 0 && k < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + k];
+      } break;
+      default: {
+        int row = dm.w3k_row(l, k);
+        if (row >= 0 && n < dm.h2) v = th[dm.th_w3(l) + (int64_t)row * dm.h2 + n];
+      } break;
+    }
+    out[idx] = PT::cvt(v);
   }
 }
 
@@ -1599,32 +1615,61 @@ struct AdamHyper {
   float bc1, bc2;  // 1 - b1^t, 1 - b2^t
 };
 
+// Global norm: fixed-order tree sum of the per-block partials, one block.
+// sqrt of the sum goes to part[n_part] (and, as float, to norm_out).
+__global__ void __launch_bounds__(256) norm_kernel(double* part, int n_part, float* norm_out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n_part; i += 256) s += part[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double r[4];
+  if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt((r[0] + r[1]) + (r[2] + r[3]));
+    part[n_part] = nrm;
+    if (norm_out) *norm_out = (float)nrm;
+  }
+}
+
 // optax.chain(clip_by_global_norm(max_norm), adamw(lr, b1, b2, eps, wd)):
 // u = g (or g / |g| * max_norm when |g| >= max_norm); m, v moments;
 // theta += -lr * (m_hat / (sqrt(v_hat) + eps) + wd * theta)
+__device__ __forceinline__ void adamw_one(float& th, float g, float& mi, float& vi, const AdamHyper& h, bool clip,
+                                          float gn) {
+  float u = g;
+  if (clip) u = (u / gn) * h.max_norm;
+  mi = (1.f - h.b1) * u + h.b1 * mi;
+  vi = (1.f - h.b2) * u * u + h.b2 * vi;
+  const float mh = mi / h.bc1, vh = vi / h.bc2;
+  const float up = mh / (sqrtf(vh) + h.eps) + h.wd * th;
+  th = th + (-h.lr) * up;
+}
+
 __global__ void __launch_bounds__(256) adamw_kernel(float* theta, const float* __restrict__ g, float* m, float* v,
-                                                    int64_t n, AdamHyper h, const double* part, int n_part,
-                                                    float* norm_out) {
-  __shared__ float s_norm;
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < n_part; ++i) t += part[i];
-    s_norm = (float)sqrt(t);
-    if (blockIdx.x == 0 && norm_out) *norm_out = s_norm;
-  }
-  __syncthreads();
-  const float gn = s_norm;
+                                                    int64_t n, AdamHyper h, const double* nrm) {
+  const float gn = (float)nrm[0];
   const bool clip = !(gn < h.max_norm);
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    float u = g[i];
-    if (clip) u = (u / gn) * h.max_norm;
-    float mi = (1.f - h.b1) * u + h.b1 * m[i];
-    float vi = (1.f - h.b2) * u * u + h.b2 * v[i];
-    m[i] = mi;
-    v[i] = vi;
-    float mh = mi / h.bc1, vh = vi / h.bc2;
-    float up = mh / (sqrtf(vh) + h.eps) + h.wd * theta[i];
-    theta[i] = theta[i] + (-h.lr) * up;
+  const int64_t n4 = n / 4;
+  float4* th4 = reinterpret_cast<float4*>(theta);
+  const float4* g4 = reinterpret_cast<const float4*>(g);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256) {
+    float4 t = th4[i], gg = g4[i], mm = m4[i], vv = v4[i];
+    adamw_one(t.x, gg.x, mm.x, vv.x, h, clip, gn);
+    adamw_one(t.y, gg.y, mm.y, vv.y, h, clip, gn);
+    adamw_one(t.z, gg.z, mm.z, vv.z, h, clip, gn);
+    adamw_one(t.w, gg.w, mm.w, vv.w, h, clip, gn);
+    th4[i] = t;
+    m4[i] = mm;
+    v4[i] = vv;
+  }
+  for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    float t = theta[i], mm = m[i], vv = v[i];
+    adamw_one(t, g[i], mm, vv, h, clip, gn);
+    theta[i] = t;
+    m[i] = mm;
+    v[i] = vv;
   }
 }
 
@@ -1703,15 +1748,20 @@ extern "C" int fmc_flow_pack(void* stream, const fmc_flow* fl) {
   int rc = check_flow(fl, fd);
   if (rc) return rc;
   FMC_ARG(fl->packed != nullptr && fl->packed_bias != nullptr, "packed buffers are NULL");
-  const int64_t n = fd.dm.pk_total() + fd.dm.pb_total();
-  unsigned grid = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+  const FlowDims& dm = fd.dm;
+  int64_t mx = 0;  // largest image of a layer
+  for (int l = 0; l < (dm.L < 2 ? dm.L : 2); ++l) {
+    const int64_t c[5] = {(int64_t)dm.h2p() * dm.h1p(), (int64_t)dm.n3p(l) * dm.h2p(), (int64_t)dm.n3r(l) * dm.h2p(),
+                          (int64_t)dm.h1p() * dm.kc(l), dm.pb_size(l)};
+    for (int64_t v : c) mx = v > mx ? v : mx;
+  }
+  unsigned gx = (unsigned)((mx + 255) / 256 < 128 ? (mx + 255) / 256 : 128);
+  dim3 grid(gx < 1 ? 1 : gx, (unsigned)(9 * dm.L));
   hipStream_t st = (hipStream_t)stream;
   if (fl->precision == FMC_PREC_BF16)
-    hipLaunchKernelGGL(flow_pack_kernel<PBF16>, dim3(grid), dim3(256), 0, st, fd, (__bf16*)fl->packed,
-                       fl->packed_bias);
+    hipLaunchKernelGGL(flow_pack_kernel<PBF16>, grid, dim3(256), 0, st, fd, (__bf16*)fl->packed, fl->packed_bias);
   else
-    hipLaunchKernelGGL(flow_pack_kernel<PF32>, dim3(grid), dim3(256), 0, st, fd, (float*)fl->packed,
-                       fl->packed_bias);
+    hipLaunchKernelGGL(flow_pack_kernel<PF32>, grid, dim3(256), 0, st, fd, (float*)fl->packed, fl->packed_bias);
   return check_launch("flow pack");
 }
 
@@ -1799,14 +1849,13 @@ extern "C" int fmc_flow_weight_grad(void* stream, const fmc_flow* fl, int32_t la
   FMC_ARG(layer >= 0 && layer < fl->n_layers, "layer out of range");
   FMC_ARG(store && affine_partials && grad, "NULL pointer");
   FMC_ARG(store_ld >= 64 && store_ld % 64 == 0, "store_ld must be a positive multiple of 64");
-  const FlowDims& dm = fd.dm;
-  const int h1p = dm.h1p(), h2p = dm.h2p();
-  const int ntiles = (h1p / 32) * (dm.ncp(layer) / 32) + (h2p / 32) * (h1p / 32) + (dm.n3(layer) / 32) * (h2p / 32);
+  const int total = wgrad_ntiles(fd.dm, layer);
   hipStream_t st = (hipStream_t)stream;
   const bool bf = fl->precision == FMC_PREC_BF16;
   FMC_ARG(n_partials >= store_ld / 32, "affine_partials holds %d slots, need %lld", n_partials,
           (long long)(store_ld / 32));
-  dim3 g((unsigned)(ntiles + 1)), b(64 * WG_SPLIT);
+  const int per = (total + WG_XCDS - 1) / WG_XCDS;
+  dim3 g((unsigned)(per * WG_XCDS)), b(64 * WG_SPLIT);
   if (bf)
     hipLaunchKernelGGL(flow_wgrad_kernel<PBF16>, g, b, 0, st, fd, layer, store, store_ld, affine_partials,
                        (int)(store_ld / 32), grad);
@@ -1821,19 +1870,24 @@ extern "C" int fmc_adamw_clip(void* stream, float* theta, const float* grad, flo
                               double* workspace, int32_t workspace_len, float* grad_norm) {
   FMC_ARG(theta && grad && m && v && workspace, "NULL pointer");
   FMC_ARG(n >= 0 && step >= 1, "n must be >= 0 and step >= 1");
-  FMC_ARG(workspace_len >= 1, "workspace_len must be >= 1");
+  FMC_ARG(workspace_len >= 2, "workspace_len must be >= 2");
+  FMC_ARG(((uintptr_t)theta | (uintptr_t)grad | (uintptr_t)m | (uintptr_t)v) % 16 == 0,
+          "theta, grad, m and v must be 16-byte aligned");
   if (n == 0) return FMC_OK;
   int nb = (int)((n + 255) / 256);
-  if (nb > workspace_len) nb = workspace_len;
+  if (nb > workspace_len - 1) nb = workspace_len - 1;
   if (nb > 1024) nb = 1024;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(sumsq_kernel, dim3(nb), dim3(256), 0, st, grad, n, workspace);
   int rc = check_launch("adamw sumsq");
   if (rc) return rc;
+  hipLaunchKernelGGL(norm_kernel, dim3(1), dim3(256), 0, st, workspace, nb, grad_norm);
+  if ((rc = check_launch("adamw norm"))) return rc;
   AdamHyper h{lr, b1, b2, eps, weight_decay, max_norm, (float)(1.0 - pow((double)b1, step)),
               (float)(1.0 - pow((double)b2, step))};
-  int nb2 = (int)((n + 255) / 256);
-  if (nb2 > 4096) nb2 = 4096;
-  hipLaunchKernelGGL(adamw_kernel, dim3(nb2), dim3(256), 0, st, theta, grad, m, v, n, h, workspace, nb, grad_norm);
+  int nb2 = (int)((n / 4 + 255) / 256);
+  if (nb2 > 2048) nb2 = 2048;
+  if (nb2 < 1) nb2 = 1;
+  hipLaunchKernelGGL(adamw_kernel, dim3(nb2), dim3(256), 0, st, theta, grad, m, v, n, h, workspace + nb);
   return check_launch("adamw update");
 }

@@ -177,9 +177,14 @@ struct FlowDims {
     return trans(l, c * G() + jj) * P() + p;
   }
 
-  // ---- feature-major activation store of one backward layer ([features, ld] of T)
+  // ---- activation / gradient store of one backward layer (elements of T).
+  // Feature sections start at multiples of 32 rows. Fragment-blocked: the
+  // 32 rows x KSTEP samples of row block rb, batch slice s are one contiguous
+  // block of 64 lanes x EPL elements, element (row 32 rb + r, sample
+  // KSTEP s + j) at lane r + 32 (j / EPL), position j % EPL, so one MFMA
+  // operand fragment along the batch is one coalesced 64-lane read.
   FMC_HD int64_t st_c(int) const { return 0; }
-  FMC_HD int64_t st_h1(int l) const { return kc(l); }
+  FMC_HD int64_t st_h1(int l) const { return ncp(l); }
   FMC_HD int64_t st_h2(int l) const { return st_h1(l) + h1p(); }
   FMC_HD int64_t st_gp1(int l) const { return st_h2(l) + h2p(); }
   FMC_HD int64_t st_gp2(int l) const { return st_gp1(l) + h1p(); }

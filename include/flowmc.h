@@ -275,7 +275,8 @@ int fmc_flow_pass(void* stream, const fmc_flow* flow, int32_t direction, int32_t
 /* Adjoint of forward layer `layer` (autodiff of NFModel.loss_fn,
  * nf_model/base.py:98-100, done by hand): given the layer input x_layer,
  * dL/dlog_prob per sample (g_logp) and dL/d(layer output) (gy), writes
- * dL/d(layer input) (gx), the feature-major activation store
+ * dL/d(layer input) (gx), the activation / gradient store (fragment-blocked,
+ * layout in flowmc_amd/csrc/fmc_flow_layout.h)
  * [store_rows, store_ld] (store_ld = n rounded up to 64) and ScalarAffine
  * partial gradients per 32 samples (n_partials >= store_ld / 32 slots of 2). */
 int fmc_flow_backward_layer(void* stream, const fmc_flow* flow, int32_t layer,
@@ -290,7 +291,8 @@ int fmc_flow_weight_grad(void* stream, const fmc_flow* flow, int32_t layer, cons
 
 /* optax.chain(clip_by_global_norm(max_norm), adamw(lr, b1, b2, eps, weight_decay))
  * applied to the flat vector theta (src/flowMC/resource/optimizer.py:19-22);
- * step counts from 1. workspace: >= min(1024, ceil(n/256)) doubles.
+ * step counts from 1. workspace: workspace_len >= 2 doubles (up to 1024 partial
+ * sums and the norm are kept there); theta, grad, m, v 16-byte aligned.
  * grad_norm (may be NULL) receives the global gradient norm. */
 int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n,
                    float lr, float b1, float b2, float eps, float weight_decay, float max_norm,
