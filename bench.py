@@ -86,13 +86,16 @@ def nfmcmc_config(args):
     MALA step 0.05, RQ-spline 10 layers / 8 bins / [256, 256], 100 local +
     100 global steps per round, ``--steps`` training and production rounds.
     Chosen here (the same for every path): 3 epochs per round over 100k
-    examples drawn from the last 100 rows of every chain, batch 10k, AdamW lr
-    3e-4 (more epochs over the same slowly-moving chain clouds overfit the 4.9M
-    parameters and the global acceptance collapses), and sampling with the
-    exact inverse of the forward map."""
-    return dict(n_local=100, n_global=100, n_loops=args.steps, n_epochs=args.epochs, batch_size=10000,
+    examples drawn from the last 200 rows of every chain (this round's MALA
+    rows and the previous round's NF rows; MALA at 0.05 leaves the 100 local
+    rows near-copies of one point, so a 100-row window trains on only ~4096
+    distinct points per round), batch 10k, AdamW lr 2e-4 (more epochs over
+    the same slowly-moving chain clouds overfit the 4.9M parameters and the
+    global acceptance collapses), and sampling with the exact inverse of the
+    forward map."""
+    return dict(n_local=100, n_global=100, n_loops=args.steps, n_epochs=args.epochs, batch_size=args.batch,
                 n_max_examples=args.nmax, step_size=0.05, lr=args.lr, n_layers=10, n_bins=8, hidden=[256, 256],
-                exact_inverse=True)
+                exact_inverse=True, history_window=args.window)
 
 
 def nfmcmc_initial_positions(target, n_chains, device, seed=1):
@@ -163,16 +166,17 @@ def bench_nfmcmc(args):
                                  loops, cfg["n_epochs"], mala_step_size=cfg["step_size"], rq_spline_hidden_units=H,
                                  rq_spline_n_bins=K, rq_spline_n_layers=L, learning_rate=cfg["lr"],
                                  batch_size=cfg["batch_size"], n_max_examples=cfg["n_max_examples"],
-                                 precision=args.precision, exact_inverse=cfg["exact_inverse"])
+                                 precision=args.precision, exact_inverse=cfg["exact_inverse"],
+                                 history_window=cfg["history_window"])
         return b, Sampler(d, n_chains, k2, resource_strategy_bundles=b)
 
-    x0 = nfmcmc_initial_positions(target, C_, dev)
+    x0 = nfmcmc_initial_positions(target, C_, dev, seed=1 + args.seed)
     # warm-up: a short run of the same shapes (kernels, allocator, workspaces)
     if args.warmup > 0:
         _, s = make(C_, args.warmup, fr.PRNGKey(7))
         s.sample(x0.clone(), {})
         torch.cuda.synchronize()
-    bundle, sampler = make(C_, args.steps, fr.PRNGKey(0))
+    bundle, sampler = make(C_, args.steps, fr.PRNGKey(args.seed))
     theta0 = bundle.resources["model"].theta.detach().clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -211,6 +215,7 @@ def bench_nfmcmc(args):
                    "training_rounds": args.steps, "production_rounds": args.steps, "local_steps": cfg["n_local"],
                    "global_steps": cfg["n_global"], "epochs": cfg["n_epochs"], "batch": cfg["batch_size"],
                    "n_max_examples": cfg["n_max_examples"], "lr": cfg["lr"],
+                   "history_window": cfg["history_window"],
                    "initial_positions": "one exact mixture draw per chain"},
     }
     del res, sampler, bundle
@@ -551,8 +556,11 @@ def main():
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--precision", default="bfloat16")
     ap.add_argument("--epochs", type=int, default=3)
-    ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--nmax", type=int, default=100000)
+    ap.add_argument("--batch", type=int, default=10000)
+    ap.add_argument("--window", type=int, default=200)
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--lpc", type=int, default=0)
     ap.add_argument("--nv", type=int, default=0)
     ap.add_argument("--block", type=int, default=256)

@@ -195,7 +195,7 @@ def run_plain_nfmcmc(theta, cfg, target, x0, seed=0, time_budget_s=None):
         return x
 
     def train(buf, filled):
-        data = buf[:, :filled][:, -100:].reshape(-1, d)  # history window of 100 rows (train_model.py:66-80)
+        data = buf[:, :filled][:, -cfg.get("history_window", 100):].reshape(-1, d)  # train_model.py:66-80
         data = data[torch.randint(0, data.shape[0], (cfg["n_max_examples"],), device=dev, generator=g)]
         flow.data_mean = data.mean(0)
         flow.data_cov = torch.cov(data.T)

@@ -693,7 +693,24 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     }
     xs[m * ldx + j] = v;
   }
-  for (int m = tid; m < MT; m += 256) ldr[m] = 0.f;
+  __syncthreads();
+  // exact inverse with BASE_LOGP: the pass returns the flow log-density of
+  // its output, log N(z) - sum of the inverse log-dets; start from -log N(z)
+  const bool inv_logq = (a.dir == 2) && (a.flags & FL_BASE);
+  for (int m = tid; m < MT; m += 256) {
+    float v = 0.f;
+    if (inv_logq) {
+      float q = 0.f, lg = 0.f;
+      for (int j = 0; j < d; ++j) {
+        float var = th[dm.th_base_cov() + (int64_t)j * d + j];
+        float u = xs[m * ldx + j] - th[dm.th_base_mean() + j];
+        q += u * u / var;
+        lg += logf(var);
+      }
+      v = 0.5f * q + 0.5f * lg + 0.91893853320467274f * (float)d;
+    }
+    ldr[m] = v;
+  }
   __syncthreads();
 
   for (int step = 0; step < dm.L; ++step) {
@@ -896,6 +913,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       }
       ld += -0.5f * q - 0.5f * lg - 0.91893853320467274f * (float)d;
     }
+    if (inv_logq) ld = -ld;
     if (a.logdet) a.logdet[row] = ld;
   }
   if (a.x_out) {
@@ -1778,6 +1796,9 @@ extern "C" int fmc_flow_pass(void* stream, const fmc_flow* fl, int32_t direction
   FMC_ARG(gy_out == nullptr || (g_logp != nullptr && direction == 0 && (flags & FL_BASE)),
           "gy_out needs g_logp, the forward direction and the base log-prob flag");
   FMC_ARG(saved == nullptr || direction == 0, "saved layer inputs are only produced by the forward direction");
+  FMC_ARG(!(flags & FL_BASE) || direction != 1,
+          "the base log-prob flag needs the forward or the exact inverse direction (the reference's inverse "
+          "order is not the inverse of the forward map)");
   FMC_ARG(sample0 >= 0 && sample0 + n <= 0xFFFFFFFFLL, "sample index range exceeds 32 bits");
   if (n == 0) return FMC_OK;
   size_t shm = 0;

@@ -3,7 +3,8 @@
 Per chain, ``n_steps`` proposals are drawn from the flow (inverse pass over
 Philox base noise, one launch for all C*S proposals), their flow log-density
 is evaluated by a forward pass (the reference evaluates log_prob explicitly,
-NF_proposal.py:150-167, so the inverse-order quirk of the flow is honoured),
+NF_proposal.py:150-167, so the inverse-order quirk of the flow is honoured);
+with ``exact_inverse`` it comes from the sampling pass itself,
 the target is evaluated on them, and the independence Metropolis–Hastings
 scan (NF_proposal.py:91-126) runs in fmc_nf_mh, which writes the thinned
 buffer rows directly.
@@ -58,13 +59,21 @@ class NFProposal(ProposalBase):
         S = int(n_steps)
         model = self.model
         k0, k1 = frandom.key_words(rng_key)
-        # 1. proposals [C*S, d] from the flow
-        if noise is not None:
-            prop = model.sample_from_noise(noise.reshape(C_ * S, d))
+        # 1. proposals [C*S, d] from the flow and 2. their flow log-density.
+        # With the exact inverse one pass gives both (log N(z) minus the
+        # inverse log-det); the reference's layer order needs the explicit
+        # forward log_prob (NF_proposal.py:150-167).
+        if getattr(model, "exact_inverse", False):
+            if noise is not None:
+                prop, q_prop = model.sample_and_log_prob(noise=noise.reshape(C_ * S, d))
+            else:
+                prop, q_prop = model.sample_and_log_prob(rng_key, C_ * S)
         else:
-            prop = model.sample(rng_key, C_ * S)
-        # 2. flow log-density of proposals and of the current points
-        q_prop = model.log_prob(prop)
+            if noise is not None:
+                prop = model.sample_from_noise(noise.reshape(C_ * S, d))
+            else:
+                prop = model.sample(rng_key, C_ * S)
+            q_prop = model.log_prob(prop)
         q0 = model.log_prob(x0)
         # 3. target log-density (take_steps.py:202 evaluates the current point)
         lp_prop = _eval_logpdf(logpdf, prop, data)

@@ -282,6 +282,20 @@ class MaskedCouplingRQSpline(NFModel):
                           xb.shape[0])
         return x.reshape(*lead, -1)
 
+    def sample_and_log_prob(self, rng_key=None, n_samples: int = 0, noise=None):
+        """Samples and their flow log-density (``log_prob``'s value) from one
+        exact-inverse pass: log N(z) minus the inverse log-determinant. Only
+        with ``exact_inverse`` (the reference's inverse order is not the
+        inverse of the forward map, so its samples do not have that density)."""
+        if not self.exact_inverse:
+            raise ValueError("sample_and_log_prob needs exact_inverse=True")
+        if noise is not None:
+            xb, lead = self._as_batch(noise)
+            x, lq = self._pass(INVERSE_EXACT, SAMPLE_BASE | STANDARDIZE | BASE_LOGP, xb, xb.shape[0])
+            return x.reshape(*lead, -1), lq.reshape(lead)
+        k = frandom.key_words(rng_key)
+        return self._pass(INVERSE_EXACT, SAMPLE_BASE | STANDARDIZE | BASE_LOGP, None, int(n_samples), key=k)
+
     # ------------------------------------------------------------ training
     def _workspace(self, B: int) -> _Workspace:
         ws = self._ws.get(B)

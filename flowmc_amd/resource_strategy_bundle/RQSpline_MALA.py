@@ -2,7 +2,10 @@
 
 Same resources, strategies and strategy order as the reference. Extra
 keyword arguments: ``precision`` of the flow's conditioner GEMMs
-("bfloat16" default, "float32") and ``fused_steps`` (MALA steps per launch).
+("bfloat16" default, "float32"), ``fused_steps`` (MALA steps per launch),
+``exact_inverse`` (sample with the true inverse of the forward map) and
+``history_window`` (rows per chain TrainModel draws from; 100 as in the
+reference bundle, which leaves TrainModel's default).
 """
 
 from __future__ import annotations
@@ -34,7 +37,8 @@ class RQSpline_MALA_Bundle(ResourceStrategyBundle):
                  rq_spline_n_bins: int = 8, rq_spline_n_layers: int = 4, learning_rate: float = 1e-3,
                  batch_size: int = 10000, n_max_examples: int = 10000, local_thinning: int = 1,
                  global_thinning: int = 1, n_NFproposal_batch_size: int = 10000, verbose: bool = False,
-                 precision: str = "bfloat16", fused_steps: int = 64, exact_inverse: bool = False):
+                 precision: str = "bfloat16", fused_steps: int = 64, exact_inverse: bool = False,
+                 history_window: int = 100):
         n_training_steps = (n_local_steps // local_thinning * n_training_loops
                             + n_global_steps // global_thinning * n_training_loops)
         n_production_steps = (n_local_steps // local_thinning * n_production_loops
@@ -99,7 +103,7 @@ class RQSpline_MALA_Bundle(ResourceStrategyBundle):
                                         thinning=global_thinning, chain_batch_size=chain_batch_size, verbose=verbose)
         model_trainer = TrainModel("model", "positions_training", "optimizer", loss_buffer_name="loss_buffer",
                                    n_epochs=n_epochs, batch_size=batch_size, n_max_examples=n_max_examples,
-                                   verbose=verbose)
+                                   history_window=history_window, verbose=verbose)
         update_state = UpdateState(
             "sampler_state",
             ["target_positions", "target_log_prob", "target_local_accs", "target_global_accs", "training"],

@@ -266,6 +266,11 @@ enum { FMC_FLOW_STANDARDIZE = 1, FMC_FLOW_BASE_LOGP = 2, FMC_FLOW_SAMPLE_BASE = 
  *   stream 5 keyed by (key0,key1) with counter (dim/4, sample0+i, 0, 5) when
  *   x_in is NULL) mapped to N(base_mean, diag base_cov); with STANDARDIZE
  *   the output is y * sqrt(diag data_cov) + data_mean (sample :490-496).
+ *   INVERSE_EXACT with BASE_LOGP: `logdet` receives the flow log-density of
+ *   the output (log_prob's value, without the -sum log sigma the reference
+ *   omits too): log N(z; base) - sum of the inverse log-dets, where z is the
+ *   pass input after SAMPLE_BASE. Refused for the reference-order INVERSE,
+ *   whose output density is not the forward map's.
  * saved (FORWARD only, may be NULL): [n_layers, n, d] input of every layer.
  * gy_out (FORWARD + BASE_LOGP): g_logp[i] * d log N / dy. */
 int fmc_flow_pass(void* stream, const fmc_flow* flow, int32_t direction, int32_t flags,

@@ -104,6 +104,32 @@ def test_flow_passes_match_oracle(gpu, prec, case):
     assert all(checks), checks
 
 
+@pytest.mark.parametrize("prec", ["float32", "bfloat16"])
+@pytest.mark.parametrize("case", CASES[:3])
+def test_exact_inverse_samples_carry_their_log_prob(gpu, prec, case):
+    """One exact-inverse pass returns samples and log N(z) - inverse log-det;
+    that equals the forward log_prob of the same samples (change of
+    variables) and the FP64 oracle's log_prob of them."""
+    d, L, K, h1, h2 = case
+    m, p = make(d, L, K, h1, h2, prec, seed=6)
+    m.exact_inverse = True
+    g = torch.Generator().manual_seed(7)
+    z = torch.randn(400, d, generator=g)
+    x, lq = m.sample_and_log_prob(noise=z.cuda())
+    xs = m.sample_from_noise(z.cuda())
+    assert float((x - xs).abs().max()) == 0.0
+    lf = m.log_prob(x)
+    tol = TOL[prec]
+    q = 1.0 if prec == "float32" else 0.99
+    assert within(lq, lf.double().cpu(), 5 * tol, "inverse-pass log q vs forward log_prob", q)
+    with (F.emulate_bf16() if prec == "bfloat16" else _null()):
+        ref = F.log_prob(p, x.double().cpu(), K)
+    assert within(lq, ref, 5 * tol, "inverse-pass log q vs oracle log_prob", q)
+    m.exact_inverse = False
+    with pytest.raises(ValueError):
+        m.sample_and_log_prob(noise=z.cuda())
+
+
 class _null:
     def __enter__(self):
         return None

@@ -57,3 +57,8 @@ def test_invalid_flow_rejected():
     f = N.fmc_flow(4, 2, 8, 32, 32, 1, -10.0, 10.0, 1e-4, 1e-4, 1, None, None)
     with pytest.raises(N.FlowMCNativeError, match="packed buffers"):
         N.call("fmc_flow_pass", None, C.byref(f), 0, 0, 1, 4, 0, 0, 0, None, None, None, None, None)
+    # the base log-prob of a reference-order inverse pass is refused (its output
+    # does not have the forward map's density); checked before any launch
+    f = N.fmc_flow(4, 2, 8, 32, 32, 1, -10.0, 10.0, 1e-4, 1e-4, 8, 16, 24)
+    with pytest.raises(N.FlowMCNativeError, match="base log-prob"):
+        N.call("fmc_flow_pass", None, C.byref(f), 1, 2 | 4, 0, 4, 0, 0, 0, None, None, None, None, None)
