@@ -20,6 +20,10 @@ Workloads (``--workload``):
   hmc_funnels   32 Neal's funnels of log-uniform dimension in [8, 256], 256
                 chains each, 10 leapfrog steps of 0.05, one batched launch per
                 16 steps (chain-steps/s).
+  grw_rosenbrock_128d  fused Gaussian random walk, same target / chains /
+                step (chain-steps/s).
+  flow_pass     the exact-inverse flow pass over the 409,600 proposals of one
+                global step of nfmcmc_gmm64 (samples/s; profiling target).
 ``--gpus N`` with N > 1 exits: the build runs on a single GPU.
 """
 
@@ -290,6 +294,16 @@ def cpu_baseline_nfmcmc(args, cfg, gpu_wall, min_ess, C_):
 
 # --------------------------------------------------------------- C3 -------
 def bench_mala_rosenbrock(args):
+    return _bench_local_rosenbrock(args, "mala")
+
+
+def bench_grw_rosenbrock(args):
+    """Fused Gaussian random walk on the same 128-D Rosenbrock target (16384
+    chains, step 0.01, 64 steps per launch)."""
+    return _bench_local_rosenbrock(args, "grw")
+
+
+def _bench_local_rosenbrock(args, kind):
     import torch
 
     import flowmc_amd._native as N
@@ -316,8 +330,9 @@ def bench_mala_rosenbrock(args):
         k0, k1 = fr.key_words(key)
         for s0 in range(0, n_steps, K):
             ns = min(K, n_steps - s0)
-            N.call("fmc_mala_run", sp, C.byref(desc), x.data_ptr(), lp.data_ptr(), C_, ns, s0, dt, k0, k1, None, None,
-                   pos.data_ptr(), lps.data_ptr(), acc.data_ptr(), T, 0, 1, cfgp)
+            N.call("fmc_mala_run" if kind == "mala" else "fmc_grw_run", sp, C.byref(desc), x.data_ptr(),
+                   lp.data_ptr(), C_, ns, s0, dt, k0, k1, None, None, pos.data_ptr(), lps.data_ptr(), acc.data_ptr(),
+                   T, 0, 1, cfgp)
 
     run(args.warmup, fr.PRNGKey(1))
     torch.cuda.synchronize()
@@ -333,25 +348,27 @@ def bench_mala_rosenbrock(args):
     n_launch = (steps + K - 1) // K
     bytes_total = C_ * (steps * (4 * d + 8) + n_launch * (8 * d + 8))
     achieved = bytes_total / (gpu_ms * 1e-3) / 1e9
-    flops = C_ * steps * (17 * d + 12 * d)
+    flops = C_ * steps * ((17 * d + 12 * d) if kind == "mala" else (8 * d + 8 * d))
+    kname = "chain_kernel<1,0> (fused MALA)" if kind == "mala" else "chain_kernel<1,2> (fused GRW)"
     out = {
         "value": C_ * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "chain_kernel<1,0> (fused MALA)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
                      "kernel_ms": gpu_ms / n_launch, "bytes_per_launch": bytes_total / n_launch},
         "extra": {"fp32_tflops": flops / (gpu_ms * 1e-3) / 1e12, "acceptance": float(acc[:, :steps].mean())},
-        "config": {"workload": "mala_rosenbrock_128d", "chains": C_, "dim": d, "step_size": dt,
+        "config": {"workload": f"{kind}_rosenbrock_128d", "chains": C_, "dim": d, "step_size": dt,
                    "fused_steps_per_launch": K, "thinning": 1},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_mala_rosenbrock(args)
+        out["cpu_baseline"] = cpu_baseline_mala_rosenbrock(args, kind)
     return out
 
 
-def cpu_baseline_mala_rosenbrock(args):
-    """Oracle MALA (2 value_and_grad per step as MALA.py:59,71-73), FP32, on a
-    bounded sample of chains and steps, timed on the host cores."""
+def cpu_baseline_mala_rosenbrock(args, kind="mala"):
+    """Oracle MALA (2 value_and_grad per step as MALA.py:59,71-73) or GRW (one
+    target evaluation per step), FP32, on a bounded sample of chains and steps,
+    timed on the host cores."""
     import torch
 
     from oracle import chains as OC
@@ -363,16 +380,21 @@ def cpu_baseline_mala_rosenbrock(args):
     x = torch.randn(C_, d, generator=g)
     fn = lambda z: OT.rosenbrock(z)
     steps = 0
+    lp = fn(x)
     OC.mala_step(x, fn, dt, torch.randn(C_, d, generator=g), torch.log(torch.rand(C_, generator=g)))
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < args.cpu_seconds:
         xi = torch.randn(C_, d, generator=g)
         lu = torch.log(torch.rand(C_, generator=g))
-        x, _, _, _ = OC.mala_step(x, fn, dt, xi, lu)
+        if kind == "mala":
+            x, _, _, _ = OC.mala_step(x, fn, dt, xi, lu)
+        else:
+            x, lp, _, _ = OC.grw_step(x, lp, fn, dt, xi, lu)
         steps += 1
     el = time.perf_counter() - t0
     return {"value": C_ * steps / el, "unit": "chain-steps/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{C_} chains x {steps} MALA steps, d={d}, FP32 torch-CPU oracle restatement ({cpu_model()})"}
+            "sample": f"{C_} chains x {steps} {kind.upper()} steps, d={d}, FP32 torch-CPU oracle restatement "
+                      f"({cpu_model()})"}
 
 
 # --------------------------------------------------------------- C2 -------
@@ -535,6 +557,8 @@ WORKLOADS = {
                      "bf16/f32"),
     "mala_rosenbrock_128d": (bench_mala_rosenbrock, "chain-steps/s (fused MALA, 128-D Rosenbrock)", "chain-steps/s",
                              "f32"),
+    "grw_rosenbrock_128d": (bench_grw_rosenbrock, "chain-steps/s (fused Gaussian random walk, 128-D Rosenbrock)",
+                            "chain-steps/s", "f32"),
     "flow_train_d32": (bench_flow_train, "flow train samples/s (RQ-spline d=32, 8 layers)", "samples/s", "bf16/f32"),
     "hmc_funnels": (bench_hmc_funnels, "chain-steps/s (batched HMC, 32 funnels d in [8,256])", "chain-steps/s",
                     "f32/f64"),
@@ -542,9 +566,9 @@ WORKLOADS = {
                   "bf16/f32"),
 }
 DEFAULT_STEPS = {"nfmcmc_gmm64": 20, "mala_rosenbrock_128d": 10048, "flow_train_d32": 50, "hmc_funnels": 1000,
-                 "flow_pass": 10}
+                 "flow_pass": 10, "grw_rosenbrock_128d": 10048}
 DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d32": 3, "hmc_funnels": 16,
-                  "flow_pass": 1}
+                  "flow_pass": 1, "grw_rosenbrock_128d": 640}
 
 
 def main():

@@ -113,11 +113,15 @@ _SIGNATURES: dict[str, list] = {
     "fmc_target_eval": [P, C.POINTER(fmc_target), P, I64, P, P, C.POINTER(fmc_launch_cfg)],
     "fmc_mala_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, U32, U32, P, P, P, P, P, I64, I64, I32,
                      C.POINTER(fmc_launch_cfg)],
+    "fmc_grw_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, U32, U32, P, P, P, P, P, I64, I64, I32,
+                    C.POINTER(fmc_launch_cfg)],
     "fmc_hmc_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, I32, P, U32, U32, P, P, P, P, P, I64, I64,
                     I32, C.POINTER(fmc_launch_cfg)],
     "fmc_hmc_run_batched": [P, P, I32, P, I32, I32, I32, I32, F32, I32, I32, I32, I32],
     "fmc_mala_propose": [P, P, P, P, I64, I32, F32, U32, U32, I32, P],
     "fmc_mala_accept": [P, P, P, P, P, P, P, I64, I32, F32, U32, U32, I32, P, P, P, P, I64, I64, I32],
+    "fmc_grw_propose": [P, P, P, I64, I32, F32, U32, U32, I32, P],
+    "fmc_grw_accept": [P, P, P, P, P, I64, I32, U32, U32, I32, P, P, P, P, I64, I64, I32],
     "fmc_hmc_momentum": [P, P, P, I64, I32, P, U32, U32, I32, P],
     "fmc_hmc_drift": [P, P, P, P, I64, I32, F32],
     "fmc_hmc_kick": [P, P, P, I64, I32, F32],

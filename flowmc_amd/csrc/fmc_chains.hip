@@ -161,6 +161,42 @@ __device__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg&
   if (r == 0 && io.logp) io.logp[c] = lp;
 }
 
+// ------------------------------------------------------------------- GRW ---
+// GaussianRandomWalk.kernel (src/flowMC/resource/kernel/Gaussian_random_walk.py:
+// 25-61) scanned by TakeSerialSteps: x' = x + step * xi, accept iff
+// log u < logp(x') - logp(x). The log-density of the entry state is evaluated
+// once (take_steps.py:177) and carried; the target's gradient output is dead
+// and removed by the compiler.
+template <int NV>
+__device__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
+                          int r, int lpc, int gbase) {
+  const int d = t.d;
+  float4 x[NV], g[NV];
+  load_row<NV>(io.x + c * d, d, r, lpc, x);
+  float lp = target_vg<NV>(t, x, g, r, lpc, gbase);
+  const float dt = cfg.dt;
+  for (int s = 0; s < cfg.n_steps; ++s) {
+    const int tg = cfg.step0 + s;
+    float4 xi[NV], prop[NV];
+    draw_normals<NV>(io, d, r, lpc, c, s, tg, RNG_GRW_NOISE, xi);
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) f4(prop[v], e) = f4c(x[v], e) + dt * f4c(xi[v], e);
+    const float lp1 = target_vg<NV>(t, prop, g, r, lpc, gbase);
+    const float lu = draw_log_u(io, c, s, tg, RNG_GRW_UNIFORM);
+    const bool acc = lu < lp1 - lp;
+    if (acc) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x[v] = prop[v];
+      lp = lp1;
+    }
+    write_rows<NV>(io, d, r, lpc, c, tg, cfg.thinning, x, lp, acc);
+  }
+  store_row<NV>(io.x + c * d, d, r, lpc, x);
+  if (r == 0 && io.logp) io.logp[c] = lp;
+}
+
 // ------------------------------------------------------------------- HMC ---
 // HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) with the leapfrog
 // coefficient table of HMC.py:47-50: row 0 = (0, 1/2), rows 1..L = (1, 1),
@@ -259,7 +295,7 @@ __device__ __forceinline__ TargetDev stage_target(const fmc_target& tg, float* s
   return t;
 }
 
-enum KernelKind { KK_MALA = 0, KK_HMC = 1 };
+enum KernelKind { KK_MALA = 0, KK_HMC = 1, KK_GRW = 2 };
 
 template <int NV, int KIND>
 __global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, StepCfg cfg,
@@ -273,6 +309,8 @@ __global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, 
   if (c >= io.C) return;  // whole group leaves together
   if (KIND == KK_MALA)
     mala_group<NV>(t, io, cfg, c, r, lpc, gbase);
+  else if (KIND == KK_GRW)
+    grw_group<NV>(t, io, cfg, c, r, lpc, gbase);
   else
     hmc_group<NV>(t, io, cfg, c, r, lpc, gbase);
 }
@@ -381,6 +419,41 @@ __global__ void mala_accept_kernel(float* x, float* logp, float* g, const float*
   }
   if (lane == 0) {
     float lpv = acc ? lp1[c] : logp[c];
+    logp[c] = lpv;
+    if (wr && out_logp) out_logp[c * buf_len + row] = lpv;
+    if (wr && out_acc) out_acc[c * buf_len + row] = acc ? 1.f : 0.f;
+  }
+}
+
+// Gaussian random walk pieces for targets evaluated outside the library.
+__global__ void grw_propose_kernel(const float* __restrict__ x, float* __restrict__ prop, int64_t C, int d,
+                                   float dt, uint32_t k0, uint32_t k1, int step, const float* noise) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C * d) return;
+  const int64_t c = i / d;
+  const int j = (int)(i - c * d);
+  prop[i] = x[i] + dt * noise_at(noise, k0, k1, c, C, d, j, step, RNG_GRW_NOISE);
+}
+
+__global__ void grw_accept_kernel(float* x, float* logp, const float* __restrict__ prop,
+                                  const float* __restrict__ lp1, int64_t C, int d, uint32_t k0, uint32_t k1,
+                                  int step, const float* log_u, float* out_pos, float* out_logp,
+                                  float* out_acc, int64_t buf_len, int64_t cursor, int thinning) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= C) return;
+  const float lu = log_u ? log_u[c] : philox_log_uniform(k0, k1, (uint32_t)c, (uint32_t)step, RNG_GRW_UNIFORM);
+  const bool acc = lu < lp1[c] - logp[c];
+  const bool wr = (step % thinning) == 0;
+  const int64_t row = cursor + step / thinning;
+  for (int j = lane; j < d; j += 64) {
+    const int64_t k = c * d + j;
+    const float xv = acc ? prop[k] : x[k];
+    if (acc) x[k] = xv;
+    if (wr && out_pos) out_pos[(c * buf_len + row) * d + j] = xv;
+  }
+  if (lane == 0) {
+    const float lpv = acc ? lp1[c] : logp[c];
     logp[c] = lpv;
     if (wr && out_logp) out_logp[c * buf_len + row] = lpv;
     if (wr && out_acc) out_acc[c * buf_len + row] = acc ? 1.f : 0.f;
@@ -627,7 +700,7 @@ static int launch_chain(hipStream_t st, const fmc_target* t, const ChainIO& io, 
     case 4: hipLaunchKernelGGL((chain_kernel<4, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
     default: hipLaunchKernelGGL((chain_kernel<8, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
   }
-  return check_launch(KIND == KK_MALA ? "mala chain kernel" : "hmc chain kernel");
+  return check_launch(KIND == KK_MALA ? "mala chain kernel" : KIND == KK_GRW ? "grw chain kernel" : "hmc chain kernel");
 }
 
 static int check_buffers(const float* out_pos, const float* out_logp, const float* out_acc,
@@ -649,6 +722,8 @@ static int check_buffers(const float* out_pos, const float* out_logp, const floa
 }  // namespace fmc
 
 using namespace fmc;
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 extern "C" int fmc_target_eval(void* stream, const fmc_target* target, const float* x, int64_t n_points,
                                float* logp, float* grad, const fmc_launch_cfg* cfg) {
@@ -691,6 +766,52 @@ extern "C" int fmc_mala_run(void* stream, const fmc_target* target, float* x, fl
   ChainIO io{x, logp, nullptr, noise, log_u, out_pos, out_logp, out_acc, n_chains, buf_len, cursor, key0, key1};
   StepCfg sc{n_steps, step0, step_size, 0, thinning};
   return launch_chain<KK_MALA>((hipStream_t)stream, target, io, sc, L);
+}
+
+extern "C" int fmc_grw_run(void* stream, const fmc_target* target, float* x, float* logp, int64_t n_chains,
+                           int32_t n_steps, int32_t step0, float step_size, uint32_t key0, uint32_t key1,
+                           const float* noise, const float* log_u, float* out_pos, float* out_logp,
+                           float* out_acc, int64_t buf_len, int64_t cursor, int32_t thinning,
+                           const fmc_launch_cfg* cfg) {
+  int rc = check_target(target);
+  if (rc) return rc;
+  FMC_ARG(x != nullptr, "x is NULL");
+  FMC_ARG(n_chains >= 0, "n_chains < 0");
+  FMC_ARG(step_size > 0.f, "step_size must be > 0");
+  FMC_ARG((noise == nullptr) == (log_u == nullptr), "noise and log_u must be both given or both NULL");
+  if ((rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, step0, n_steps))) return rc;
+  if (n_chains == 0) return FMC_OK;
+  Layout L = choose_layout(target->n_dim, target->kind, cfg);
+  if ((rc = check_layout(L, target->n_dim))) return rc;
+  ChainIO io{x, logp, nullptr, noise, log_u, out_pos, out_logp, out_acc, n_chains, buf_len, cursor, key0, key1};
+  StepCfg sc{n_steps, step0, step_size, 0, thinning};
+  return launch_chain<KK_GRW>((hipStream_t)stream, target, io, sc, L);
+}
+
+extern "C" int fmc_grw_propose(void* stream, const float* x, float* prop, int64_t n_chains, int32_t n_dim,
+                               float step_size, uint32_t key0, uint32_t key1, int32_t step, const float* noise) {
+  FMC_ARG(x && prop, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1 && step >= 0, "bad sizes");
+  int64_t n = n_chains * n_dim;
+  if (n == 0) return FMC_OK;
+  hipLaunchKernelGGL(grw_propose_kernel, dim3(grid1(n, 256)), dim3(256), 0, (hipStream_t)stream, x, prop,
+                     n_chains, n_dim, step_size, key0, key1, step, noise);
+  return check_launch("grw propose");
+}
+
+extern "C" int fmc_grw_accept(void* stream, float* x, float* logp, const float* prop, const float* prop_logp,
+                              int64_t n_chains, int32_t n_dim, uint32_t key0, uint32_t key1, int32_t step,
+                              const float* log_u, float* out_pos, float* out_logp, float* out_acc, int64_t buf_len,
+                              int64_t cursor, int32_t thinning) {
+  FMC_ARG(x && logp && prop && prop_logp, "NULL pointer");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1, "bad sizes");
+  int rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, step, 1);
+  if (rc) return rc;
+  if (n_chains == 0) return FMC_OK;
+  hipLaunchKernelGGL(grw_accept_kernel, dim3(grid1(n_chains, 4)), dim3(256), 0, (hipStream_t)stream, x, logp,
+                     prop, prop_logp, n_chains, n_dim, key0, key1, step, log_u, out_pos, out_logp, out_acc,
+                     buf_len, cursor, thinning);
+  return check_launch("grw accept");
 }
 
 extern "C" int fmc_hmc_run(void* stream, const fmc_target* target, float* x, float* logp, int64_t n_chains,
@@ -739,7 +860,6 @@ extern "C" int fmc_hmc_run_batched(void* stream, const fmc_hmc_slot* slots, int3
   return check_launch("batched hmc kernel");
 }
 
-static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 extern "C" int fmc_mala_propose(void* stream, const float* x, const float* grad, float* prop, int64_t n_chains,
                                 int32_t n_dim, float step_size, uint32_t key0, uint32_t key1, int32_t step,

@@ -79,6 +79,8 @@ enum RngStream : uint32_t {
   RNG_HMC_UNIFORM = 3,
   RNG_NF_UNIFORM = 4,
   RNG_FLOW_BASE = 5,
+  RNG_GRW_NOISE = 7,
+  RNG_GRW_UNIFORM = 8,
 };
 
 // Four standard normals for dims 4*chunk .. 4*chunk+3 of (chain, step):

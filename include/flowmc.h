@@ -106,6 +106,22 @@ int fmc_mala_run(void* stream, const fmc_target* target, float* x, float* logp,
                  int64_t cursor, int32_t thinning, const fmc_launch_cfg* cfg);
 
 /* ------------------------------------------------------------------------ */
+/* Fused K-step Gaussian random walk.                                       */
+/* Replaces TakeSerialSteps.sample (take_steps.py:164-180) scanning          */
+/* GaussianRandomWalk.kernel (src/flowMC/resource/kernel/                   */
+/* Gaussian_random_walk.py:25-61): x' = x + step_size * xi, accept iff      */
+/* log u < logp(x') - logp(x). Same arguments, state handling, noise layout */
+/* and buffer rows as fmc_mala_run; the log-density of the entry state is   */
+/* evaluated once (take_steps.py:177) and carried. Philox streams 7 (xi)    */
+/* and 8 (u).                                                               */
+/* ------------------------------------------------------------------------ */
+int fmc_grw_run(void* stream, const fmc_target* target, float* x, float* logp,
+                int64_t n_chains, int32_t n_steps, int32_t step0, float step_size,
+                uint32_t key0, uint32_t key1, const float* noise, const float* log_u,
+                float* out_pos, float* out_logp, float* out_acc, int64_t buf_len,
+                int64_t cursor, int32_t thinning, const fmc_launch_cfg* cfg);
+
+/* ------------------------------------------------------------------------ */
 /* Fused K-step HMC.                                                        */
 /* Replaces HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) with its  */
 /* leapfrog scan (HMC.py:71-96) and coefficient table (HMC.py:47-50:        */
@@ -171,6 +187,15 @@ int fmc_mala_accept(void* stream, float* x, float* logp, float* grad, const floa
                     int32_t n_dim, float step_size, uint32_t key0, uint32_t key1,
                     int32_t step, const float* log_u, float* out_pos, float* out_logp,
                     float* out_acc, int64_t buf_len, int64_t cursor, int32_t thinning);
+/* Gaussian random walk proposal prop = x + step_size * xi
+ * (Gaussian_random_walk.py:48-53) and its MH accept with thinned buffer
+ * write (:55-60); x/logp are updated in place. */
+int fmc_grw_propose(void* stream, const float* x, float* prop, int64_t n_chains, int32_t n_dim,
+                    float step_size, uint32_t key0, uint32_t key1, int32_t step, const float* noise);
+int fmc_grw_accept(void* stream, float* x, float* logp, const float* prop, const float* prop_logp,
+                   int64_t n_chains, int32_t n_dim, uint32_t key0, uint32_t key1, int32_t step,
+                   const float* log_u, float* out_pos, float* out_logp, float* out_acc,
+                   int64_t buf_len, int64_t cursor, int32_t thinning);
 /* HMC pieces (HMC.py:128-149): momentum draw + kinetic energy (FP64),
  * drift x += a * metric * p, kick p -= b * grad, and the FP64 accept. */
 int fmc_hmc_momentum(void* stream, float* p, double* kinetic, int64_t n_chains,

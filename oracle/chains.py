@@ -72,6 +72,19 @@ def hmc_step(x, lp, fn, eps, n_leapfrog, metric, xi, log_u):
     return x_new, lp_new, acc, log_acc
 
 
+def grw_step(x, lp, fn, step_size, xi, log_u):
+    """One Gaussian random walk step — src/flowMC/resource/kernel/Gaussian_random_walk.py:25-61:
+    proposal = x + step_size * xi; accept iff log u < logp(proposal) - log_prob,
+    with the passed-in log_prob of the current point."""
+    prop = x + step_size * xi
+    lp1, _ = value_and_grad(fn, prop)
+    ratio = lp1 - lp
+    acc = log_u < ratio
+    x_new = torch.where(acc[..., None], prop, x)
+    lp_new = torch.where(acc, lp1, lp)
+    return x_new, lp_new, acc, ratio
+
+
 def take_serial_steps(kind, x0, fn, noise, log_u, thinning=1, **kw):
     """TakeSerialSteps.sample + TakeSteps thinning — src/flowMC/strategy/take_steps.py:134-180.
 
@@ -89,6 +102,8 @@ def take_serial_steps(kind, x0, fn, noise, log_u, thinning=1, **kw):
         elif kind == "hmc":
             x, lp, acc, r = hmc_step(x, lp, fn, kw["step_size"], kw["n_leapfrog"], kw["metric"], noise[s],
                                      log_u[s])
+        elif kind == "grw":
+            x, lp, acc, r = grw_step(x, lp, fn, kw["step_size"], noise[s], log_u[s])
         else:
             raise ValueError(kind)
         pos.append(x)

@@ -22,6 +22,9 @@ RNG_HMC_MOMENTUM = 2
 RNG_HMC_UNIFORM = 3
 RNG_NF_UNIFORM = 4
 RNG_FLOW_BASE = 5
+RNG_DIRECT_NORMAL = 6
+RNG_GRW_NOISE = 7
+RNG_GRW_UNIFORM = 8
 
 
 def philox4x32_10(ctr, key) -> np.ndarray:

@@ -20,6 +20,7 @@ from flowmc_amd import random as fr
 from flowmc_amd import targets as TG
 from flowmc_amd.resource.buffers import Buffer
 from flowmc_amd.resource.kernel.base import StepOutputs
+from flowmc_amd.resource.kernel.Gaussian_random_walk import GaussianRandomWalk
 from flowmc_amd.resource.kernel.HMC import HMC
 from flowmc_amd.resource.kernel.MALA import MALA
 from flowmc_amd.resource.logPDF import LogPDF
@@ -124,6 +125,8 @@ def _compare_traj(kind, name, d, C_, S, dt, thinning=1, n_leapfrog=5, seed=0):
     out = StepOutputs(pos, lps, accs, 2)
     if kind == "mala":
         prop = MALA(dt, fused_steps=32)
+    elif kind == "grw":
+        prop = GaussianRandomWalk(dt, fused_steps=32)
     else:
         prop = HMC(metric.to(torch.float32), dt, n_leapfrog, fused_steps=8)
     logpdf = LogPDF(tgt, n_dims=d)
@@ -170,6 +173,68 @@ def test_hmc_fused_matches_oracle(gpu, name, d):
     dt = {"gauss_diag": 0.2, "funnel": 0.05, "rosenbrock": 0.02, "gmm": 0.1}[name]
     acc = _compare_traj("hmc", name, d, 128, 40, dt, n_leapfrog=6)
     assert float(acc.mean()) > 0.05
+
+
+@pytest.mark.parametrize("name,d", [("gauss_dense", 5), ("rosenbrock", 128), ("gmm", 64), ("funnel", 20),
+                                    ("gauss_diag", 33)])
+def test_grw_fused_matches_oracle(gpu, name, d):
+    dt = {"gauss_dense": 0.5, "rosenbrock": 0.02, "gmm": 0.1, "funnel": 0.2, "gauss_diag": 0.3}[name]
+    acc = _compare_traj("grw", name, d, 192, 100, dt)
+    assert 0.05 < float(acc.mean()) < 1.0
+
+
+def test_grw_thinning_and_cursor(gpu):
+    _compare_traj("grw", "gauss_diag", 6, 64, 23, 0.5, thinning=4)
+
+
+def test_grw_generic_target_matches_oracle(gpu):
+    # user-supplied per-point torch callable: propose / accept kernels around a batched evaluation
+    d, C_, S, dt = 4, 96, 40, 0.6
+    g = torch.Generator().manual_seed(21)
+    x0 = torch.randn(C_, d, generator=g)
+    noise = torch.randn(S, C_, d, generator=g)
+    log_u = torch.log(torch.rand(S, C_, generator=g))
+    mean = torch.tensor([0.5, -1.0, 0.0, 2.0])
+
+    def user_logpdf(x, data):
+        return -0.5 * torch.sum((x - data["mu"]) ** 2 / torch.arange(1, 5, device=x.device, dtype=x.dtype))
+
+    pos = torch.zeros(C_, S, d, device=gpu)
+    lps = torch.zeros(C_, S, device=gpu)
+    accs = torch.zeros(C_, S, device=gpu)
+    GaussianRandomWalk(dt).sample_steps(fr.PRNGKey(0), x0.cuda(), LogPDF(user_logpdf, n_dims=d),
+                                        {"mu": mean.cuda()}, S, 1, StepOutputs(pos, lps, accs, 0),
+                                        noise=noise.cuda(), log_u=log_u.cuda())
+    fn = lambda z: -0.5 * (((z - mean.double()) ** 2) / torch.arange(1, 5, dtype=torch.float64)).sum(-1)
+    r_pos, r_lp, r_acc, _, ratios = OC.take_serial_steps("grw", x0.double(), fn, noise.double(), log_u.double(),
+                                                         step_size=dt)
+    ok = ~((ratios - log_u.double().T).abs() < MARGIN).any(1)
+    assert int((~ok).sum()) <= 3
+    assert torch.equal(accs.cpu().double()[ok], r_acc[ok])
+    assert ((pos.cpu().double()[ok] - r_pos[ok]).abs() <= 2e-4 * (1 + r_pos[ok].abs())).all()
+    assert ((lps.cpu().double()[ok] - r_lp[ok]).abs() <= 2e-4 * (1 + r_lp[ok].abs())).all()
+
+
+def test_grw_philox_mode_equals_injected_mode(gpu):
+    tgt, _ = make_target("gmm", 16)
+    C_, S, d = 300, 24, 16
+    key = fr.PRNGKey(98)
+    k0, k1 = fr.key_words(key)
+    x0 = torch.randn(C_, d, device=gpu)
+    noise = torch.empty(S, C_, d, device=gpu, dtype=torch.float32)
+    lu = torch.empty(S, C_, device=gpu, dtype=torch.float32)
+    N.call("fmc_rng_normal", N.stream_ptr(), k0, k1, C_, d, S, 0, OP.RNG_GRW_NOISE, noise.data_ptr())
+    N.call("fmc_rng_log_uniform", N.stream_ptr(), k0, k1, C_, S, 0, OP.RNG_GRW_UNIFORM, lu.data_ptr())
+    res = []
+    for nz, l in ((None, None), (noise, lu)):
+        pos = torch.zeros(C_, S, d, device=gpu, dtype=torch.float32)
+        lp = torch.zeros(C_, S, device=gpu, dtype=torch.float32)
+        ac = torch.zeros(C_, S, device=gpu, dtype=torch.float32)
+        GaussianRandomWalk(0.2, fused_steps=8).sample_steps(key, x0, LogPDF(tgt, n_dims=d), None, S, 1,
+                                                            StepOutputs(pos, lp, ac, 0), noise=nz, log_u=l)
+        res.append((pos, lp, ac))
+    for a, b in zip(*res):
+        assert torch.equal(a, b)
 
 
 def test_philox_mode_equals_injected_mode(gpu):
@@ -281,6 +346,28 @@ def test_hmc_close_gaussian(gpu):
     res, _, _ = _strategy_run(HMC(torch.eye(2), 0.1, 5), logpdf, 1, 30000, 2)
     data = res["positions"].data
     assert abs(float(data.mean())) < 3e-2 and abs(float(data.var()) - 1) < 3e-2
+
+
+def test_grw_close_gaussian(gpu):
+    # test/unit/test_kernels.py:352-392: N(0, I), step 1; 64 chains x 20k steps
+    logpdf = LogPDF(TG.GaussianDiag(torch.zeros(2, dtype=torch.float32)), n_dims=2)
+    res, _, strat = _strategy_run(GaussianRandomWalk(1.0), logpdf, 64, 20000, 2)
+    data = res["positions"].data
+    assert abs(float(data.mean())) < 3e-2 and abs(float(data.var()) - 1) < 3e-2
+    assert strat.current_position == 20000
+
+
+def test_grw_kernel_api(gpu):
+    # test_kernels.py:293-350: repr, deterministic kernel, acceptance -> 1 as the step -> 0
+    logpdf = LogPDF(TG.GaussianDiag(torch.zeros(2, dtype=torch.float32)), n_dims=2)
+    x = fr.normal(fr.PRNGKey(4), (1, 2))[0]
+    k = GaussianRandomWalk(step_size=1)
+    r1 = k.kernel(fr.PRNGKey(7), x, logpdf(x), logpdf, None)
+    r2 = k.kernel(fr.PRNGKey(7), x, logpdf(x), logpdf, None)
+    assert torch.equal(r1[0], r2[0]) and bool(r1[1] == r2[1]) and bool(r1[2] == r2[2])
+    xs = fr.normal(fr.PRNGKey(1), (100, 2))
+    _, _, acc = GaussianRandomWalk(1e-5).kernel(fr.PRNGKey(2), xs, logpdf(xs), logpdf, None)
+    assert bool(acc.all())
 
 
 def test_acceptance_goes_to_one(gpu):

@@ -139,3 +139,55 @@ def test_dynamic_update_slice_clamps():
     buf = torch.zeros(2, 5)
     out = chains.dynamic_update_slice(buf, torch.ones(2, 3), 4, 1)
     assert torch.equal(out[:, 2:], torch.ones(2, 3)) and float(out[:, :2].abs().sum()) == 0
+
+
+def test_grw_detailed_balance_gaussian():
+    # symmetric proposal: pi(x) q(y|x) min(1, e^r) == pi(y) q(x|y) min(1, e^-r),
+    # with pi and q from scipy (independent of the oracle's target code)
+    rng = np.random.default_rng(1)
+    d, step = 3, 0.8
+    A = rng.normal(size=(d, d))
+    cov = A @ A.T + np.eye(d)
+    P = np.linalg.inv(cov)
+    mu = rng.normal(size=d)
+    fn = lambda z: T.gauss_dense(z, torch.tensor(mu), torch.tensor(P))
+    pi = ss.multivariate_normal(mu, cov)
+    for _ in range(20):
+        x = torch.tensor(rng.normal(size=(1, d)))
+        xi = torch.tensor(rng.normal(size=(1, d)))
+        lp_x = torch.tensor([float(T.value_and_grad(fn, x)[0][0])])
+        _, _, _, r = chains.grw_step(x, lp_x, fn, step, xi, torch.tensor([-1e300]))
+        y = (x + step * xi).numpy()[0]
+        q = ss.multivariate_normal(np.zeros(d), step * step * np.eye(d))
+        lhs = pi.pdf(x.numpy()[0]) * q.pdf(y - x.numpy()[0]) * min(1.0, math.exp(float(r[0])))
+        rhs = pi.pdf(y) * q.pdf(x.numpy()[0] - y) * min(1.0, math.exp(-float(r[0])))
+        assert lhs == pytest.approx(rhs, rel=1e-9)
+        # the ratio is the log-density difference (the proposal is symmetric)
+        assert float(r[0]) == pytest.approx(pi.logpdf(y) - pi.logpdf(x.numpy()[0]), rel=1e-9, abs=1e-9)
+
+
+def test_grw_uses_the_passed_log_prob_and_accepts_tiny_steps():
+    g = torch.Generator().manual_seed(2)
+    fn = lambda z: -0.5 * (z * z).sum(-1)
+    x = torch.randn(200, 2, generator=g)
+    lp = fn(x)
+    xi = torch.randn(200, 2, generator=g)
+    lu = torch.log(torch.rand(200, generator=g))
+    _, _, acc, _ = chains.grw_step(x, lp, fn, 1e-5, xi, lu)
+    assert bool(acc.all())
+    # a log_prob far above the truth makes every proposal look worse: nothing accepted
+    _, lp2, acc2, _ = chains.grw_step(x, lp + 1e3, fn, 1e-5, xi, lu)
+    assert not bool(acc2.any()) and torch.equal(lp2, lp + 1e3)
+
+
+def test_grw_moments_standard_normal():
+    # reference test_kernels.py:352-392 (mean 0, variance 1 within 3e-2), many chains
+    g = torch.Generator().manual_seed(3)
+    C, d, S = 4000, 2, 400
+    fn = lambda z: -0.5 * (z * z).sum(-1)
+    x = torch.randn(C, d, generator=g)
+    noise = torch.randn(S, C, d, generator=g)
+    lu = torch.log(torch.rand(S, C, generator=g))
+    pos, _, _, _, _ = chains.take_serial_steps("grw", x, fn, noise, lu, step_size=1.0)
+    tail = pos[:, 100:]
+    assert abs(float(tail.mean())) < 3e-2 and abs(float(tail.var()) - 1.0) < 3e-2
