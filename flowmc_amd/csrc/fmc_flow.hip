@@ -81,6 +81,26 @@ struct PBF16 {
 
 __device__ __forceinline__ int out_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
 
+// (m, j) walk over an MT x n tile by the 256 threads of a workgroup, without
+// a division per element: start at tid, advance by 256 elements.
+struct TileWalk {
+  int m, j, dm, dj, n;
+  __device__ TileWalk(int tid, int n_) : n(n_) {
+    m = tid / n;
+    j = tid - m * n;
+    dm = 256 / n;
+    dj = 256 - dm * n;
+  }
+  __device__ void next() {
+    j += dj;
+    m += dm;
+    if (j >= n) {
+      j -= n;
+      ++m;
+    }
+  }
+};
+
 // One 32x32 block: acc += A[row0:row0+32, kb0*KSTEP : kb1*KSTEP] * B-image block column nb.
 template <class PT>
 __device__ __forceinline__ void mfma_block(const typename PT::T* As, int lda, int row0, const typename PT::T* Bp,
@@ -673,8 +693,8 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   const T* pk = reinterpret_cast<const T*>(f.packed);
 
   // ---- load tile (+ standardisation / base sampling)
-  for (int i = tid; i < MT * d; i += 256) {
-    int m = i / d, j = i - m * d;
+  for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+    const int m = tw.m, j = tw.j;
     int64_t row = m0 + m;
     float v = 0.f;
     if (row < a.n) {
@@ -716,8 +736,8 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   for (int step = 0; step < dm.L; ++step) {
     const int l = (a.dir == 0) ? step : dm.L - 1 - step;
     if (a.saved) {
-      for (int i = tid; i < MT * d; i += 256) {
-        int m = i / d, j = i - m * d;
+      for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+        const int m = tw.m, j = tw.j;
         if (m0 + m < a.n) a.saved[((int64_t)l * a.n + m0 + m) * d + j] = xs[m * ldx + j];
       }
     }
@@ -726,8 +746,8 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     // ---- ScalarAffine (forward / reference inverse apply it before the spline)
     if (affine_first) {
       const float e = expf(a.dir == 0 ? sc : -sc);
-      for (int i = tid; i < MT * d; i += 256) {
-        int m = i / d, j = i - m * d;
+      for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+        const int m = tw.m, j = tw.j;
         float v = xs[m * ldx + j];
         xs[m * ldx + j] = (a.dir == 0) ? (v + sh) * e : v * e - sh;
       }
@@ -736,8 +756,8 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     }
     // ---- conditioner input (conditioning dims only)
     const int dc = dm.dc(l), kc = dm.kc(l);
-    for (int i = tid; i < MT * kc; i += 256) {
-      int m = i / kc, k = i - m * kc;
+    for (TileWalk tw(tid, kc); tw.m < MT; tw.next()) {
+      const int m = tw.m, k = tw.j;
       a0[m * sm.la0 + k] = PT::cvt(k < dc ? xs[m * ldx + dm.cond(l, k)] : 0.f);
     }
     __syncthreads();
@@ -888,8 +908,8 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     }
     if (!affine_first) {  // exact inverse: affine^-1 after spline^-1
       const float e = expf(-sc);
-      for (int i = tid; i < MT * d; i += 256) {
-        int m = i / d, j = i - m * d;
+      for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+        const int m = tw.m, j = tw.j;
         xs[m * ldx + j] = xs[m * ldx + j] * e - sh;
       }
       for (int m = tid; m < MT; m += 256) ldr[m] += -sc * (float)d;
@@ -917,8 +937,8 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     if (a.logdet) a.logdet[row] = ld;
   }
   if (a.x_out) {
-    for (int i = tid; i < MT * d; i += 256) {
-      int m = i / d, j = i - m * d;
+    for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+      const int m = tw.m, j = tw.j;
       int64_t row = m0 + m;
       if (row >= a.n) continue;
       float v = xs[m * ldx + j];
@@ -1031,8 +1051,8 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
   const float es = expf(sc);
 
   // ---- load x (layer input), gy, g_logp; apply the affine part
-  for (int i = tid; i < MT * d; i += 256) {
-    int m = i / d, j = i - m * d;
+  for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+    const int m = tw.m, j = tw.j;
     int64_t row = m0 + m;
     bool ok = row < a.n;
     xs[m * ldx + j] = ok ? (a.x_in[row * d + j] + sh) * es : 0.f;
@@ -1044,8 +1064,8 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
   const int dc = dm.dc(l), kc = dm.kc(l);
   T* a0 = gpb;
   const int la0 = sm.lg;
-  for (int i = tid; i < MT * kc; i += 256) {
-    int m = i / kc, k = i - m * kc;
+  for (TileWalk tw(tid, kc); tw.m < MT; tw.next()) {
+    const int m = tw.m, k = tw.j;
     a0[m * la0 + k] = PT::cvt(k < dc ? xs[m * ldx + dm.cond(l, k)] : 0.f);
   }
   __syncthreads();
@@ -1322,8 +1342,8 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
   __syncthreads();
   // ---- affine backward: xa = (x + shift) e^s, ld += d*s
   float ps = 0.f, pt = 0.f;
-  for (int i = tid; i < MT * d; i += 256) {
-    int m = i / d, j = i - m * d;
+  for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+    const int m = tw.m, j = tw.j;
     float g = gb[m * ldx + j];
     ps += g * xs[m * ldx + j];
     pt += g * es;
