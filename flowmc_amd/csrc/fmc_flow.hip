@@ -22,6 +22,7 @@
 namespace fmc {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 // --------------------------------------------------------- MFMA traits ---
@@ -50,6 +51,13 @@ struct PF32 {
   static __device__ __forceinline__ float f(T x) { return x; }
   static __device__ __forceinline__ float hsum(float x) { return x; }
   static __device__ __forceinline__ float act(float x) { return tanhf(x); }
+  // 4 consecutive elements (rows are not 16-byte aligned: 4 scalar stores)
+  static __device__ __forceinline__ void store4(T* dst, float a, float b, float c, float d) {
+    dst[0] = a;
+    dst[1] = b;
+    dst[2] = c;
+    dst[3] = d;
+  }
 };
 
 struct PBF16 {
@@ -69,6 +77,15 @@ struct PBF16 {
   }
   static __device__ __forceinline__ T cvt(float x) { return (__bf16)x; }
   static __device__ __forceinline__ float f(T x) { return (float)x; }
+  // 4 consecutive elements, one 8-byte store (dst is 8-byte aligned)
+  static __device__ __forceinline__ void store4(T* dst, float a, float b, float c, float d) {
+    bf16x4_t v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    v[2] = (__bf16)c;
+    v[3] = (__bf16)d;
+    *reinterpret_cast<bf16x4_t*>(dst) = v;
+  }
   static __device__ __forceinline__ float hsum(bf16x8_t x) {
     return (((float)x[0] + (float)x[1]) + ((float)x[2] + (float)x[3])) +
            (((float)x[4] + (float)x[5]) + ((float)x[6] + (float)x[7]));
@@ -80,6 +97,25 @@ struct PBF16 {
 };
 
 __device__ __forceinline__ int out_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+// Epilogue of a transposed hidden-layer GEMM (output rows = features, columns
+// = samples): lane l holds, for sample r*32 + l%32, features
+// fb*32 + 8g + 4(l/32) + i (g, i = 0..3) in acc[4g + i]; writes
+// tanh(acc + bias) as 4-element runs into the [sample][feature] LDS rows.
+template <class PT, int MB>
+__device__ __forceinline__ void hidden_epilogue_tr(typename PT::T* hb, int lh, const float* bias, int fb,
+                                                   const f32x16_t (&acc)[MB], int lane) {
+  const int f0 = fb * 32 + 4 * (lane >> 5);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 b4 = *reinterpret_cast<const float4*>(bias + f0 + 8 * g);
+#pragma unroll
+    for (int r = 0; r < MB; ++r)
+      PT::store4(hb + (r * 32 + (lane & 31)) * lh + f0 + 8 * g, PT::act(acc[r][4 * g] + b4.x),
+                 PT::act(acc[r][4 * g + 1] + b4.y), PT::act(acc[r][4 * g + 2] + b4.z),
+                 PT::act(acc[r][4 * g + 3] + b4.w));
+  }
+}
 
 // (m, j) walk over an MT x n tile by the 256 threads of a workgroup, without
 // a division per element: start at tid, advance by 256 elements.
@@ -400,20 +436,24 @@ struct RsBin {
 __device__ __forceinline__ float rs_softplus(float v) { return fmaxf(v, 0.f) + __logf(1.f + __expf(-fabsf(v))); }
 
 // knots of the bin holding t (t along x when INV is false, along y otherwise)
-template <bool INV>
-__device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int K, const FlowDev& f, float t) {
+// KT > 0: the bin count is the compile-time constant KT (loops unrolled to
+// exactly KT, no predication); KT == 0: runtime K <= 10, loops predicated.
+template <bool INV, int KT>
+__device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int Kr, const FlowDev& f, float t) {
+  constexpr int N = KT > 0 ? KT : 10;
+  const int K = KT > 0 ? KT : Kr;
   float mw = -INFINITY, mh = -INFINITY;
 #pragma unroll
-  for (int i = 0; i < 10; ++i)
-    if (i < K) {
+  for (int i = 0; i < N; ++i)
+    if (KT > 0 || i < K) {
       mw = fmaxf(mw, pp[i]);
       mh = fmaxf(mh, pp[10 + i]);
     }
   float ew[10], eh[10], sw = 0.f, sh = 0.f;
 #pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    ew[i] = (i < K) ? __expf(pp[i] - mw) : 0.f;
-    eh[i] = (i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+  for (int i = 0; i < N; ++i) {
+    ew[i] = (KT > 0 || i < K) ? __expf(pp[i] - mw) : 0.f;
+    eh[i] = (KT > 0 || i < K) ? __expf(pp[10 + i] - mh) : 0.f;
     sw += ew[i];
     sh += eh[i];
   }
@@ -423,8 +463,8 @@ __device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int K, const Flow
   float xc = f.lo, yc = f.lo;
   bool found = false;
 #pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    if (i < K) {
+  for (int i = 0; i < N; ++i) {
+    if (KT > 0 || i < K) {
       const float xn = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
       const float yn = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
       const bool hit = !found && (INV ? (t < yn) : (t < xn));
@@ -444,6 +484,7 @@ __device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int K, const Flow
   return b;
 }
 
+template <int KT>
 __device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowDev& f, float x, float& y, float& ld) {
   if (x <= f.lo || x >= f.hi) {
     const float u = (x <= f.lo) ? pp[20] : pp[30];
@@ -453,7 +494,7 @@ __device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowD
     ld = __logf(dd);
     return;
   }
-  const RsBin b = rs_bin<false>(pp, K, f, x);
+  const RsBin b = rs_bin<false, KT>(pp, K, f, x);
   const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
   const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = hh / w;
   const float z = fminf(fmaxf((x - b.xk) / w, 0.f), 1.f);
@@ -465,6 +506,7 @@ __device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowD
   ld = 2.f * __logf(s) + __logf(dk1 * z2 + 2.f * s * zz + dk * q) - 2.f * __logf(den);
 }
 
+template <int KT>
 __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowDev& f, float yv, float& x, float& ld) {
   if (yv <= f.lo || yv >= f.hi) {
     const float u = (yv <= f.lo) ? pp[20] : pp[30];
@@ -474,7 +516,7 @@ __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowD
     ld = -__logf(dd);
     return;
   }
-  const RsBin b = rs_bin<true>(pp, K, f, yv);
+  const RsBin b = rs_bin<true, KT>(pp, K, f, yv);
   const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
   const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = hh / w;
   const float wv = fminf(fmaxf((yv - b.yk) / hh, 0.f), 1.f);
@@ -495,8 +537,11 @@ __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowD
 // Register-spline forward + vector-Jacobian product (the reverse sweep of
 // spline_vjp): given dL/dy (gy) and dL/dlogdet (gl), writes dL/d(slot
 // logits) to go (zero in padding slots) and returns dL/dx.
-__device__ __forceinline__ float rs_vjp(const float (&pp)[32], int K, const FlowDev& f, float x, float gy, float gl,
+template <int KT>
+__device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const FlowDev& f, float x, float gy, float gl,
                                         float (&go)[32]) {
+  constexpr int N = KT > 0 ? KT : 10;
+  const int K = KT > 0 ? KT : Kr;
 #pragma unroll
   for (int i = 0; i < 32; ++i) go[i] = 0.f;
   if (x <= f.lo || x >= f.hi) {
@@ -511,16 +556,16 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int K, const Flow
   }
   float mw = -INFINITY, mh = -INFINITY;
 #pragma unroll
-  for (int i = 0; i < 10; ++i)
-    if (i < K) {
+  for (int i = 0; i < N; ++i)
+    if (KT > 0 || i < K) {
       mw = fmaxf(mw, pp[i]);
       mh = fmaxf(mh, pp[10 + i]);
     }
   float ew[10], eh[10], sw = 0.f, sh = 0.f;
 #pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    ew[i] = (i < K) ? __expf(pp[i] - mw) : 0.f;
-    eh[i] = (i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+  for (int i = 0; i < N; ++i) {
+    ew[i] = (KT > 0 || i < K) ? __expf(pp[i] - mw) : 0.f;
+    eh[i] = (KT > 0 || i < K) ? __expf(pp[10 + i] - mh) : 0.f;
     sw += ew[i];
     sh += eh[i];
   }
@@ -532,8 +577,8 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int K, const Flow
   int k = 0;
   bool found = false;
 #pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    if (i < K) {
+  for (int i = 0; i < N; ++i) {
+    if (KT > 0 || i < K) {
       const float xn = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
       const float yn = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
       const bool hit = !found && (x < xn);
@@ -606,8 +651,8 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int K, const Flow
   const float dotw = span * (G1x * pw_lt + G2x * (pw_lt + pwk));
   const float doth = span * (G1y * ph_lt + G2y * (ph_lt + phk));
 #pragma unroll
-  for (int m = 0; m < 10; ++m) {
-    if (m < K) {
+  for (int m = 0; m < N; ++m) {
+    if (KT > 0 || m < K) {
       const float gwm = span * ((m < k ? G1x : 0.f) + (m <= k ? G2x : 0.f));
       const float ghm = span * ((m < k ? G1y : 0.f) + (m <= k ? G2y : 0.f));
       go[m] = ew[m] * isw * (gwm - dotw);
@@ -616,8 +661,8 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int K, const Flow
   }
   const float gk = g_dk / (1.f + __expf(-uk)), gk1 = g_dk1 / (1.f + __expf(-uk1));
 #pragma unroll
-  for (int i = 0; i < 10; ++i)
-    if (i < K) go[20 + i] = (i == k ? gk : 0.f) + (i == k + 1 ? gk1 : 0.f);
+  for (int i = 0; i < N; ++i)
+    if (KT > 0 || i < K) go[20 + i] = (i == k ? gk : 0.f) + (i == k + 1 ? gk1 : 0.f);
   go[30] = (k + 1 == K) ? gk1 : 0.f;
   return g_x;
 }
@@ -671,7 +716,7 @@ struct PassSmem {
 // SIMD: three workgroups share a CU (LDS ~43 KB each at d=64, [256,256]) and
 // hide each other's LDS / L2 latency; measured faster than 64-sample tiles
 // with more B-fragment reuse per wave.
-template <class PT, int MB>
+template <class PT, int MB, int KT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 flow_pass_kernel(FlowDev f, PassArgs a) {
   using T = typename PT::T;
@@ -770,19 +815,10 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       for (int c0 = wave; c0 < NBt; c0 += 8) {
         const int ncb = (c0 + 4 < NBt) ? 2 : 1;
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH>(a0, sm.la0, W + dm.pk_w1(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+        mfma_panel<PT, MB, 2, PF_DEPTH, true>(a0, sm.la0, W + dm.pk_w1(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          if (j < ncb) {
-            const int col = (c0 + 4 * j) * 32 + (lane & 31);
-            const float bb = bl[col];
-#pragma unroll
-            for (int r = 0; r < MB; ++r)
-#pragma unroll
-              for (int v = 0; v < 16; ++v)
-                hb1[(r * 32 + out_row(v, lane)) * sm.lh1 + col] = PT::cvt(PT::act(acc[j][r][v] + bb));
-          }
-        }
+        for (int j = 0; j < 2; ++j)
+          if (j < ncb) hidden_epilogue_tr<PT, MB>(hb1, sm.lh1, bl, c0 + 4 * j, acc[j], lane);
       }
     }
     __syncthreads();
@@ -792,19 +828,10 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       for (int c0 = wave; c0 < NBt; c0 += 8) {
         const int ncb = (c0 + 4 < NBt) ? 2 : 1;
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH>(hb1, sm.lh1, W + dm.pk_w2(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+        mfma_panel<PT, MB, 2, PF_DEPTH, true>(hb1, sm.lh1, W + dm.pk_w2(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          if (j < ncb) {
-            const int col = (c0 + 4 * j) * 32 + (lane & 31);
-            const float bb = bl[h1p + col];
-#pragma unroll
-            for (int r = 0; r < MB; ++r)
-#pragma unroll
-              for (int v = 0; v < 16; ++v)
-                hb2[(r * 32 + out_row(v, lane)) * sm.lh2 + col] = PT::cvt(PT::act(acc[j][r][v] + bb));
-          }
-        }
+        for (int j = 0; j < 2; ++j)
+          if (j < ncb) hidden_epilogue_tr<PT, MB>(hb2, sm.lh2, bl + h1p, c0 + 4 * j, acc[j], lane);
       }
     }
     __syncthreads();
@@ -846,9 +873,9 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
             const float xv = xs[m * ldx + j];
             float yv, ldv;
             if (a.dir == 0)
-              rs_fwd(pp, K, f, xv, yv, ldv);
+              rs_fwd<KT>(pp, K, f, xv, yv, ldv);
             else
-              rs_inv(pp, K, f, xv, yv, ldv);
+              rs_inv<KT>(pp, K, f, xv, yv, ldv);
             xs[m * ldx + j] = yv;
             ldacc[r] += ldv;
           }
@@ -1019,7 +1046,7 @@ __device__ __forceinline__ void store_fm(typename PT::T* st, int64_t ld, int row
   }
 }
 
-template <class PT, int MB, int HMAX>
+template <class PT, int MB, int HMAX, int KT>
 __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a) {
   using T = typename PT::T;
   constexpr int MT = 32 * MB;
@@ -1179,7 +1206,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
           }
           const int m = r * 32 + (lane & 31);
           if (valid) {
-            const float gxv = rs_vjp(pp, K, f, xs[m * ldx + j], gb[m * ldx + j], glp[m], go);
+            const float gxv = rs_vjp<KT>(pp, K, f, xs[m * ldx + j], gb[m * ldx + j], glp[m], go);
             gb[m * ldx + j] = gxv;
           } else {
 #pragma unroll
@@ -1381,16 +1408,20 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
 // 32-sample tiles. Hidden widths up to 256: register budget held to two
 // waves per SIMD (no spills), and with ~68 KB of LDS per tile two workgroups
 // share a CU. Wider hidden layers keep the default budget.
-template <class PT>
+template <class PT, int KT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 flow_bwd_kernel_h256(FlowDev f, BwdArgs a) {
-  flow_bwd_body<PT, 1, 256>(f, a);
+  flow_bwd_body<PT, 1, 256, KT>(f, a);
 }
 
-template <class PT>
+template <class PT, int KT>
 __global__ void __launch_bounds__(256) flow_bwd_kernel_h512(FlowDev f, BwdArgs a) {
-  flow_bwd_body<PT, 1, 512>(f, a);
+  flow_bwd_body<PT, 1, 512, KT>(f, a);
 }
+
+// compile-time bin count for the register spline: 8 and 10 (the common
+// configurations), 0 = runtime K (any K <= 10, and the chunked path)
+inline int spline_kt(int K) { return K == 8 ? 8 : (K == 10 ? 10 : 0); }
 
 // --------------------------------------------------- weight gradients ----
 // One workgroup of WG_SPLIT wavefronts per 32x32 tile of gW = G^T Act: the
@@ -1829,13 +1860,18 @@ extern "C" int fmc_flow_pass(void* stream, const fmc_flow* fl, int32_t direction
   const int MT = 32 * mb;
   dim3 g((unsigned)((n + MT - 1) / MT)), b(256);
   hipStream_t st = (hipStream_t)stream;
+#define FMC_PASS(PT_, KT_)                                                 \
+  do {                                                                     \
+    set_lds(flow_pass_kernel<PT_, 1, KT_>, shm);                           \
+    hipLaunchKernelGGL((flow_pass_kernel<PT_, 1, KT_>), g, b, shm, st, fd, a); \
+  } while (0)
+  const int kt = spline_kt(fd.dm.K);
   if (bf) {
-    set_lds(flow_pass_kernel<PBF16, 1>, shm);
-    hipLaunchKernelGGL((flow_pass_kernel<PBF16, 1>), g, b, shm, st, fd, a);
+    if (kt == 8) FMC_PASS(PBF16, 8); else if (kt == 10) FMC_PASS(PBF16, 10); else FMC_PASS(PBF16, 0);
   } else {
-    set_lds(flow_pass_kernel<PF32, 1>, shm);
-    hipLaunchKernelGGL((flow_pass_kernel<PF32, 1>), g, b, shm, st, fd, a);
+    if (kt == 8) FMC_PASS(PF32, 8); else if (kt == 10) FMC_PASS(PF32, 10); else FMC_PASS(PF32, 0);
   }
+#undef FMC_PASS
   return check_launch("flow pass");
 }
 
@@ -1862,23 +1898,25 @@ extern "C" int fmc_flow_backward_layer(void* stream, const fmc_flow* fl, int32_t
   BwdArgs a{layer, x_layer, n, g_logp, gy, gx, store, store_ld, affine_partials};
   dim3 g((unsigned)nblk), b(256);
   hipStream_t st = (hipStream_t)stream;
+#define FMC_BWD(KERN, PT_, KT_)                                     \
+  do {                                                              \
+    set_lds(KERN<PT_, KT_>, shm);                                   \
+    hipLaunchKernelGGL((KERN<PT_, KT_>), g, b, shm, st, fd, a);     \
+  } while (0)
+#define FMC_BWD_K(KERN, PT_)                                        \
+  do {                                                              \
+    if (kt == 8) FMC_BWD(KERN, PT_, 8);                             \
+    else if (kt == 10) FMC_BWD(KERN, PT_, 10);                      \
+    else FMC_BWD(KERN, PT_, 0);                                     \
+  } while (0)
+  const int kt = spline_kt(fd.dm.K);
   if (hmax <= 256) {
-    if (bf) {
-      set_lds(flow_bwd_kernel_h256<PBF16>, shm);
-      hipLaunchKernelGGL(flow_bwd_kernel_h256<PBF16>, g, b, shm, st, fd, a);
-    } else {
-      set_lds(flow_bwd_kernel_h256<PF32>, shm);
-      hipLaunchKernelGGL(flow_bwd_kernel_h256<PF32>, g, b, shm, st, fd, a);
-    }
+    if (bf) FMC_BWD_K(flow_bwd_kernel_h256, PBF16); else FMC_BWD_K(flow_bwd_kernel_h256, PF32);
   } else {
-    if (bf) {
-      set_lds(flow_bwd_kernel_h512<PBF16>, shm);
-      hipLaunchKernelGGL(flow_bwd_kernel_h512<PBF16>, g, b, shm, st, fd, a);
-    } else {
-      set_lds(flow_bwd_kernel_h512<PF32>, shm);
-      hipLaunchKernelGGL(flow_bwd_kernel_h512<PF32>, g, b, shm, st, fd, a);
-    }
+    if (bf) FMC_BWD_K(flow_bwd_kernel_h512, PBF16); else FMC_BWD_K(flow_bwd_kernel_h512, PF32);
   }
+#undef FMC_BWD_K
+#undef FMC_BWD
   return check_launch("flow backward");
 }
 
