@@ -91,8 +91,12 @@ struct PBF16 {
            (((float)x[4] + (float)x[5]) + ((float)x[6] + (float)x[7]));
   }
   static __device__ __forceinline__ float act(float x) {
-    float e = __expf(2.f * x);
-    return 1.f - 2.f / (e + 1.f);
+    // 1 - 2 / (e^{2x} + 1); the reciprocal from v_rcp_f32 refined by one
+    // Newton step (correctly rounded but for rare last-bit cases)
+    const float den = __expf(2.f * x) + 1.f;
+    float r = __builtin_amdgcn_rcpf(den);
+    r = fmaf(fmaf(-den, r, 1.f), r, r);
+    return fmaf(-2.f, r, 1.f);
   }
 };
 
