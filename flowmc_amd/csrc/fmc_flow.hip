@@ -121,6 +121,18 @@ __device__ __forceinline__ void hidden_epilogue_tr(typename PT::T* hb, int lh, c
   }
 }
 
+// a / b from v_rcp_f32 with one Newton step on the reciprocal and one on the
+// quotient: the IEEE-rounded quotient but for rare last-bit cases, at half
+// the instructions of the full division sequence (operands here are finite
+// and normal; b = 0 gives a NaN that the callers clamp, as the IEEE
+// division's 0/0 would)
+__device__ __forceinline__ float ndiv(float a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  r = fmaf(fmaf(-b, r, 1.f), r, r);
+  const float q = a * r;
+  return fmaf(fmaf(-b, q, a), r, q);
+}
+
 // (m, j) walk over an MT x n tile by the 256 threads of a workgroup, without
 // a division per element: start at tid, advance by 256 elements.
 struct TileWalk {
@@ -462,7 +474,7 @@ __device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int Kr, const Flo
     sh += eh[i];
   }
   const float span = (f.hi - f.lo) - K * f.mb;
-  const float cw = span / sw, ch = span / sh;
+  const float cw = ndiv(span, sw), ch = ndiv(span, sh);
   RsBin b{f.lo, f.hi, f.lo, f.hi, pp[20], pp[21]};
   float xc = f.lo, yc = f.lo;
   bool found = false;
@@ -500,13 +512,13 @@ __device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowD
   }
   const RsBin b = rs_bin<false, KT>(pp, K, f, x);
   const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
-  const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = hh / w;
-  const float z = fminf(fmaxf((x - b.xk) / w, 0.f), 1.f);
+  const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = ndiv(hh, w);
+  const float z = fminf(fmaxf(ndiv(x - b.xk, w), 0.f), 1.f);
   const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
   const float st = dk1 + dk - 2.f * s;
   const float num = hh * (s * z2 + dk * zz);
   const float den = s + st * zz;
-  y = b.yk + num / den;
+  y = b.yk + ndiv(num, den);
   ld = 2.f * __logf(s) + __logf(dk1 * z2 + 2.f * s * zz + dk * q) - 2.f * __logf(den);
 }
 
@@ -522,15 +534,15 @@ __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowD
   }
   const RsBin b = rs_bin<true, KT>(pp, K, f, yv);
   const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
-  const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = hh / w;
-  const float wv = fminf(fmaxf((yv - b.yk) / hh, 0.f), 1.f);
+  const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = ndiv(hh, w);
+  const float wv = fminf(fmaxf(ndiv(yv - b.yk, hh), 0.f), 1.f);
   const float st = dk1 + dk - 2.f * s;
   const float c = -s * wv;
   const float bq = dk - st * wv;
   const float a = s - bq;
   const float disc = bq * bq - 4.f * a * c;
   const float sq = disc > 0.f ? sqrtf(fmaxf(disc, 1.17549435e-38f)) : 0.f;
-  float z = (bq >= 0.f) ? (2.f * c) / (-bq - sq) : (-bq + sq) / (2.f * a);
+  float z = (bq >= 0.f) ? ndiv(2.f * c, -bq - sq) : ndiv(-bq + sq, 2.f * a);
   z = fminf(fmaxf(z, 0.f), 1.f);
   x = w * z + b.xk;
   const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
