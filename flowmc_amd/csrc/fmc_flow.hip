@@ -564,8 +564,8 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
     const bool left = x <= f.lo;
     const float u = (left ? pp[20] : pp[30]) + f.off;
     const float dd = rs_softplus(u) + f.ms;
-    const float gd = gy * (x - (left ? f.lo : f.hi)) + gl / dd;
-    const float gu = gd / (1.f + __expf(-u));
+    const float gd = gy * (x - (left ? f.lo : f.hi)) + ndiv(gl, dd);
+    const float gu = ndiv(gd, 1.f + __expf(-u));
     go[20] = left ? gu : 0.f;
     go[30] = left ? 0.f : gu;
     return gy * dd;
@@ -586,7 +586,8 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
     sh += eh[i];
   }
   const float span = (f.hi - f.lo) - K * f.mb;
-  const float cw = span / sw, ch = span / sh, isw = 1.f / sw, ish = 1.f / sh;
+  const float isw = ndiv(1.f, sw), ish = ndiv(1.f, sh);
+  const float cw = span * isw, ch = span * ish;
   float xk = f.lo, yk = f.lo, xk1 = f.hi, yk1 = f.hi, uk = pp[20], uk1 = pp[21];
   float pw_lt = 0.f, ph_lt = 0.f, pwk = 0.f, phk = 0.f;  // softmax mass of bins < k, and of bin k
   float xc = f.lo, yc = f.lo;
@@ -621,8 +622,8 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
   uk += f.off;
   uk1 += f.off;
   const float dk = rs_softplus(uk) + f.ms, dk1 = rs_softplus(uk1) + f.ms;
-  const float w = xk1 - xk, hh = yk1 - yk, s = hh / w;
-  const float t = (x - xk) / w;
+  const float w = xk1 - xk, hh = yk1 - yk, s = ndiv(hh, w);
+  const float t = ndiv(x - xk, w);
   const float z = fminf(fmaxf(t, 0.f), 1.f);
   const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
   const float st = dk1 + dk - 2.f * s;
@@ -630,10 +631,10 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
   const float den = s + st * zz;
   const float A = dk1 * z2 + 2.f * s * zz + dk * q;
   // reverse sweep of y = yk + num/den, ld = 2 log s + log A - 2 log den
-  const float g_num = gy / den;
-  const float g_den = -gy * num / (den * den) - 2.f * gl / den;
-  float g_s = 2.f * gl / s;
-  const float g_A = gl / A;
+  const float g_num = ndiv(gy, den);
+  const float g_den = -ndiv(gy * num, den * den) - ndiv(2.f * gl, den);
+  float g_s = ndiv(2.f * gl, s);
+  const float g_A = ndiv(gl, A);
   float g_yk = gy;
   float g_dk1 = g_A * z2, g_z2 = g_A * dk1, g_zz = g_A * 2.f * s, g_dk = g_A * q;
   const float g_q = g_A * dk;
@@ -652,11 +653,11 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
   g_z2 -= g_zz;
   const float g_z = g_zz + g_z2 * 2.f * z - g_q * 2.f * (1.f - z);
   const float g_t = (t > 0.f && t < 1.f) ? g_z : 0.f;
-  const float g_x = g_t / w;
-  float g_xk = -g_t / w;
-  float g_w = -g_t * t / w;
-  g_hh += g_s / w;
-  g_w += -g_s * s / w;
+  const float g_x = ndiv(g_t, w);
+  float g_xk = -g_x;
+  float g_w = -ndiv(g_t * t, w);
+  g_hh += ndiv(g_s, w);
+  g_w += -ndiv(g_s * s, w);
   const float g_yk1 = g_hh;
   g_yk -= g_hh;
   const float g_xk1 = g_w;
@@ -675,7 +676,7 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
       go[10 + m] = eh[m] * ish * (ghm - doth);
     }
   }
-  const float gk = g_dk / (1.f + __expf(-uk)), gk1 = g_dk1 / (1.f + __expf(-uk1));
+  const float gk = ndiv(g_dk, 1.f + __expf(-uk)), gk1 = ndiv(g_dk1, 1.f + __expf(-uk1));
 #pragma unroll
   for (int i = 0; i < N; ++i)
     if (KT > 0 || i < K) go[20 + i] = (i == k ? gk : 0.f) + (i == k + 1 ? gk1 : 0.f);
