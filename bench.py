@@ -511,6 +511,119 @@ def bench_flow_pass(args):
     }
 
 
+# ----------------------------------------------------- parallel tempering --
+def bench_pt_gmm64(args):
+    """ParallelTempering in training mode on the 64-D 8-mode mixture: 4096
+    chains x 5 temperatures (linspace(1, 5), the bundle's ladder), 100 tempered
+    MALA steps (step 0.05) per call, then the swap sweep and the temperature
+    adaptation. A step is one strategy call; the value counts tempered
+    chain-steps (chains x temperatures x MALA steps) per second of wall time.
+    The dominant kernel is the fused tempered MALA launch (one per call),
+    timed with HIP events on the launch stream in a separate pass."""
+    import torch
+
+    from flowmc_amd import random as fr
+    from flowmc_amd import targets as TG
+    from flowmc_amd.resource.buffers import Buffer
+    from flowmc_amd.resource.kernel.MALA import MALA
+    from flowmc_amd.resource.logPDF import LogPDF, TemperedPDF
+    from flowmc_amd.resource.states import State
+    from flowmc_amd.strategy.parallel_tempering import ParallelTempering
+
+    dev = torch.device("cuda")
+    C_, T, d, S, dt = args.chains or 4096, 5, 64, 100, 0.05
+    tgt = TG.gaussian_mixture_64d(seed=0, device=dev)
+    tpdf = TemperedPDF(LogPDF(tgt, n_dims=d), None, n_dims=d, n_temps=T)
+    temps = Buffer("temperature", (T,), 0, device=dev)
+    temps.update_buffer(torch.linspace(1.0, 5.0, T, dtype=torch.float64).float())
+    tpos = Buffer("tempered_positions", (C_, T - 1, d), 2, device=dev)
+    x = nfmcmc_initial_positions(tgt, C_, dev)
+    tpos.update_buffer(x[:, None].expand(C_, T - 1, d))
+    import flowmc_amd._native as N
+
+    cfg = N.fmc_launch_cfg(args.lpc, args.nv, args.block, 0) if args.lpc else None
+    res = {"tlp": tpdf, "mala": MALA(dt, fused_steps=S, launch_cfg=cfg), "tp": tpos, "T": temps,
+           "state": State({"training": True}, name="sampler_state")}
+    strat = ParallelTempering(S, "tlp", "mala", ["tp", "T"], "state")
+    key = fr.PRNGKey(args.seed)
+    for _ in range(args.warmup):
+        key, res, x = strat(key, res, x, {})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        key, res, x = strat(key, res, x, {})
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the dominant kernel alone: the fused tempered MALA launch over C*T rows
+    stream = torch.cuda.current_stream()
+    pos = torch.cat([x[:, None], tpos.data], 1).contiguous()
+    acc = torch.empty(C_ * T, S, dtype=torch.float32, device=dev)
+    reps = 5
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for i in range(reps):
+        pos, _ = res["mala"].sample_tempered(fr.PRNGKey(100 + i), pos, tpdf, temps.data, None, S, accepts=acc)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / reps
+    rows, K = C_ * T, tgt.n_modes
+    bytes_launch = rows * (8 * d + 4 + 4 * S)  # state in + out, log-density out, decisions
+    achieved = bytes_launch / (k_ms * 1e-3) / 1e9
+    flops = rows * S * (7 * d * K + 10 * d)  # GMM value + gradient per step, MALA update and ratio
+    t_final = res["T"].data.tolist()
+    return {
+        "value": C_ * T * S * args.steps / wall,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "chain_kernel (fused tempered MALA, one launch of 100 steps per call)",
+                     "kernel_ms": k_ms, "bytes_per_launch": bytes_launch},
+        "extra": {"fp32_tflops_estimate": flops / (k_ms * 1e-3) / 1e12, "local_acceptance": float(acc.mean()),
+                  "temperatures_after": t_final},
+        "config": {"workload": "pt_mala_gmm64", "chains": C_, "temperatures": T, "max_temperature": 5.0,
+                   "tempered_steps": S, "step_size": dt, "dim": d, "modes": K,
+                   "layout": [args.lpc, args.nv, args.block] if args.lpc else "default"},
+    } | ({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_pt(args, tgt, T, S, dt)})
+
+
+def cpu_baseline_pt(args, tgt, T, S, dt):
+    """Oracle parallel tempering (tempered MALA steps, swap sweep, adaptation),
+    FP32 torch on the host cores, on a bounded sample of chains."""
+    import torch
+
+    from oracle import chains as OC
+    from oracle import targets as OT
+    from oracle import tempering as OPT
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    C_, d = max(8, args.cpu_chains // 16), tgt.n_dim
+    means, sigmas = tgt.means.float().cpu(), tgt.sigmas.float().cpu()
+    fn = lambda z: OT.gmm_diag(z, means, sigmas)
+    temps = torch.linspace(1.0, 5.0, T)
+    g = torch.Generator().manual_seed(0)
+    x = means[torch.randint(0, means.shape[0], (C_ * T,), generator=g)] + torch.randn(C_ * T, d, generator=g)
+    fr_ = OPT.tempered_rows(fn, temps, C_)
+    steps = calls = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        for _ in range(S):
+            x, _, _, _ = OC.mala_step(x, lambda z: fr_(z).to(z.dtype), dt, torch.randn(C_ * T, d, generator=g),
+                                      torch.log(torch.rand(C_ * T, generator=g)))
+            steps += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        xs = x.reshape(C_, T, d)
+        xs, _, acc = OPT.exchange(xs, fn(x).reshape(C_, T), temps, torch.log(torch.rand(C_, T - 1, generator=g)))
+        temps = OPT.adapt_temperature(temps, acc).float()
+        fr_ = OPT.tempered_rows(fn, temps, C_)
+        x = xs.reshape(C_ * T, d)
+        calls += 1
+    el = time.perf_counter() - t0
+    return {"value": C_ * T * steps / el, "unit": "tempered chain-steps/s", "cores": torch.get_num_threads(),
+            "kind": "port", "sample": f"{C_} chains x {T} temperatures, {steps} tempered MALA steps in {calls} "
+                                      f"strategy calls, FP32 torch-CPU oracle restatement ({cpu_model()})"}
+
+
 # --------------------------------------------------------------- C4 -------
 def bench_hmc_funnels(args):
     import torch
@@ -564,11 +677,13 @@ WORKLOADS = {
                     "f32/f64"),
     "flow_pass": (bench_flow_pass, "flow samples/s (inverse pass, d=64, 10 layers, [256,256])", "samples/s",
                   "bf16/f32"),
+    "pt_mala_gmm64": (bench_pt_gmm64, "tempered chain-steps/s (parallel tempering, 64-D 8-mode mixture, 5 temperatures)",
+                      "tempered chain-steps/s", "f32"),
 }
 DEFAULT_STEPS = {"nfmcmc_gmm64": 20, "mala_rosenbrock_128d": 10048, "flow_train_d32": 50, "hmc_funnels": 1000,
-                 "flow_pass": 10, "grw_rosenbrock_128d": 10048}
+                 "flow_pass": 10, "grw_rosenbrock_128d": 10048, "pt_mala_gmm64": 50}
 DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d32": 3, "hmc_funnels": 16,
-                  "flow_pass": 1, "grw_rosenbrock_128d": 640}
+                  "flow_pass": 1, "grw_rosenbrock_128d": 640, "pt_mala_gmm64": 3}
 
 
 def main():

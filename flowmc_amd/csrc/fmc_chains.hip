@@ -25,6 +25,8 @@ struct ChainIO {
   int64_t buf_len;
   int64_t cursor;
   uint32_t key0, key1;
+  const float* temps;  // tempered rows: row c targets log p / temps[c % n_temps] (NULL: untempered)
+  int n_temps;
 };
 
 struct StepCfg {
@@ -105,6 +107,24 @@ __device__ __forceinline__ void write_rows(const ChainIO& io, int d, int r, int 
   }
 }
 
+// Tempered rows (parallel tempering): row c of a tempered launch targets
+// log p(x) / T with T = temps[c % n_temps] — TemperedPDF.tempered_log_pdf
+// (src/flowMC/resource/logPDF.py:104-106) with a zero log prior. The value and
+// the gradient of the target are scaled by beta = 1/T after each evaluation.
+__device__ __forceinline__ float row_beta(const ChainIO& io, int64_t c) {
+  return io.temps ? 1.f / io.temps[c % io.n_temps] : 1.f;
+}
+
+template <int NV>
+__device__ __forceinline__ float temper(const ChainIO& io, float beta, float lp, float4 (&g)[NV]) {
+  if (io.temps == nullptr) return lp;
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) f4(g[v], e) *= beta;
+  return beta * lp;
+}
+
 // ------------------------------------------------------------------ MALA ---
 // One chain group runs cfg.n_steps MALA steps (src/flowMC/resource/kernel/
 // MALA.py:26-89). The gradient at the current point is carried from the
@@ -116,7 +136,8 @@ __device__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg&
   const int d = t.d;
   float4 x[NV], g[NV];
   load_row<NV>(io.x + c * d, d, r, lpc, x);
-  float lp = target_vg<NV>(t, x, g, r, lpc, gbase);
+  const float beta = row_beta(io, c);
+  float lp = temper<NV>(io, beta, target_vg<NV>(t, x, g, r, lpc, gbase), g);
   const float dt = cfg.dt;
   const float h = 0.5f * dt * dt;
   const float inv2dt2 = 1.f / (2.f * dt * dt);
@@ -133,7 +154,7 @@ __device__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg&
         f4(prop[v], e) = f4c(x[v], e) + h * f4c(g[v], e) + n;
         s1 += n * n;
       }
-    float lp1 = target_vg<NV>(t, prop, g1, r, lpc, gbase);
+    float lp1 = temper<NV>(io, beta, target_vg<NV>(t, prop, g1, r, lpc, gbase), g1);
     float s2 = 0.f;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
@@ -173,7 +194,8 @@ __device__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
   const int d = t.d;
   float4 x[NV], g[NV];
   load_row<NV>(io.x + c * d, d, r, lpc, x);
-  float lp = target_vg<NV>(t, x, g, r, lpc, gbase);
+  const float beta = row_beta(io, c);
+  float lp = temper<NV>(io, beta, target_vg<NV>(t, x, g, r, lpc, gbase), g);
   const float dt = cfg.dt;
   for (int s = 0; s < cfg.n_steps; ++s) {
     const int tg = cfg.step0 + s;
@@ -183,7 +205,7 @@ __device__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
     for (int v = 0; v < NV; ++v)
 #pragma unroll
       for (int e = 0; e < 4; ++e) f4(prop[v], e) = f4c(x[v], e) + dt * f4c(xi[v], e);
-    const float lp1 = target_vg<NV>(t, prop, g, r, lpc, gbase);
+    const float lp1 = temper<NV>(io, beta, target_vg<NV>(t, prop, g, r, lpc, gbase), g);
     const float lu = draw_log_u(io, c, s, tg, RNG_GRW_UNIFORM);
     const bool acc = lu < lp1 - lp;
     if (acc) {
@@ -214,7 +236,8 @@ __device__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
 #pragma unroll
     for (int v = 0; v < NV; ++v) m[v] = make_float4(1.f, 1.f, 1.f, 1.f);
   }
-  float lp = target_vg<NV>(t, x, g, r, lpc, gbase);
+  const float beta = row_beta(io, c);
+  float lp = temper<NV>(io, beta, target_vg<NV>(t, x, g, r, lpc, gbase), g);
   const float eps = cfg.dt;
   const int L = cfg.n_leapfrog;
   for (int s = 0; s < cfg.n_steps; ++s) {
@@ -250,7 +273,7 @@ __device__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
       for (int v = 0; v < NV; ++v)
 #pragma unroll
         for (int e = 0; e < 4; ++e) f4(xl[v], e) += eps * (f4c(m[v], e) * f4c(p[v], e));
-      lpl = target_vg<NV>(t, xl, gl, r, lpc, gbase);
+      lpl = temper<NV>(io, beta, target_vg<NV>(t, xl, gl, r, lpc, gbase), gl);
 #pragma unroll
       for (int v = 0; v < NV; ++v)
 #pragma unroll
@@ -591,6 +614,97 @@ __global__ void nf_mh_last_kernel(const float* __restrict__ x0, const float* __r
 }
 
 // -------------------------------------------------------------- RNG -------
+// --------------------------------------------------- tempering exchange ---
+// ParallelTempering._exchange (src/flowMC/strategy/parallel_tempering.py:
+// 291-372), PT_CPB chains per block. Per chain, the pairs (i, i+1),
+// i = 0 .. T-2, are visited in order and the swap is accepted iff
+//   log u < (1/T[i+1] - 1/T[i]) * (L_i - L_{i+1})
+// with the log-likelihoods of the walkers currently in slots i and i+1, so a
+// walker can climb several slots in one sweep. Once pair i is decided slot i
+// is final, so the scan carries only the walker that may still move up (its
+// source slot and log-likelihood). Phase 1 (one thread per chain) writes the
+// slot -> source permutation to LDS and the log-likelihoods and decisions to
+// memory; phase 2 gathers the rows, one wave per row.
+constexpr int PT_CPB = 64;
+constexpr int PT_TMAX = 64;
+
+__global__ void __launch_bounds__(256) pt_exchange_kernel(const float* __restrict__ x_in, float* __restrict__ x_out,
+                                                          float* loglik, const float* __restrict__ temps,
+                                                          int64_t C, int T, int d, uint32_t k0, uint32_t k1,
+                                                          const float* __restrict__ log_u,
+                                                          float* __restrict__ acc) {
+  __shared__ int src[PT_CPB * PT_TMAX];
+  const int64_t c0 = (int64_t)blockIdx.x * PT_CPB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nc = (C - c0) < PT_CPB ? (int)(C - c0) : PT_CPB;
+  if (tid < nc) {
+    const int64_t c = c0 + tid;
+    float* L = loglik + c * T;
+    int* sl = src + tid * PT_TMAX;
+    int cur = 0;
+    float lc = L[0];
+    for (int i = 0; i + 1 < T; ++i) {
+      const float ln = L[i + 1];
+      const float ratio = (1.f / temps[i + 1] - 1.f / temps[i]) * (lc - ln);
+      const float lu = log_u ? log_u[c * (T - 1) + i]
+                             : philox_log_uniform(k0, k1, (uint32_t)c, (uint32_t)i, RNG_PT_SWAP);
+      const bool a = lu < ratio;
+      acc[c * (T - 1) + i] = a ? 1.f : 0.f;
+      if (a) {  // slot i takes walker i+1, the carried walker moves up to slot i+1
+        sl[i] = i + 1;
+        L[i] = ln;
+      } else {  // slot i keeps the carried walker, walker i+1 is carried on
+        sl[i] = cur;
+        L[i] = lc;
+        cur = i + 1;
+        lc = ln;
+      }
+    }
+    sl[T - 1] = cur;
+    L[T - 1] = lc;
+  }
+  __syncthreads();
+  for (int row = wave; row < nc * T; row += 4) {
+    const int cl = row / T, t = row - cl * T;
+    const int64_t base = (c0 + cl) * T;
+    const float* s = x_in + (base + src[cl * PT_TMAX + t]) * d;
+    float* o = x_out + (base + t) * d;
+    for (int j = lane; j < d; j += 64) o[j] = s[j];
+  }
+}
+
+// ParallelTempering._adapt_temperature (parallel_tempering.py:400-436), in
+// place, one block: a_p = mean over chains of the swap decisions of pair p;
+// damping_p = (100 / C) (a_p - a_{p+1});
+// T'[i] = T'[i-1] + (T[i] - T[i-1]) exp(damping_{i-1}) for i = 1 .. T-2;
+// the first and last temperatures are unchanged.
+__global__ void __launch_bounds__(256) pt_adapt_kernel(float* temps, const float* __restrict__ acc, int64_t C,
+                                                       int T) {
+  __shared__ float rate[PT_TMAX];
+  __shared__ float part[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int p = 0; p + 1 < T; ++p) {
+    float s = 0.f;  // sums of 0/1 values: exact below 2^24 per partial
+    for (int64_t c = tid; c < C; c += 256) s += acc[c * (T - 1) + p];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) part[wave] = s;
+    __syncthreads();
+    if (tid == 0) rate[p] = ((part[0] + part[1]) + (part[2] + part[3])) / (float)C;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const float k = 100.f / (float)C;
+    float prev_new = temps[0], prev_old = temps[0];
+    for (int i = 1; i + 1 < T; ++i) {
+      const float old = temps[i];
+      const float nw = prev_new + (old - prev_old) * expf(k * (rate[i - 1] - rate[i]));
+      temps[i] = nw;
+      prev_new = nw;
+      prev_old = old;
+    }
+  }
+}
+
 __global__ void philox_blocks_kernel(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2,
                                      uint32_t c3, int64_t n, uint32_t* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -786,6 +900,62 @@ extern "C" int fmc_grw_run(void* stream, const fmc_target* target, float* x, flo
   ChainIO io{x, logp, nullptr, noise, log_u, out_pos, out_logp, out_acc, n_chains, buf_len, cursor, key0, key1};
   StepCfg sc{n_steps, step0, step_size, 0, thinning};
   return launch_chain<KK_GRW>((hipStream_t)stream, target, io, sc, L);
+}
+
+extern "C" int fmc_tempered_run(void* stream, int32_t kernel, const fmc_target* target, float* x, float* logp,
+                                int64_t n_chains, int32_t n_temps, const float* temperatures, int32_t n_steps,
+                                int32_t step0, float step_size, int32_t n_leapfrog, const float* metric,
+                                uint32_t key0, uint32_t key1, const float* noise, const float* log_u,
+                                float* out_acc, int64_t acc_len, const fmc_launch_cfg* cfg) {
+  int rc = check_target(target);
+  if (rc) return rc;
+  FMC_ARG(kernel == FMC_KERNEL_MALA || kernel == FMC_KERNEL_HMC || kernel == FMC_KERNEL_GRW,
+          "unknown kernel %d", kernel);
+  FMC_ARG(x != nullptr && temperatures != nullptr, "x or temperatures is NULL");
+  FMC_ARG(n_chains >= 0 && n_temps >= 1, "n_chains must be >= 0 and n_temps >= 1");
+  FMC_ARG(kernel == FMC_KERNEL_HMC || step_size > 0.f, "step_size must be > 0");
+  FMC_ARG(n_leapfrog >= 0, "n_leapfrog < 0");
+  FMC_ARG((noise == nullptr) == (log_u == nullptr), "noise and log_u must be both given or both NULL");
+  if ((rc = check_buffers(nullptr, nullptr, out_acc, acc_len, 0, 1, step0, n_steps))) return rc;
+  const int64_t rows = n_chains * n_temps;
+  if (rows == 0) return FMC_OK;
+  Layout L = choose_layout(target->n_dim, target->kind, cfg);
+  if ((rc = check_layout(L, target->n_dim))) return rc;
+  ChainIO io{x, logp, kernel == FMC_KERNEL_HMC ? metric : nullptr, noise, log_u, nullptr, nullptr, out_acc,
+             rows, acc_len, 0, key0, key1, temperatures, n_temps};
+  StepCfg sc{n_steps, step0, step_size, n_leapfrog, 1};
+  hipStream_t st = (hipStream_t)stream;
+  if (kernel == FMC_KERNEL_MALA) return launch_chain<KK_MALA>(st, target, io, sc, L);
+  if (kernel == FMC_KERNEL_GRW) return launch_chain<KK_GRW>(st, target, io, sc, L);
+  return launch_chain<KK_HMC>(st, target, io, sc, L);
+}
+
+extern "C" int fmc_pt_exchange(void* stream, const float* x_in, float* x_out, float* loglik,
+                               const float* temperatures, int64_t n_chains, int32_t n_temps, int32_t n_dim,
+                               uint32_t key0, uint32_t key1, const float* log_u, float* accepts) {
+  FMC_ARG(x_in && x_out && loglik && temperatures, "NULL pointer");
+  FMC_ARG(x_in != x_out, "x_out must not alias x_in");
+  FMC_ARG(n_chains >= 0 && n_dim >= 1, "bad sizes");
+  FMC_ARG(n_temps >= 1 && n_temps <= PT_TMAX, "n_temps must be in [1, %d] (got %d)", PT_TMAX, n_temps);
+  FMC_ARG(n_temps == 1 || accepts != nullptr, "accepts is NULL");
+  if (n_chains == 0) return FMC_OK;
+  const int64_t grid = (n_chains + PT_CPB - 1) / PT_CPB;
+  FMC_ARG(grid <= 0x7fffffffLL, "too many chains for one launch");
+  hipLaunchKernelGGL(pt_exchange_kernel, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, x_in, x_out,
+                     loglik, temperatures, n_chains, n_temps, n_dim, key0, key1, log_u, accepts);
+  return check_launch("tempering exchange");
+}
+
+extern "C" int fmc_pt_adapt(void* stream, float* temperatures, const float* accepts, int64_t n_chains,
+                            int32_t n_temps) {
+  FMC_ARG(temperatures != nullptr, "temperatures is NULL");
+  FMC_ARG(n_temps >= 1 && n_temps <= PT_TMAX, "n_temps must be in [1, %d] (got %d)", PT_TMAX, n_temps);
+  FMC_ARG(n_chains >= 1, "n_chains must be >= 1");
+  if (n_temps < 3) return FMC_OK;  // no interior temperature to move
+  FMC_ARG(accepts != nullptr, "accepts is NULL");
+  hipLaunchKernelGGL(pt_adapt_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, temperatures, accepts,
+                     n_chains, n_temps);
+  return check_launch("temperature adaptation");
 }
 
 extern "C" int fmc_grw_propose(void* stream, const float* x, float* prop, int64_t n_chains, int32_t n_dim,

@@ -81,6 +81,7 @@ enum RngStream : uint32_t {
   RNG_FLOW_BASE = 5,
   RNG_GRW_NOISE = 7,
   RNG_GRW_UNIFORM = 8,
+  RNG_PT_SWAP = 9,
 };
 
 // Four standard normals for dims 4*chunk .. 4*chunk+3 of (chain, step):

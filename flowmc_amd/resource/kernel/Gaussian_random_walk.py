@@ -9,7 +9,7 @@ import torch
 from ... import _native
 from ... import random as frandom
 from ..logPDF import LogPDF
-from .base import ProposalBase, StepOutputs, steps_needed
+from .base import KERNEL_GRW, ProposalBase, StepOutputs, steps_needed
 from .NF_proposal import _eval_logpdf
 
 
@@ -121,6 +121,9 @@ class GaussianRandomWalk(ProposalBase):
                          _native.dptr(out.positions, name="position buffer"),
                          _native.dptr(out.log_probs, name="log_prob buffer"),
                          _native.dptr(out.accepts, name="acceptance buffer"), out.buf_len, out.cursor, thinning)
+
+    def _tempered_launch(self, n_dim, device):
+        return KERNEL_GRW, self.step_size, 0, None
 
     def print_parameters(self):
         print("Gaussian Random Walk parameters:")

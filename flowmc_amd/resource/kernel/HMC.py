@@ -9,7 +9,7 @@ import torch
 from ... import _native
 from ... import random as frandom
 from ..logPDF import LogPDF
-from .base import ProposalBase, StepOutputs, steps_needed
+from .base import KERNEL_HMC, ProposalBase, StepOutputs, steps_needed
 from .MALA import _prep_state
 
 
@@ -150,6 +150,9 @@ class HMC(ProposalBase):
             same = (x == xl).all(-1, keepdim=True)
             g = torch.where(same, gl, g).contiguous()
         self.last_log_prob = lp
+
+    def _tempered_launch(self, n_dim, device):
+        return KERNEL_HMC, self.step_size, self.n_leapfrog, self.metric(n_dim, device)
 
     def print_parameters(self):
         print("HMC parameters:")

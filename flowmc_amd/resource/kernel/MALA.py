@@ -9,7 +9,7 @@ import torch
 from ... import _native
 from ... import random as frandom
 from ..logPDF import LogPDF
-from .base import ProposalBase, StepOutputs, steps_needed
+from .base import KERNEL_MALA, ProposalBase, StepOutputs, steps_needed
 
 
 def _prep_state(position: torch.Tensor) -> torch.Tensor:
@@ -116,6 +116,9 @@ class MALA(ProposalBase):
                          _native.dptr(out.log_probs, name="log_prob buffer"),
                          _native.dptr(out.accepts, name="acceptance buffer"), out.buf_len, out.cursor, thinning)
         self.last_log_prob = lp
+
+    def _tempered_launch(self, n_dim, device):
+        return KERNEL_MALA, self.step_size, 0, None
 
     def print_parameters(self):
         print("MALA parameters:")

@@ -108,3 +108,94 @@ class LogPDF(Resource):
 
     def load_resource(self, path):
         raise NotImplementedError
+
+
+class TemperedPDF(LogPDF):
+    """Tempered log-density — src/flowMC/resource/logPDF.py:85-115.
+
+    ``tempered_log_pdf(T, x, data) = log_likelihood(x, data) / T + log_prior(x, data)``;
+    calling the object returns the untempered log-likelihood (logPDF.py:101-102),
+    which is what the exchange step compares. ``n_temps`` and ``max_temp`` are
+    accepted and unused, as in the reference.
+
+    ``log_prior=None`` is the zero prior: with a built-in target the tempered
+    local steps then run in the fused chain kernel (fmc_tempered_run); any other
+    prior is evaluated with torch and the steps take the per-step path.
+    """
+
+    def __repr__(self):
+        return "TemperedPDF with " + str(self.n_dims) + " dimensions"
+
+    def __init__(self, log_likelihood: Callable, log_prior: Optional[Callable] = None, variables=None,
+                 n_dims: Optional[int] = None, n_temps: int = 5, max_temp: float = 100, batched: bool = False):
+        if isinstance(log_likelihood, LogPDF):
+            batched = log_likelihood.batched
+            if variables is None and n_dims is None:
+                variables = log_likelihood.variables
+            log_likelihood = log_likelihood.log_pdf
+        super().__init__(log_likelihood, variables, n_dims, batched=batched)
+        self.log_prior = log_prior
+
+    @property
+    def fused_tempering(self) -> bool:
+        """Built-in likelihood and zero prior: tempered steps run fused on the device."""
+        return self.is_native and self.log_prior is None
+
+    def _prior(self, x: torch.Tensor, data) -> torch.Tensor:
+        if self.log_prior is None:
+            return torch.zeros(x.shape[:-1], dtype=torch.float32, device=x.device)
+        if x.ndim == 1:
+            return torch.as_tensor(self.log_prior(x, data), dtype=torch.float32, device=x.device).reshape(())
+        flat = x.reshape(-1, x.shape[-1])
+        try:
+            v = torch.func.vmap(self.log_prior, in_dims=(0, None))(flat, data)
+        except (RuntimeError, ValueError):
+            v = torch.stack([torch.as_tensor(self.log_prior(xi, data)).reshape(()) for xi in flat])
+        return torch.as_tensor(v, dtype=torch.float32, device=x.device).reshape(x.shape[:-1])
+
+    def tempered_log_pdf(self, temperatures, x: torch.Tensor, data=None) -> torch.Tensor:
+        """(1 / T) * log_likelihood(x) + log_prior(x); T broadcasts against x's leading dims."""
+        base = self(x, data)
+        T = torch.as_tensor(temperatures, dtype=base.dtype, device=base.device)
+        return (1.0 / T) * base + self._prior(x, data)
+
+
+class TemperedRows(LogPDF):
+    """Rows [R, d] of tempered chains seen as one batched log-density: row r
+    targets tempered_log_pdf(temperatures[r % n_temps], x_r). The per-step
+    kernel paths evaluate and differentiate it like any batched LogPDF."""
+
+    def __init__(self, tempered: TemperedPDF, temperatures: torch.Tensor):
+        super().__init__(tempered.log_pdf, tempered.variables, batched=True)
+        self.tempered = tempered
+        self.temperatures = torch.as_tensor(temperatures, dtype=torch.float32)
+
+    @property
+    def is_native(self) -> bool:
+        return False
+
+    def _row_temps(self, x):
+        T = self.temperatures.to(x.device)
+        return T.repeat(x.shape[0] // T.numel())
+
+    def __call__(self, x: torch.Tensor, data=None) -> torch.Tensor:
+        return self.tempered.tempered_log_pdf(self._row_temps(x), x, data)
+
+    def value_and_grad(self, x: torch.Tensor, data=None):
+        t = self.tempered
+        beta = 1.0 / self._row_temps(x)
+        if t.is_native:
+            v, g = t.log_pdf.value_and_grad(x)
+        else:
+            v, g = LogPDF.value_and_grad(t, x, data)
+        v, g = beta * v, beta[:, None] * g
+        if t.log_prior is not None:
+            with torch.enable_grad():
+                xr = x.detach().clone().requires_grad_(True)
+                pv = t._prior(xr, data)
+                if pv.requires_grad:
+                    (pg,) = torch.autograd.grad(pv.sum(), xr)
+                else:
+                    pg = torch.zeros_like(xr)
+            v, g = v + pv.detach(), g + pg
+        return v.to(torch.float32).contiguous(), g.to(torch.float32).contiguous()

@@ -138,6 +138,49 @@ int fmc_hmc_run(void* stream, const fmc_target* target, float* x, float* logp,
                 float* out_acc, int64_t buf_len, int64_t cursor, int32_t thinning,
                 const fmc_launch_cfg* cfg);
 
+/* ------------------------------------------------------------------------ */
+/* Parallel tempering.                                                      */
+/* fmc_tempered_run replaces ParallelTempering._ensemble_step               */
+/* (src/flowMC/strategy/parallel_tempering.py:135-290): n_steps local steps */
+/* of `kernel` (fmc_kernel_kind) for every (chain, temperature) row; x is   */
+/* [n_chains, n_temps, d] (row c*n_temps + t), and row r targets            */
+/* TemperedPDF.tempered_log_pdf (src/flowMC/resource/logPDF.py:104-106)     */
+/* with a zero log prior: log p(x) / temperatures[r % n_temps]. logp [rows] */
+/* receives the tempered log-density of the final state (may be NULL);      */
+/* out_acc [rows, acc_len] (may be NULL) the decision of step step0 + s at  */
+/* column step0 + s. Noise layout, Philox streams, metric and step_size     */
+/* are those of fmc_mala_run / fmc_hmc_run / fmc_grw_run over the rows.     */
+/* ------------------------------------------------------------------------ */
+typedef enum fmc_kernel_kind {
+  FMC_KERNEL_MALA = 0,
+  FMC_KERNEL_HMC = 1,
+  FMC_KERNEL_GRW = 2
+} fmc_kernel_kind;
+
+int fmc_tempered_run(void* stream, int32_t kernel, const fmc_target* target, float* x, float* logp,
+                     int64_t n_chains, int32_t n_temps, const float* temperatures, int32_t n_steps,
+                     int32_t step0, float step_size, int32_t n_leapfrog, const float* metric,
+                     uint32_t key0, uint32_t key1, const float* noise, const float* log_u,
+                     float* out_acc, int64_t acc_len, const fmc_launch_cfg* cfg);
+
+/* Replaces ParallelTempering._exchange (parallel_tempering.py:291-372) for */
+/* all chains: per chain, adjacent pairs i = 0 .. n_temps-2 in order, swap  */
+/* iff log u < (1/T[i+1] - 1/T[i]) (L_i - L_{i+1}) on the current slots.    */
+/* x_in / x_out [n_chains, n_temps, d] (no aliasing); loglik [n_chains,     */
+/* n_temps] holds the untempered log-likelihoods of x_in on entry and of    */
+/* x_out on exit; accepts [n_chains, n_temps-1] receives 1.0f / 0.0f.       */
+/* log_u [n_chains, n_temps-1] or NULL (Philox stream 9, counter (chain,    */
+/* pair)). n_temps <= 64.                                                   */
+int fmc_pt_exchange(void* stream, const float* x_in, float* x_out, float* loglik,
+                    const float* temperatures, int64_t n_chains, int32_t n_temps, int32_t n_dim,
+                    uint32_t key0, uint32_t key1, const float* log_u, float* accepts);
+
+/* Replaces ParallelTempering._adapt_temperature (parallel_tempering.py:     */
+/* 400-436), in place on temperatures[n_temps] from the exchange decisions  */
+/* accepts [n_chains, n_temps-1]. n_temps <= 64.                            */
+int fmc_pt_adapt(void* stream, float* temperatures, const float* accepts, int64_t n_chains,
+                 int32_t n_temps);
+
 /* Many independent HMC problems of uneven dimension in ONE launch.        */
 /* slots[n_slots] (device memory) describe each problem; items[n_items]    */
 /* (device memory) are the (slot, first chain, n chains) tiles, one per    */

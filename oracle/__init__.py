@@ -20,4 +20,13 @@ Pinning (what independently checks each function; see tests/test_oracle_*.py):
                     finite-difference derivative, change of variables via a
                     full Jacobian, quadrature normalisation in 2-D.
   optim             torch.optim.AdamW + global-norm clip (independent code).
+  tempering         exchange: invariance of the product of tempered
+                    Gaussians under the swap sweep (moments of every slot),
+                    the cold slot receiving a better hot walker, a walker
+                    climbing several slots in one sweep, equal temperatures
+                    => every swap accepted; adaptation: equal acceptance =>
+                    ladder unchanged, a hand-worked 4-rung example, the
+                    reference's own [C, T] shaped call; tempered steps: the
+                    mala / hmc / grw oracles on log p / T, T = 1 equals the
+                    untempered step.
 """

@@ -25,6 +25,7 @@ RNG_FLOW_BASE = 5
 RNG_DIRECT_NORMAL = 6
 RNG_GRW_NOISE = 7
 RNG_GRW_UNIFORM = 8
+RNG_PT_SWAP = 9
 
 
 def philox4x32_10(ctr, key) -> np.ndarray:
