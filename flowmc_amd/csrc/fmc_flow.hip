@@ -1452,6 +1452,7 @@ inline int spline_kt(int K) { return K == 8 ? 8 : (K == 10 ? 10 : 0); }
 // on XCD b % 8; tiles are dealt so that XCD x gets a contiguous range of
 // them, i.e. the tiles sharing a G row panel share one L2.
 constexpr int WG_SPLIT = 8;
+static_assert(16 % WG_SPLIT == 0, "each wave finishes a whole number of accumulator registers");
 constexpr int WG_XCDS = 8;
 
 __host__ __device__ inline int wgrad_ntiles(const FlowDims& dm, int l) {
@@ -1547,49 +1548,46 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
   for (int v = 0; v < 16; ++v) red[wave][v][lane] = acc[v];
   rsum[wave][lane] = bsum;
   __syncthreads();
-  if (wave == 0) {
-    float tot[16];
+  // each wave finishes 16 / WG_SPLIT accumulator registers (fixed-order sum
+  // over the waves' partials); the last wave also finishes the bias row
+  constexpr int VPW = 16 / WG_SPLIT;
+  const int k = kb * 32 + (lane & 31);
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      float t = 0.f;
+  for (int u = 0; u < VPW; ++u) {
+    const int v = wave * VPW + u;
+    float t = 0.f;
 #pragma unroll
-      for (int w = 0; w < WG_SPLIT; ++w) t += red[w][v][lane];
-      tot[v] = t;
+    for (int w = 0; w < WG_SPLIT; ++w) t += red[w][v][lane];
+    const int n = nb * 32 + out_row(v, lane);
+    int64_t idx = -1;
+    if (prob == 0) {
+      if (n < dm.h1 && k < dm.dc(l)) idx = dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k);
+    } else if (prob == 1) {
+      if (n < dm.h2 && k < dm.h1) idx = dm.th_w2(l) + (int64_t)n * dm.h1 + k;
+    } else {
+      int row = dm.gp_row(l, n);
+      if (row >= 0 && k < dm.h2) idx = dm.th_w3(l) + (int64_t)row * dm.h2 + k;
     }
-    const int k = kb * 32 + (lane & 31);
+    if (idx >= 0) grad[idx] = t;
+  }
+  if (want_bias && wave == WG_SPLIT - 1) {
+    // lane l and l+32 hold the two halves of G row nb*32 + l%32
+    float b = 0.f;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int n = nb * 32 + out_row(v, lane);
+    for (int w = 0; w < WG_SPLIT; ++w) b += rsum[w][lane];
+    b += __shfl_xor(b, 32, 64);
+    const int n = nb * 32 + lane;
+    if (lane < 32) {
       int64_t idx = -1;
       if (prob == 0) {
-        if (n < dm.h1 && k < dm.dc(l)) idx = dm.th_w1(l) + (int64_t)n * dm.d + dm.cond(l, k);
+        if (n < dm.h1) idx = dm.th_b1(l) + n;
       } else if (prob == 1) {
-        if (n < dm.h2 && k < dm.h1) idx = dm.th_w2(l) + (int64_t)n * dm.h1 + k;
+        if (n < dm.h2) idx = dm.th_b2(l) + n;
       } else {
         int row = dm.gp_row(l, n);
-        if (row >= 0 && k < dm.h2) idx = dm.th_w3(l) + (int64_t)row * dm.h2 + k;
+        if (row >= 0) idx = dm.th_b3(l) + row;
       }
-      if (idx >= 0) grad[idx] = tot[v];
-    }
-    if (want_bias) {
-      // lane l and l+32 hold the two halves of G row nb*32 + l%32
-      float b = 0.f;
-#pragma unroll
-      for (int w = 0; w < WG_SPLIT; ++w) b += rsum[w][lane];
-      b += __shfl_xor(b, 32, 64);
-      const int n = nb * 32 + lane;
-      if (lane < 32) {
-        int64_t idx = -1;
-        if (prob == 0) {
-          if (n < dm.h1) idx = dm.th_b1(l) + n;
-        } else if (prob == 1) {
-          if (n < dm.h2) idx = dm.th_b2(l) + n;
-        } else {
-          int row = dm.gp_row(l, n);
-          if (row >= 0) idx = dm.th_b3(l) + row;
-        }
-        if (idx >= 0) grad[idx] = b;
-      }
+      if (idx >= 0) grad[idx] = b;
     }
   }
 }

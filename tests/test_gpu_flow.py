@@ -203,6 +203,31 @@ def test_flow_gradient_is_additive_over_batch_tiles(gpu, prec):
     assert float((gx - gxs).abs().max()) <= 1e-6 * float(gxs.abs().max())
 
 
+@pytest.mark.parametrize("hidden", [128, 256])
+def test_flow_gradient_independent_of_the_backward_schedule(gpu, hidden):
+    """Weight gradients on a side stream beside the next adjoint (hidden <= 128)
+    or on one stream: the same kernels on the same data, so the same bits."""
+    d, L, K = 32, 4, 10
+    m, _ = make(d, L, K, hidden, hidden, "bfloat16", seed=6)
+    g = torch.Generator().manual_seed(7)
+    B = 10000
+    x = (torch.randn(B, d, generator=g) * 2).float().cuda()
+    w = (torch.randn(B, generator=g) / B).float().cuda()
+    g1, gx1 = m.grad_log_prob_sum(x, w)
+    ws = m._ws[B]
+    assert ws.overlap == (hidden <= 128)
+    if not ws.overlap:  # give the one-stream workspace what the overlapped schedule needs
+        ws.store.append(torch.empty_like(ws.store[0]))
+        ws.aff.append(torch.empty_like(ws.aff[0]))
+        ws.side = torch.cuda.Stream()
+        ws.ev_adj = [torch.cuda.Event() for _ in range(L)]
+        ws.ev_wg = [torch.cuda.Event() for _ in range(L)]
+    ws.overlap = not ws.overlap
+    for _ in range(2):
+        g2, gx2 = m.grad_log_prob_sum(x, w)
+        assert torch.equal(g1, g2) and torch.equal(gx1, gx2)
+
+
 @pytest.mark.parametrize("case", CASES[:3])
 def test_flow_bf16_gradient_vs_fp64_reference_init(gpu, case):
     """BF16 adjoint against the exact FP64 gradient at reference-like init."""
