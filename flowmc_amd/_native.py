@@ -121,6 +121,7 @@ _SIGNATURES: dict[str, list] = {
     "fmc_tempered_run": [P, I32, C.POINTER(fmc_target), P, P, I64, I32, P, I32, I32, F32, I32, P, U32, U32, P, P, P,
                          I64, C.POINTER(fmc_launch_cfg)],
     "fmc_pt_exchange": [P, P, P, P, P, I64, I32, I32, U32, U32, P, P],
+    "fmc_finite_rows": [P, P, I64, I32, P],
     "fmc_pt_adapt": [P, P, P, I64, I32],
     "fmc_mala_propose": [P, P, P, P, I64, I32, F32, U32, U32, I32, P],
     "fmc_mala_accept": [P, P, P, P, P, P, P, I64, I32, F32, U32, U32, I32, P, P, P, P, I64, I64, I32],

@@ -614,6 +614,33 @@ __global__ void nf_mh_last_kernel(const float* __restrict__ x0, const float* __r
 }
 
 // -------------------------------------------------------------- RNG -------
+// ---------------------------------------------------- finite-row mask ---
+// out[r] = 1 iff every entry of row r is finite (jnp.isfinite(data).all(-1)
+// in TrainModel, src/flowMC/strategy/train_model.py:66-80). A row is read by
+// g lanes of one wavefront (g a power of two), float4-wide when vec is set;
+// one ballot per wavefront combines the lanes of each row.
+__global__ void __launch_bounds__(256) finite_rows_kernel(const float* __restrict__ x, int64_t n_rows, int d,
+                                                          int g, int vec, int32_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int sub = lane & (g - 1);
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / g;
+  bool ok = true;
+  if (row < n_rows) {
+    const float* p = x + row * d;
+    if (vec) {
+      for (int j = sub * 4; j < d; j += g * 4) {
+        const float4 v = *reinterpret_cast<const float4*>(p + j);
+        ok = ok && isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
+      }
+    } else {
+      for (int j = sub; j < d; j += g) ok = ok && isfinite(p[j]);
+    }
+  }
+  const unsigned long long bad = __ballot(!ok);
+  const unsigned long long grp = (g == 64) ? ~0ull : (((1ull << g) - 1ull) << (lane & ~(g - 1)));
+  if (sub == 0 && row < n_rows) out[row] = (bad & grp) ? 0 : 1;
+}
+
 // --------------------------------------------------- tempering exchange ---
 // ParallelTempering._exchange (src/flowMC/strategy/parallel_tempering.py:
 // 291-372), PT_CPB chains per block. Per chain, the pairs (i, i+1),
@@ -928,6 +955,21 @@ extern "C" int fmc_tempered_run(void* stream, int32_t kernel, const fmc_target* 
   if (kernel == FMC_KERNEL_MALA) return launch_chain<KK_MALA>(st, target, io, sc, L);
   if (kernel == FMC_KERNEL_GRW) return launch_chain<KK_GRW>(st, target, io, sc, L);
   return launch_chain<KK_HMC>(st, target, io, sc, L);
+}
+
+extern "C" int fmc_finite_rows(void* stream, const float* data, int64_t n_rows, int32_t n_dim, int32_t* out) {
+  FMC_ARG(data != nullptr && out != nullptr, "NULL pointer");
+  FMC_ARG(n_rows >= 0 && n_dim >= 1, "bad sizes");
+  if (n_rows == 0) return FMC_OK;
+  const int vec = ((n_dim & 3) == 0) && (((uintptr_t)data & 15) == 0);
+  const int units = vec ? n_dim / 4 : n_dim;
+  int g = 1;
+  while (g < units && g < 64) g *= 2;
+  const int64_t grid = (n_rows * g + 255) / 256;
+  FMC_ARG(grid <= 0x7fffffffLL, "too many rows for one launch");
+  hipLaunchKernelGGL(finite_rows_kernel, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, data, n_rows,
+                     n_dim, g, vec, out);
+  return check_launch("finite rows");
 }
 
 extern "C" int fmc_pt_exchange(void* stream, const float* x_in, float* x_out, float* loglik,

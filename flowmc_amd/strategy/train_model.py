@@ -4,6 +4,7 @@ from __future__ import annotations
 
 import torch
 
+from .. import _native
 from .. import random as frandom
 from ..resource.buffers import Buffer
 from ..resource.model.nf_model.base import NFModel
@@ -12,19 +13,35 @@ from .base import Strategy
 
 
 def select_training_data(data: torch.Tensor, history_window: int, n_max_examples: int, key) -> torch.Tensor:
-    """train_model.py:66-80: keep rows whose every entry is finite, take the last
-    ``history_window`` of them per chain, then ``n_max_examples`` draws with
-    replacement."""
+    """train_model.py:66-80: keep rows whose every entry is finite, reshape them
+    to (n_chains, -1, n_dims), take ``[:, -history_window:]``, then draw
+    ``n_max_examples`` rows with replacement.
+
+    Only the drawn rows are gathered: the finite-row mask comes from one pass
+    of fmc_finite_rows over the buffer, its running count locates the k-th
+    finite row (chain-major order, as the reference's boolean indexing lists
+    them), and the slice is applied to ranks. Same rows, same order, same
+    errors as filtering and reshaping the whole buffer, without copying it.
+    """
     n_chains, n_dims = data.shape[0], data.shape[-1]
-    finite = torch.isfinite(data).all(-1)
-    rows = data[finite]
-    if rows.shape[0] % n_chains != 0:
+    flat = data.reshape(-1, n_dims)
+    if flat.dtype != torch.float32 or not flat.is_contiguous():
+        flat = flat.to(torch.float32).contiguous()
+    mask = torch.empty(flat.shape[0], dtype=torch.int32, device=flat.device)
+    _native.call("fmc_finite_rows", _native.stream_ptr(), _native.dptr(flat, name="training buffer"),
+                 flat.shape[0], n_dims, _native.dptr(mask, torch.int32))
+    rank = torch.cumsum(mask, 0, dtype=torch.int64)
+    n_finite = int(rank[-1]) if rank.numel() else 0
+    if n_finite % n_chains != 0:
         raise ValueError("chains hold different numbers of finite rows; cannot reshape to (n_chains, -1, n_dims)")
-    rows = rows.reshape(n_chains, -1, n_dims)[:, -history_window:].reshape(-1, n_dims)
-    if rows.shape[0] == 0:
+    m = n_finite // n_chains
+    start, stop, _ = slice(-history_window, None).indices(m)
+    w = max(0, stop - start)
+    if n_chains * w == 0:
         raise ValueError("no finite rows in the training buffer")
-    idx = frandom.choice(key, rows.shape[0], n_max_examples, device=rows.device)
-    return rows[idx]
+    idx = frandom.choice(key, n_chains * w, n_max_examples, device=flat.device)
+    k = (idx // w) * m + start + idx % w  # rank of the drawn row among the finite rows
+    return flat[torch.searchsorted(rank, k + 1)]
 
 
 class TrainModel(Strategy):

@@ -139,6 +139,14 @@ int fmc_hmc_run(void* stream, const fmc_target* target, float* x, float* logp,
                 const fmc_launch_cfg* cfg);
 
 /* ------------------------------------------------------------------------ */
+/* Finite-row mask of a row-major [n_rows, n_dim] buffer: out[r] = 1 iff    */
+/* every entry of row r is finite, else 0. Replaces                         */
+/* jnp.isfinite(data).all(axis=-1) in TrainModel's training-data selection  */
+/* (src/flowMC/strategy/train_model.py:66-80).                              */
+/* ------------------------------------------------------------------------ */
+int fmc_finite_rows(void* stream, const float* data, int64_t n_rows, int32_t n_dim, int32_t* out);
+
+/* ------------------------------------------------------------------------ */
 /* Parallel tempering.                                                      */
 /* fmc_tempered_run replaces ParallelTempering._ensemble_step               */
 /* (src/flowMC/strategy/parallel_tempering.py:135-290): n_steps local steps */
