@@ -99,6 +99,18 @@ class fmc_flow(C.Structure):
     ]
 
 
+class fmc_rnvp(C.Structure):
+    _fields_ = [
+        ("n_dim", C.c_int32),
+        ("n_layers", C.c_int32),
+        ("n_hidden", C.c_int32),
+        ("dt", C.c_float),
+        ("theta", C.c_void_p),
+        ("packed", C.c_void_p),
+        ("layout", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -122,6 +134,11 @@ _SIGNATURES: dict[str, list] = {
                          I64, C.POINTER(fmc_launch_cfg)],
     "fmc_pt_exchange": [P, P, P, P, P, I64, I32, I32, U32, U32, P, P],
     "fmc_finite_rows": [P, P, I64, I32, P],
+    "fmc_rnvp_sizes": [C.POINTER(fmc_rnvp), P, P, P, P],
+    "fmc_rnvp_pack": [P, C.POINTER(fmc_rnvp)],
+    "fmc_rnvp_pass": [P, C.POINTER(fmc_rnvp), I32, I32, P, I64, U32, U32, I64, P, P, P, P, P],
+    "fmc_rnvp_backward_layer": [P, C.POINTER(fmc_rnvp), I32, P, I64, P, P, P, P, I64],
+    "fmc_rnvp_weight_grad": [P, C.POINTER(fmc_rnvp), I32, P, I64, P],
     "fmc_pt_adapt": [P, P, P, I64, I32],
     "fmc_mala_propose": [P, P, P, P, I64, I32, F32, U32, U32, I32, P],
     "fmc_mala_accept": [P, P, P, P, P, P, P, I64, I32, F32, U32, U32, I32, P, P, P, P, I64, I64, I32],

@@ -299,6 +299,54 @@ int fmc_rng_log_uniform(void* stream, uint32_t key0, uint32_t key1, int64_t n_ch
 
 
 /* ------------------------------------------------------------------------ */
+/* RealNVP (src/flowMC/resource/model/nf_model/realNVP.py:102-228).        */
+/* Per layer a MaskedCouplingLayer around an MLPAffine (common.py:127-208):  */
+/* scale and shift MLPs [d, hidden, d] with ReLU read the conditioning dims; */
+/* s = tanh(MLP_s) dt, t = MLP_t dt, y = (x + t) e^s on the transformed     */
+/* dims, log-det = sum s. FP32 throughout.                                  */
+/* theta (FP32): per layer, for the scale then the shift MLP: W1 [h][d],    */
+/* b1 [h], W2 [d][h], b2 [d]; then data_mean [d], data_cov [d][d],          */
+/* base_mean [d], base_cov [d][d] (only the diagonals of the covariances    */
+/* are used). layout (device int32): per layer [n_cond, n_trans,            */
+/* cond[n_dim], trans[n_dim]] (realNVP.py:155-158 masks).                   */
+/* ------------------------------------------------------------------------ */
+typedef struct fmc_rnvp {
+  int32_t n_dim, n_layers, n_hidden;
+  float dt;               /* MLPAffine dt (1 in RealNVP)                    */
+  const float* theta;
+  float* packed;          /* packed_size floats, written by fmc_rnvp_pack   */
+  const int32_t* layout;
+} fmc_rnvp;
+
+/* Sizes: theta and packed (floats), weight-gradient partials per sample     */
+/* tile (floats) and the tile (samples).                                    */
+int fmc_rnvp_sizes(const fmc_rnvp* flow, int64_t* theta_size, int64_t* packed_size,
+                   int64_t* partial_per_tile, int32_t* tile);
+/* theta -> packed first-layer weights and biases (after every update).     */
+int fmc_rnvp_pack(void* stream, const fmc_rnvp* flow);
+/* One pass over n samples: dir 0 = RealNVP.forward (+ log N(y; 0, I) with  */
+/* FMC_FLOW_BASE_LOGP: RealNVP.log_prob, realNVP.py:207-214), dir 1 =       */
+/* RealNVP.inverse; flags as fmc_flow_pass (STANDARDIZE: data statistics;   */
+/* SAMPLE_BASE with x_in NULL: base_dist draws from Philox stream 5 at      */
+/* sample index sample0 + row, RealNVP.sample realNVP.py:200-205). logdet   */
+/* [n] receives the summed log-dets (+ base log-density); saved [L, n, d]   */
+/* the input of every layer (forward only); gy_out [n, d] with g_logp [n]   */
+/* the gradient of g_logp * log N(y) at the output.                         */
+int fmc_rnvp_pass(void* stream, const fmc_rnvp* flow, int32_t dir, int32_t flags,
+                  const float* x_in, int64_t n, uint32_t key0, uint32_t key1, int64_t sample0,
+                  float* x_out, float* logdet, float* saved, const float* g_logp, float* gy_out);
+/* Adjoint of one layer (forward direction) for the loss sum g_logp *       */
+/* log_prob: replaces eqx.filter_value_and_grad in NFModel.train_step       */
+/* (nf_model/base.py:98-125) for RealNVP. gy [n, d] = dL/d(layer output)    */
+/* -> gx [n, d] (no aliasing); partials [tiles, partial_per_tile] receive   */
+/* the per-tile weight gradients, summed by fmc_rnvp_weight_grad.           */
+int fmc_rnvp_backward_layer(void* stream, const fmc_rnvp* flow, int32_t layer, const float* x_in,
+                            int64_t n, const float* g_logp, const float* gy, float* gx,
+                            float* partials, int64_t n_partial_tiles);
+int fmc_rnvp_weight_grad(void* stream, const fmc_rnvp* flow, int32_t layer, const float* partials,
+                         int64_t n_tiles, float* grad);
+
+/* ------------------------------------------------------------------------ */
 /* MaskedCouplingRQSpline flow                                              */
 /* (src/flowMC/resource/model/nf_model/rqSpline.py:364-504 with the spline  */
 /* of rqSpline.py:20-361 and the blocks of common.py:68-293).               */

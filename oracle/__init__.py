@@ -20,6 +20,10 @@ Pinning (what independently checks each function; see tests/test_oracle_*.py):
                     finite-difference derivative, change of variables via a
                     full Jacobian, quadrature normalisation in 2-D.
   optim             torch.optim.AdamW + global-norm clip (independent code).
+  realnvp           exact inverse, log|det J| from a full autograd Jacobian,
+                    2-D quadrature normalisation (= prod sigma, the missing
+                    standardisation term), identity at zero output weights,
+                    the alternating masks, the reference tests' shapes.
   tempering         exchange: invariance of the product of tempered
                     Gaussians under the swap sweep (moments of every slot),
                     the cold slot receiving a better hot walker, a walker
