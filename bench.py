@@ -511,6 +511,82 @@ def bench_flow_pass(args):
     }
 
 
+# ---------------------------------------------------------------- RealNVP --
+def bench_realnvp_train(args):
+    """RealNVP training (d=32, 8 layers, hidden 256, batch 10k, AdamW with
+    clipping) on a synthetic 4-mode mixture; a step is one optimizer step. The
+    sampling pass (409,600 samples from base_dist) is timed beside it."""
+    import torch
+
+    from flowmc_amd import random as fr
+    from flowmc_amd.resource.model.nf_model.realNVP import RealNVP
+    from flowmc_amd.resource.optimizer import adamw
+
+    dev = torch.device("cuda")
+    d, L, H, N, B = 32, 8, 256, 100000, 10000
+    g = torch.Generator().manual_seed(0)
+    means = torch.rand(4, d, generator=g) * 6 - 3
+    lab = torch.randint(0, 4, (N,), generator=g)
+    data = (means[lab] + torch.randn(N, d, generator=g)).to(dev)
+    m = RealNVP(d, L, H, fr.PRNGKey(0), device=dev)
+    m.set_data_stats(data.mean(0), torch.cov(data.T))
+    opt = adamw(1e-3, max_norm=1.0)
+    state = opt.init(m)
+    perm = torch.randperm(N, device=dev)
+    batches = [data[perm[i * B:(i + 1) * B]].contiguous() for i in range(N // B)]
+    for i in range(max(1, args.warmup)):
+        m.train_step(batches[i % len(batches)], opt, state)
+    torch.cuda.synchronize()
+    l0 = float(-m.log_prob(data[:20000]).mean())
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        loss, m, state = m.train_step(batches[i % len(batches)], opt, state)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    l1 = float(-m.log_prob(data[:20000]).mean())
+    fl_fwd = 4 * H * d * L  # two MLPs, d_c + d_t = d inputs + outputs, per sample
+    fl_step = 4 * fl_fwd * B  # forward + adjoint (recompute, two transposed products, two weight products)
+    achieved = fl_step * args.steps / wall / 1e12
+    n_s = 409600
+    ms_s = time_kernel(lambda: m.sample(fr.PRNGKey(1), n_s), reps=5)
+    out = {
+        "value": args.steps * B / wall,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "train step (fused forward + per-layer adjoint + partial reduction + AdamW), FP32",
+                     "flops_per_step": fl_step},
+        "extra": {"loss_before": l0, "loss_after": l1, "sample_ms_409600": ms_s,
+                  "sample_tflops": fl_fwd * n_s / (ms_s * 1e-3) / 1e12},
+        "config": {"workload": "realnvp_train_d32", "dim": d, "layers": L, "hidden": H, "batch": B, "examples": N,
+                   "optimizer": "adamw(1e-3) + clip 1.0"},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_realnvp(args, d, L, H, B)
+    return out
+
+
+def cpu_baseline_realnvp(args, d, L, H, B):
+    import torch
+
+    from oracle import realnvp as R
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    p = R.init_params(d, L, H, dtype=torch.float32)
+    theta = R.to_flat(p).clone().requires_grad_(True)
+    nb = min(B, args.cpu_chains)
+    x = torch.randn(nb, d)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        loss = R.loss(R.from_flat(theta, d, L, H), x)
+        (gt,) = torch.autograd.grad(loss, theta)
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": steps * nb / el, "unit": "samples/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"{steps} forward+backward steps on {nb}-sample batches, FP32 torch-CPU oracle ({cpu_model()})"}
+
+
 # ----------------------------------------------------- parallel tempering --
 def bench_pt_gmm64(args):
     """ParallelTempering in training mode on the 64-D 8-mode mixture: 4096
@@ -677,13 +753,17 @@ WORKLOADS = {
                     "f32/f64"),
     "flow_pass": (bench_flow_pass, "flow samples/s (inverse pass, d=64, 10 layers, [256,256])", "samples/s",
                   "bf16/f32"),
+    "realnvp_train_d32": (bench_realnvp_train, "RealNVP train samples/s (d=32, 8 layers, hidden 256, batch 10k)",
+                          "samples/s", "f32"),
     "pt_mala_gmm64": (bench_pt_gmm64, "tempered chain-steps/s (parallel tempering, 64-D 8-mode mixture, 5 temperatures)",
                       "tempered chain-steps/s", "f32"),
 }
 DEFAULT_STEPS = {"nfmcmc_gmm64": 20, "mala_rosenbrock_128d": 10048, "flow_train_d32": 50, "hmc_funnels": 1000,
-                 "flow_pass": 10, "grw_rosenbrock_128d": 10048, "pt_mala_gmm64": 50}
+                 "flow_pass": 10, "grw_rosenbrock_128d": 10048, "pt_mala_gmm64": 50,
+                 "realnvp_train_d32": 50}
 DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d32": 3, "hmc_funnels": 16,
-                  "flow_pass": 1, "grw_rosenbrock_128d": 640, "pt_mala_gmm64": 3}
+                  "flow_pass": 1, "grw_rosenbrock_128d": 640, "pt_mala_gmm64": 3,
+                  "realnvp_train_d32": 3}
 
 
 def main():
