@@ -138,7 +138,7 @@ _SIGNATURES: dict[str, list] = {
     "fmc_rnvp_pack": [P, C.POINTER(fmc_rnvp)],
     "fmc_rnvp_pass": [P, C.POINTER(fmc_rnvp), I32, I32, P, I64, U32, U32, I64, P, P, P, P, P],
     "fmc_rnvp_backward_layer": [P, C.POINTER(fmc_rnvp), I32, P, I64, P, P, P, P, I64],
-    "fmc_rnvp_weight_grad": [P, C.POINTER(fmc_rnvp), I32, P, I64, P],
+    "fmc_rnvp_weight_grad": [P, C.POINTER(fmc_rnvp), I32, P, I64, I64, P],
     "fmc_pt_adapt": [P, P, P, I64, I32],
     "fmc_mala_propose": [P, P, P, P, I64, I32, F32, U32, U32, I32, P],
     "fmc_mala_accept": [P, P, P, P, P, P, P, I64, I32, F32, U32, U32, I32, P, P, P, P, I64, I64, I32],

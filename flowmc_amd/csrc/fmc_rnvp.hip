@@ -5,18 +5,22 @@
 //   s = tanh(MLP_s(c)) dt,  t = MLP_t(c) dt,  y = (x + t) e^s,  log-det = sum s.
 // The model is FP32 end to end. Its GEMMs are small (d x hidden per sample and
 // layer), and on CDNA4 the FP32 vector FMA rate equals the FP32 MFMA rate, so
-// the products are FP32 FMA loops over LDS-resident activations (samples in
-// float4 groups) with weights streamed from an L2-resident packed copy.
+// the products are FP32 FMA register tiles (4 outputs x 4 samples per thread)
+// over LDS-resident activations, with weights read as aligned float4 rows of
+// an L2-resident packed copy. Products with few outputs (the d_t scale / shift
+// rows, the d_c conditioning gradients) split their reduction over the lanes
+// of a wavefront and combine them with xor shuffles.
 //
-// Work mapping: a workgroup of 256 threads owns RN_TS = 32 samples for the
-// whole pass (all layers), activations in LDS. The adjoint runs one layer per
-// launch and writes per-tile weight-gradient partials; a second kernel sums
-// them over tiles in a fixed order and scatters them into the gradient vector.
+// Work mapping: a workgroup of 256 threads owns RN_TS = 16 samples for the
+// whole pass (all layers), activations in LDS (~42 KB at d=32, hidden 256, so
+// three workgroups share a CU). The adjoint runs one layer per launch and
+// writes per-tile weight-gradient partials; a second kernel sums them over
+// tiles in a fixed order and scatters them into the gradient vector.
 #include "fmc_common.h"
 
 namespace fmc {
 
-constexpr int RN_TS = 32;       // samples per workgroup
+constexpr int RN_TS = 16;       // samples per workgroup
 constexpr int RN_THREADS = 256;
 constexpr int RN_Q = RN_TS / 4;  // float4 sample groups
 
@@ -40,18 +44,35 @@ __host__ __device__ inline int64_t rn_stats(int d, int h, int L) { return L * rn
 __host__ __device__ inline int64_t rn_theta_size(int d, int h, int L) {
   return rn_stats(d, h, L) + 2LL * d + 2LL * d * d;
 }
-// ---- packed copy, per (layer, p): W1T [d][h] (row k = k-th conditioning
-// dim), b1 [h], b2c [d] (entry i = i-th transformed dim); the second-layer
-// weights are read from theta (rows of W2 are contiguous in the hidden index).
-__host__ __device__ inline int64_t rn_pk_mlp(int d, int h) { return (int64_t)d * h + h + d; }
+// ---- packed copy, per (layer, p), with d4 / h4 = d / h rounded up to 4 and
+// zero padding: W1T [d4][h4] (W1T[k][j] = W1[j][cond_k]), W1c [h4][d4] (its
+// transpose), W2T [h4][d4] (W2T[j][i] = W2[trans_i][j]), W2c [d4][h4] (its
+// transpose), b1 [h4], b2c [d4] (b2c[i] = b2[trans_i]). Every row starts on a
+// 16-byte boundary.
+__host__ __device__ inline int rn_r4(int v) { return (v + 3) & ~3; }
+__host__ __device__ inline int64_t rn_pk_mlp(int d, int h) {
+  return 4LL * rn_r4(d) * rn_r4(h) + rn_r4(h) + rn_r4(d);
+}
 __host__ __device__ inline int64_t rn_pk(int d, int h, int l, int p) { return (2LL * l + p) * rn_pk_mlp(d, h); }
+struct RnPk {
+  const float *W1T, *W1c, *W2T, *W2c, *b1, *b2c;
+  __device__ RnPk(const float* base, int d4, int h4) {
+    const int64_t s = (int64_t)d4 * h4;
+    W1T = base;
+    W1c = base + s;
+    W2T = base + 2 * s;
+    W2c = base + 3 * s;
+    b1 = base + 4 * s;
+    b2c = b1 + h4;
+  }
+};
 // ---- layout: per layer [n_cond, n_trans, cond[d], trans[d]]
 __host__ __device__ inline const int32_t* rn_lay(const RnvpDev& f, int l) { return f.layout + (int64_t)l * (2 + 2 * f.d); }
 
 __host__ __device__ inline int rn_rup16(int v) { return (v + 15) & ~15; }
 
 struct RnSmem {
-  int xs, gb, cT, hT, ghT, sT, tT, ldr, glp, total;
+  int xs, gb, cT, hT, ghT, sT, tT, gcT, ldr, glp, total;
   int ldx, d4, h4;
   __host__ __device__ RnSmem(int d, int h, bool bwd) {
     ldx = d + 1;
@@ -72,6 +93,8 @@ struct RnSmem {
     o += d4 * RN_TS * 4;
     tT = o;  // [d4][TS] shift outputs / their gradients
     o += d4 * RN_TS * 4;
+    gcT = o;  // [d4][TS] conditioning-input gradients of one MLP (adjoint)
+    o += bwd ? d4 * RN_TS * 4 : 0;
     ldr = o;
     o += RN_TS * 4;
     glp = o;
@@ -80,46 +103,105 @@ struct RnSmem {
   }
 };
 
-// H[j][m] = relu(b1[j] + sum_k W1T[k][j] c[k][m]) for the 32 samples of the tile.
-__device__ __forceinline__ void rn_gemm1(const float* __restrict__ W1T, const float* __restrict__ b1, int h, int dc,
-                                         const float* cT, float* hT) {
-  for (int o = threadIdx.x; o < h * RN_Q; o += RN_THREADS) {
-    const int j = o % h, q = o / h;
-    const float b = b1[j];
-    float4 acc = make_float4(b, b, b, b);
-    for (int k = 0; k < dc; ++k) {
-      const float w = W1T[(int64_t)k * h + j];
-      const float4 c = *reinterpret_cast<const float4*>(cT + k * RN_TS + 4 * q);
-      acc.x = fmaf(w, c.x, acc.x);
-      acc.y = fmaf(w, c.y, acc.y);
-      acc.z = fmaf(w, c.z, acc.z);
-      acc.w = fmaf(w, c.w, acc.w);
+#define RN_FMA16(acc, w, c)                    \
+  do {                                         \
+    acc[0][0] = fmaf(w.x, c.x, acc[0][0]);     \
+    acc[0][1] = fmaf(w.x, c.y, acc[0][1]);     \
+    acc[0][2] = fmaf(w.x, c.z, acc[0][2]);     \
+    acc[0][3] = fmaf(w.x, c.w, acc[0][3]);     \
+    acc[1][0] = fmaf(w.y, c.x, acc[1][0]);     \
+    acc[1][1] = fmaf(w.y, c.y, acc[1][1]);     \
+    acc[1][2] = fmaf(w.y, c.z, acc[1][2]);     \
+    acc[1][3] = fmaf(w.y, c.w, acc[1][3]);     \
+    acc[2][0] = fmaf(w.z, c.x, acc[2][0]);     \
+    acc[2][1] = fmaf(w.z, c.y, acc[2][1]);     \
+    acc[2][2] = fmaf(w.z, c.z, acc[2][2]);     \
+    acc[2][3] = fmaf(w.z, c.w, acc[2][3]);     \
+    acc[3][0] = fmaf(w.w, c.x, acc[3][0]);     \
+    acc[3][1] = fmaf(w.w, c.y, acc[3][1]);     \
+    acc[3][2] = fmaf(w.w, c.z, acc[3][2]);     \
+    acc[3][3] = fmaf(w.w, c.w, acc[3][3]);     \
+  } while (0)
+
+// Wide product, h4 output rows: out[r][m] = sum_k W[k][r] in[k][m] (W rows of
+// length h4), one 4-row x 4-sample tile per thread. MODE 0: out = relu(bias +
+// .); MODE 1: out = . where hmask > 0, else 0 (ReLU derivative). Padding rows
+// come out zero (zero weights and bias).
+template <int MODE>
+__device__ __forceinline__ void rn_wide(const float* __restrict__ W, const float* __restrict__ bias, int K, int h4,
+                                        const float* inT, float* outT, const float* hmask) {
+  const int tr = h4 >> 2;
+  for (int o = threadIdx.x; o < tr * RN_Q; o += RN_THREADS) {
+    const int jq = o % tr, q = o / tr;
+    float acc[4][4];
+    const float4 b = (MODE == 0) ? *reinterpret_cast<const float4*>(bias + 4 * jq) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      acc[0][v] = b.x;
+      acc[1][v] = b.y;
+      acc[2][v] = b.z;
+      acc[3][v] = b.w;
     }
-    acc.x = fmaxf(acc.x, 0.f);
-    acc.y = fmaxf(acc.y, 0.f);
-    acc.z = fmaxf(acc.z, 0.f);
-    acc.w = fmaxf(acc.w, 0.f);
-    *reinterpret_cast<float4*>(hT + j * RN_TS + 4 * q) = acc;
+    for (int k = 0; k < K; ++k) {
+      const float4 w = *reinterpret_cast<const float4*>(W + (int64_t)k * h4 + 4 * jq);
+      const float4 c = *reinterpret_cast<const float4*>(inT + k * RN_TS + 4 * q);
+      RN_FMA16(acc, w, c);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float4 r = make_float4(acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
+      const int row = 4 * jq + u;
+      if (MODE == 0) {
+        r.x = fmaxf(r.x, 0.f);
+        r.y = fmaxf(r.y, 0.f);
+        r.z = fmaxf(r.z, 0.f);
+        r.w = fmaxf(r.w, 0.f);
+      } else {
+        const float4 hv = *reinterpret_cast<const float4*>(hmask + row * RN_TS + 4 * q);
+        r.x = hv.x > 0.f ? r.x : 0.f;
+        r.y = hv.y > 0.f ? r.y : 0.f;
+        r.z = hv.z > 0.f ? r.z : 0.f;
+        r.w = hv.w > 0.f ? r.w : 0.f;
+      }
+      *reinterpret_cast<float4*>(outT + row * RN_TS + 4 * q) = r;
+    }
   }
 }
 
-// out[i][m] = b2c[i] + sum_j W2[trans_i][j] H[j][m] for the transformed dims i < dtr.
-__device__ __forceinline__ void rn_gemm2(const float* __restrict__ W2, const float* __restrict__ b2c, int h, int dtr,
-                                         const int32_t* trans, const float* hT, float* outT) {
-  for (int o = threadIdx.x; o < dtr * RN_Q; o += RN_THREADS) {
-    const int i = o % dtr, q = o / dtr;
-    const float* w = W2 + (int64_t)trans[i] * h;
-    const float b = b2c[i];
-    float4 acc = make_float4(b, b, b, b);
-    for (int j = 0; j < h; ++j) {
-      const float wj = w[j];
-      const float4 v = *reinterpret_cast<const float4*>(hT + j * RN_TS + 4 * q);
-      acc.x = fmaf(wj, v.x, acc.x);
-      acc.y = fmaf(wj, v.y, acc.y);
-      acc.z = fmaf(wj, v.z, acc.z);
-      acc.w = fmaf(wj, v.w, acc.w);
+// Narrow product, n output rows (written up to n rounded to 4; the padding
+// rows come out zero): out[r][m] = bias[r] + sum_{j < K} W[j][r] in[j][m] (W
+// rows of length ldw). The reduction over j is split over S lanes of one
+// wavefront (S a power of two, tiles * S <= 256) and combined with xor
+// shuffles in a fixed order.
+__device__ __forceinline__ void rn_narrow(const float* __restrict__ W, int ldw, const float* __restrict__ bias, int K,
+                                          int n, const float* inT, float* outT) {
+  const int tr = (n + 3) >> 2;
+  const int tiles = tr * RN_Q;
+  int S = 1;
+  while (S < 64 && tiles * S * 2 <= RN_THREADS) S <<= 1;
+  for (int o = threadIdx.x; o < tiles * S; o += RN_THREADS) {
+    const int tile = o / S, s = o - tile * S;
+    const int iq = tile % tr, q = tile / tr;
+    float acc[4][4] = {};
+    for (int j = s; j < K; j += S) {
+      const float4 w = *reinterpret_cast<const float4*>(W + (int64_t)j * ldw + 4 * iq);
+      const float4 c = *reinterpret_cast<const float4*>(inT + j * RN_TS + 4 * q);
+      RN_FMA16(acc, w, c);
     }
-    *reinterpret_cast<float4*>(outT + i * RN_TS + 4 * q) = acc;
+    for (int off = S >> 1; off > 0; off >>= 1)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += __shfl_xor(acc[u][v], off, 64);
+    if (s == 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int row = 4 * iq + u;
+        const float b = bias ? bias[row] : 0.f;
+        *reinterpret_cast<float4*>(outT + row * RN_TS + 4 * q) =
+            make_float4(acc[u][0] + b, acc[u][1] + b, acc[u][2] + b, acc[u][3] + b);
+      }
+    }
   }
 }
 
@@ -133,25 +215,39 @@ __device__ __forceinline__ void rn_load_cond(const float* xs, int ldx, const int
 
 // ------------------------------------------------------------- packing ---
 __global__ void __launch_bounds__(256) rnvp_pack_kernel(RnvpDev f, float* packed) {
-  const int d = f.d, h = f.h;
+  const int d = f.d, h = f.h, d4 = rn_r4(d), h4 = rn_r4(h);
   const int lp = blockIdx.y, l = lp >> 1, p = lp & 1;
   const int32_t* lay = rn_lay(f, l);
   const int dc = lay[0], dtr = lay[1];
   const int32_t* cond = lay + 2;
   const int32_t* trans = lay + 2 + d;
-  const float* src = f.theta + rn_mlp(d, h, l, p);
+  const float* W1 = f.theta + rn_mlp(d, h, l, p);
+  const float* b1 = W1 + (int64_t)h * d;
+  const float* W2 = b1 + h;
+  const float* b2 = W2 + (int64_t)d * h;
   float* dst = packed + rn_pk(d, h, l, p);
-  const int64_t n = rn_pk_mlp(d, h);
+  const int64_t sq = (int64_t)d4 * h4, n = rn_pk_mlp(d, h);
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    float v;
-    if (e < (int64_t)d * h) {  // W1T[k][j] = W1[j][cond_k]
-      const int k = (int)(e / h), j = (int)(e - (int64_t)k * h);
-      v = (k < dc) ? src[(int64_t)j * d + cond[k]] : 0.f;
-    } else if (e < (int64_t)d * h + h) {
-      v = src[(int64_t)h * d + (e - (int64_t)d * h)];
-    } else {
-      const int i = (int)(e - (int64_t)d * h - h);
-      v = (i < dtr) ? src[2LL * h * d + h + trans[i]] : 0.f;
+    float v = 0.f;
+    const int region = (e < 4 * sq) ? (int)(e / sq) : 4;
+    const int64_t r = e - (int64_t)region * sq;
+    if (region == 0) {  // W1T[k][j]
+      const int k = (int)(r / h4), j = (int)(r - (int64_t)k * h4);
+      if (k < dc && j < h) v = W1[(int64_t)j * d + cond[k]];
+    } else if (region == 1) {  // W1c[j][k]
+      const int j = (int)(r / d4), k = (int)(r - (int64_t)j * d4);
+      if (k < dc && j < h) v = W1[(int64_t)j * d + cond[k]];
+    } else if (region == 2) {  // W2T[j][i]
+      const int j = (int)(r / d4), i = (int)(r - (int64_t)j * d4);
+      if (i < dtr && j < h) v = W2[(int64_t)trans[i] * h + j];
+    } else if (region == 3) {  // W2c[i][j]
+      const int i = (int)(r / h4), j = (int)(r - (int64_t)i * h4);
+      if (i < dtr && j < h) v = W2[(int64_t)trans[i] * h + j];
+    } else if (r < h4) {  // b1
+      if (r < h) v = b1[r];
+    } else {  // b2c
+      const int i = (int)(r - h4);
+      if (i < dtr) v = b2[trans[i]];
     }
     dst[e] = v;
   }
@@ -232,11 +328,10 @@ __global__ void __launch_bounds__(RN_THREADS) rnvp_pass_kernel(RnvpDev f, RnPass
     rn_load_cond(xs, ldx, cond, dc, sm.d4, cT);
     __syncthreads();
     for (int p = 0; p < 2; ++p) {
-      const float* pk = f.packed + rn_pk(d, h, l, p);
-      rn_gemm1(pk, pk + (int64_t)d * h, h, dc, cT, hT);
+      const RnPk pk(f.packed + rn_pk(d, h, l, p), sm.d4, sm.h4);
+      rn_wide<0>(pk.W1T, pk.b1, dc, sm.h4, cT, hT, nullptr);
       __syncthreads();
-      rn_gemm2(th + rn_mlp(d, h, l, p) + (int64_t)h * d + h, pk + (int64_t)d * h + h, h, dtr, trans, hT,
-               p == 0 ? sT : tT);
+      rn_narrow(pk.W2T, sm.d4, pk.b2c, h, dtr, hT, p == 0 ? sT : tT);
       __syncthreads();
     }
     // coupling on the transformed dims; sT keeps s for the log-det sum
@@ -353,6 +448,7 @@ __global__ void __launch_bounds__(RN_THREADS) rnvp_bwd_kernel(RnvpDev f, RnBwdAr
   float* ghT = reinterpret_cast<float*>(smem + sm.ghT);
   float* sT = reinterpret_cast<float*>(smem + sm.sT);
   float* tT = reinterpret_cast<float*>(smem + sm.tT);
+  float* gcT = reinterpret_cast<float*>(smem + sm.gcT);
   float* glp = reinterpret_cast<float*>(smem + sm.glp);
   const int tid = threadIdx.x, ldx = sm.ldx;
   const int64_t m0 = (int64_t)blockIdx.x * RN_TS;
@@ -369,30 +465,21 @@ __global__ void __launch_bounds__(RN_THREADS) rnvp_bwd_kernel(RnvpDev f, RnBwdAr
     gb[m * ldx + j] = ok ? a.gy[row * d + j] : 0.f;
   }
   for (int m = tid; m < RN_TS; m += RN_THREADS) glp[m] = (m0 + m < a.n) ? a.g_logp[m0 + m] : 0.f;
-  // zero the padding rows of the transposed buffers (read by the 4x4 tiles)
-  for (int o = tid; o < (sm.d4 - dtr) * RN_TS; o += RN_THREADS) {
-    sT[dtr * RN_TS + o] = 0.f;
-    tT[dtr * RN_TS + o] = 0.f;
-  }
-  for (int o = tid; o < (sm.h4 - h) * RN_TS; o += RN_THREADS) {
-    hT[h * RN_TS + o] = 0.f;
-    ghT[h * RN_TS + o] = 0.f;
-  }
   __syncthreads();
   rn_load_cond(xs, ldx, cond, dc, sm.d4, cT);
   __syncthreads();
   // ---- recompute scale / shift pre-activations
   for (int p = 0; p < 2; ++p) {
-    const float* pk = f.packed + rn_pk(d, h, l, p);
-    rn_gemm1(pk, pk + (int64_t)d * h, h, dc, cT, hT);
+    const RnPk pk(f.packed + rn_pk(d, h, l, p), sm.d4, sm.h4);
+    rn_wide<0>(pk.W1T, pk.b1, dc, sm.h4, cT, hT, nullptr);
     __syncthreads();
-    rn_gemm2(f.theta + rn_mlp(d, h, l, p) + (int64_t)h * d + h, pk + (int64_t)d * h + h, h, dtr, trans, hT,
-             p == 0 ? sT : tT);
+    rn_narrow(pk.W2T, sm.d4, pk.b2c, h, dtr, hT, p == 0 ? sT : tT);
     __syncthreads();
   }
   // ---- coupling reverse sweep: y = (x + t) e^s, log-det += s
   //   g_x = g e^s (transformed dim), g_t = g e^s, g_s = g y + g_logp;
-  //   g_spre = g_s dt (1 - tanh^2), g_tpre = g_t dt (in place of sT / tT)
+  //   g_spre = g_s dt (1 - tanh^2), g_tpre = g_t dt (in place of sT / tT;
+  //   their padding rows stay zero)
   for (int o = tid; o < dtr * RN_TS; o += RN_THREADS) {
     const int i = o / RN_TS, m = o - i * RN_TS;
     const int j = trans[i];
@@ -411,56 +498,25 @@ __global__ void __launch_bounds__(RN_THREADS) rnvp_bwd_kernel(RnvpDev f, RnBwdAr
   __syncthreads();
   float* part = a.part + (int64_t)blockIdx.x * rn_layer_size(d, h);
   for (int p = 0; p < 2; ++p) {
-    const float* pk = f.packed + rn_pk(d, h, l, p);
-    const float* W1 = f.theta + rn_mlp(d, h, l, p);
-    const float* W2 = W1 + (int64_t)h * d + h;
+    const RnPk pk(f.packed + rn_pk(d, h, l, p), sm.d4, sm.h4);
     const float* goT = (p == 0) ? sT : tT;
     float* pp = part + p * rn_mlp_size(d, h);
-    rn_gemm1(pk, pk + (int64_t)d * h, h, dc, cT, hT);
+    rn_wide<0>(pk.W1T, pk.b1, dc, sm.h4, cT, hT, nullptr);
     __syncthreads();
     // g_H[j][m] = sum_i W2[trans_i][j] g_out[i][m], masked by ReLU' (H > 0)
-    for (int o = tid; o < h * RN_Q; o += RN_THREADS) {
-      const int j = o % h, q = o / h;
-      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (int i = 0; i < dtr; ++i) {
-        const float w = W2[(int64_t)trans[i] * h + j];
-        const float4 g = *reinterpret_cast<const float4*>(goT + i * RN_TS + 4 * q);
-        acc.x = fmaf(w, g.x, acc.x);
-        acc.y = fmaf(w, g.y, acc.y);
-        acc.z = fmaf(w, g.z, acc.z);
-        acc.w = fmaf(w, g.w, acc.w);
-      }
-      const float4 hv = *reinterpret_cast<const float4*>(hT + j * RN_TS + 4 * q);
-      acc.x = hv.x > 0.f ? acc.x : 0.f;
-      acc.y = hv.y > 0.f ? acc.y : 0.f;
-      acc.z = hv.z > 0.f ? acc.z : 0.f;
-      acc.w = hv.w > 0.f ? acc.w : 0.f;
-      *reinterpret_cast<float4*>(ghT + j * RN_TS + 4 * q) = acc;
-    }
+    rn_wide<1>(pk.W2c, nullptr, dtr, sm.h4, goT, ghT, hT);
     __syncthreads();
     // tile partials: dW1[j][k] = sum_m gH[j][m] c[k][m]; db1; dW2[i][j] = sum_m g_out[i][m] H[j][m]; db2
     rn_outer(ghT, h, cT, dc, pp, d);
     rn_rowsum(ghT, h, pp + (int64_t)h * d);
     rn_outer(goT, dtr, hT, h, pp + (int64_t)h * d + h, h);
     rn_rowsum(goT, dtr, pp + 2LL * h * d + h);
-    // g_c[m][k] += sum_j W1[j][cond_k] gH[j][m]   (one thread per (k, 4 samples))
-    for (int o = tid; o < dc * RN_Q; o += RN_THREADS) {
-      const int k = o % dc, q = o / dc;
-      const float* w = W1 + cond[k];
-      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (int j = 0; j < h; ++j) {
-        const float wj = w[(int64_t)j * d];
-        const float4 g = *reinterpret_cast<const float4*>(ghT + j * RN_TS + 4 * q);
-        acc.x = fmaf(wj, g.x, acc.x);
-        acc.y = fmaf(wj, g.y, acc.y);
-        acc.z = fmaf(wj, g.z, acc.z);
-        acc.w = fmaf(wj, g.w, acc.w);
-      }
-      const int jd = cond[k];
-      gb[(4 * q + 0) * ldx + jd] += acc.x;
-      gb[(4 * q + 1) * ldx + jd] += acc.y;
-      gb[(4 * q + 2) * ldx + jd] += acc.z;
-      gb[(4 * q + 3) * ldx + jd] += acc.w;
+    // g_c[k][m] = sum_j W1[j][cond_k] gH[j][m], added to the conditioning dims
+    rn_narrow(pk.W1c, sm.d4, nullptr, h, dc, ghT, gcT);
+    __syncthreads();
+    for (int o = tid; o < dc * RN_TS; o += RN_THREADS) {
+      const int k = o / RN_TS, m = o - k * RN_TS;
+      gb[m * ldx + cond[k]] += gcT[o];
     }
     __syncthreads();
   }
@@ -471,10 +527,36 @@ __global__ void __launch_bounds__(RN_THREADS) rnvp_bwd_kernel(RnvpDev f, RnBwdAr
   }
 }
 
-// Sum the tile partials of one layer in tile order and scatter them to the
+// Sum of the tile partials of one layer, in two fixed-order stages so that
+// enough lanes stream the partial buffer: stage 1 sums chunk c of the tiles
+// (tiles c, c + C, c + 2C, ...) into row n_tiles + c of the buffer; stage 2
+// sums the C chunk rows in order and scatters the result to the
 // reference-shaped leaves of the gradient vector.
-__global__ void __launch_bounds__(256) rnvp_wgrad_kernel(RnvpDev f, int l, const float* __restrict__ part,
-                                                         int64_t n_tiles, float* __restrict__ grad) {
+constexpr int RN_RED_CHUNKS = 16;
+
+__host__ __device__ inline int rn_chunks(int64_t n_tiles) {
+  const int64_t c = (n_tiles + 15) / 16;
+  return (int)(c < 1 ? 1 : (c > RN_RED_CHUNKS ? RN_RED_CHUNKS : c));
+}
+
+__global__ void __launch_bounds__(256) rnvp_wgrad_stage1(int64_t LS, float* part, int64_t n_tiles, int C) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (e >= LS) return;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int64_t t = c;
+  for (; t + 3 * C < n_tiles; t += 4 * C) {
+    s0 += part[t * LS + e];
+    s1 += part[(t + C) * LS + e];
+    s2 += part[(t + 2 * C) * LS + e];
+    s3 += part[(t + 3 * C) * LS + e];
+  }
+  for (; t < n_tiles; t += C) s0 += part[t * LS + e];
+  part[(n_tiles + c) * LS + e] = (s0 + s1) + (s2 + s3);
+}
+
+__global__ void __launch_bounds__(256) rnvp_wgrad_stage2(RnvpDev f, int l, const float* __restrict__ part,
+                                                         int64_t n_tiles, int C, float* __restrict__ grad) {
   const int d = f.d, h = f.h;
   const int64_t LS = rn_layer_size(d, h), MS = rn_mlp_size(d, h);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,7 +583,7 @@ __global__ void __launch_bounds__(256) rnvp_wgrad_kernel(RnvpDev f, int l, const
   }
   if (dst < 0) return;
   float s = 0.f;
-  for (int64_t t = 0; t < n_tiles; ++t) s += part[t * LS + e];
+  for (int c = 0; c < C; ++c) s += part[(n_tiles + c) * LS + e];
   grad[rn_mlp(d, h, l, p) + dst] = s;
 }
 
@@ -539,7 +621,7 @@ extern "C" int fmc_rnvp_sizes(const fmc_rnvp* r, int64_t* theta_size, int64_t* p
   if (rc) return rc;
   if (theta_size) *theta_size = rn_theta_size(f.d, f.h, f.L);
   if (packed_size) *packed_size = 2LL * f.L * rn_pk_mlp(f.d, f.h);
-  if (partial_per_tile) *partial_per_tile = rn_layer_size(f.d, f.h);
+  if (partial_per_tile) *partial_per_tile = rn_layer_size(f.d, f.h);  // rows: tiles + chunk rows
   if (tile) *tile = RN_TS;
   return FMC_OK;
 }
@@ -549,7 +631,7 @@ extern "C" int fmc_rnvp_pack(void* stream, const fmc_rnvp* r) {
   int rc = check_rnvp(r, f, true);
   if (rc) return rc;
   const int64_t n = rn_pk_mlp(f.d, f.h);
-  const unsigned gx = (unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+  const unsigned gx = (unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
   hipLaunchKernelGGL(rnvp_pack_kernel, dim3(gx, 2 * f.L), dim3(256), 0, (hipStream_t)stream, f, r->packed);
   return check_launch("realnvp pack");
 }
@@ -600,16 +682,22 @@ extern "C" int fmc_rnvp_backward_layer(void* stream, const fmc_rnvp* r, int32_t 
   return check_launch("realnvp adjoint");
 }
 
-extern "C" int fmc_rnvp_weight_grad(void* stream, const fmc_rnvp* r, int32_t layer, const float* partials,
-                                    int64_t n_tiles, float* grad) {
+extern "C" int fmc_rnvp_weight_grad(void* stream, const fmc_rnvp* r, int32_t layer, float* partials,
+                                    int64_t n_tiles, int64_t partial_rows, float* grad) {
   RnvpDev f;
   int rc = check_rnvp(r, f, false);
   if (rc) return rc;
   FMC_ARG(layer >= 0 && layer < f.L, "layer %d out of range", layer);
   FMC_ARG(partials && grad, "NULL pointer");
-  FMC_ARG(n_tiles >= 0, "n_tiles < 0");
+  FMC_ARG(n_tiles >= 1, "n_tiles must be >= 1");
+  const int C = rn_chunks(n_tiles);
+  FMC_ARG(partial_rows >= n_tiles + C, "partials hold %lld rows, %lld needed (tiles + %d reduction rows)",
+          (long long)partial_rows, (long long)(n_tiles + C), C);
   const int64_t LS = rn_layer_size(f.d, f.h);
-  hipLaunchKernelGGL(rnvp_wgrad_kernel, dim3((unsigned)((LS + 255) / 256)), dim3(256), 0, (hipStream_t)stream, f,
-                     layer, partials, n_tiles, grad);
+  const unsigned gx = (unsigned)((LS + 255) / 256);
+  hipLaunchKernelGGL(rnvp_wgrad_stage1, dim3(gx, C), dim3(256), 0, (hipStream_t)stream, LS, partials, n_tiles, C);
+  if ((rc = check_launch("realnvp weight gradient (tile sums)"))) return rc;
+  hipLaunchKernelGGL(rnvp_wgrad_stage2, dim3(gx), dim3(256), 0, (hipStream_t)stream, f, layer, partials, n_tiles,
+                     C, grad);
   return check_launch("realnvp weight gradient");
 }

@@ -55,7 +55,8 @@ class _Workspace:
         self.gy = torch.empty((B, d), dtype=torch.float32, device=dev)
         self.gx = torch.empty((B, d), dtype=torch.float32, device=dev)
         self.n_tiles = (B + model._tile - 1) // model._tile
-        self.part = torch.empty((self.n_tiles, model._part_per_tile), dtype=torch.float32, device=dev)
+        # tile rows plus the reduction's chunk rows (at most 16)
+        self.part = torch.empty((self.n_tiles + 16, model._part_per_tile), dtype=torch.float32, device=dev)
         self.grad = torch.zeros(model.theta.numel(), dtype=torch.float32, device=dev)
         self.norm_ws = torch.empty(1024, dtype=torch.float64, device=dev)
         self.gnorm = torch.empty(1, dtype=torch.float32, device=dev)
@@ -217,7 +218,7 @@ class _RnvpBase(NFModel):
             _native.call("fmc_rnvp_backward_layer", st, C.byref(s), layer, ws.saved[layer].data_ptr(), B,
                          ws.g_logp.data_ptr(), gy.data_ptr(), gx.data_ptr(), ws.part.data_ptr(), ws.n_tiles)
             _native.call("fmc_rnvp_weight_grad", st, C.byref(s), layer, ws.part.data_ptr(), ws.n_tiles,
-                         ws.grad.data_ptr())
+                         ws.part.shape[0], ws.grad.data_ptr())
             gy, gx = gx, gy
         return gy  # dL/d(input of layer 0)
 

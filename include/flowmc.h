@@ -338,13 +338,15 @@ int fmc_rnvp_pass(void* stream, const fmc_rnvp* flow, int32_t dir, int32_t flags
 /* Adjoint of one layer (forward direction) for the loss sum g_logp *       */
 /* log_prob: replaces eqx.filter_value_and_grad in NFModel.train_step       */
 /* (nf_model/base.py:98-125) for RealNVP. gy [n, d] = dL/d(layer output)    */
-/* -> gx [n, d] (no aliasing); partials [tiles, partial_per_tile] receive   */
-/* the per-tile weight gradients, summed by fmc_rnvp_weight_grad.           */
+/* -> gx [n, d] (no aliasing); partials [rows, partial_per_tile] receive    */
+/* the per-tile weight gradients (rows 0 .. tiles-1), summed in a fixed     */
+/* order by fmc_rnvp_weight_grad, which uses up to 16 more rows             */
+/* (partial_rows >= tiles + min(16, ceil(tiles / 16))).                     */
 int fmc_rnvp_backward_layer(void* stream, const fmc_rnvp* flow, int32_t layer, const float* x_in,
                             int64_t n, const float* g_logp, const float* gy, float* gx,
                             float* partials, int64_t n_partial_tiles);
-int fmc_rnvp_weight_grad(void* stream, const fmc_rnvp* flow, int32_t layer, const float* partials,
-                         int64_t n_tiles, float* grad);
+int fmc_rnvp_weight_grad(void* stream, const fmc_rnvp* flow, int32_t layer, float* partials,
+                         int64_t n_tiles, int64_t partial_rows, float* grad);
 
 /* ------------------------------------------------------------------------ */
 /* MaskedCouplingRQSpline flow                                              */
