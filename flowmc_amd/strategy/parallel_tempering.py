@@ -79,25 +79,59 @@ class ParallelTempering(Strategy):
         return rng_key, resources, positions[:, 0]
 
     # ------------------------------------------------------------------
+    def _individual_step_body(self, kernel: ProposalBase, carry, aux):
+        """One tempered step of one walker (parallel_tempering.py:135-205):
+        carry = (key, position [d], log_prob, logpdf, temperature, data);
+        returns the new carry and (position, log_prob, do_accept)."""
+        key, position, log_prob, logpdf, temperature, data = carry
+        key, subkey = frandom.split(key)
+        pos, lp, acc = self._walker_steps(kernel, subkey, position, logpdf, temperature, data, 1)
+        return (key, pos, lp, logpdf, temperature, data), (pos, lp, acc[0])
+
+    def _individal_step(self, kernel: ProposalBase, rng_key, positions, logpdf: TemperedPDF, temperatures, data):
+        """n_steps tempered steps of one walker (parallel_tempering.py:207-248,
+        the reference's spelling): returns the last position [d], its tempered
+        log-probability and the step decisions [n_steps]."""
+        return self._walker_steps(kernel, rng_key, positions, logpdf, temperatures, data, self.n_steps)
+
+    def _walker_steps(self, kernel, key, position, logpdf, temperature, data, n):
+        x = torch.as_tensor(position, dtype=torch.float32)
+        x = (x if x.is_cuda else x.cuda()).reshape(1, 1, -1)
+        t = torch.as_tensor(temperature, dtype=torch.float32).reshape(1).to(x.device)
+        acc = torch.empty(1, max(1, n), dtype=torch.float32, device=x.device)
+        pos, lp = kernel.sample_tempered(key, x, logpdf, t, data, n, accepts=acc)
+        return pos.reshape(-1), lp.reshape(()), acc[0, :n] > 0
+
     def _ensemble_step(self, kernel: ProposalBase, rng_key, positions: torch.Tensor, logpdf: TemperedPDF,
                        temperatures: torch.Tensor, data, noise=None, log_u=None):
-        """n_steps local steps of all walkers [C, T, d] (parallel_tempering.py:
-        135-290). Returns positions [C, T, d], tempered log-probs [C, T] and the
-        step decisions [C, T, n_steps]."""
-        C_, T, _ = positions.shape
-        acc = torch.empty(C_ * T, max(1, self.n_steps), dtype=torch.float32, device=positions.device)
-        pos, lp = kernel.sample_tempered(rng_key, positions, logpdf, temperatures, data, self.n_steps, noise=noise,
+        """n_steps local steps of all walkers (parallel_tempering.py:
+        135-290): positions [C, T, d], or [T, d] for one chain as in the
+        reference's per-chain helper. Returns positions, tempered log-probs
+        [(C,) T] and the step decisions [(C,) T, n_steps]."""
+        x = torch.as_tensor(positions, dtype=torch.float32)
+        x = x if x.is_cuda else x.cuda()
+        single = x.ndim == 2
+        x = x[None] if single else x
+        C_, T, _ = x.shape
+        temps = torch.as_tensor(temperatures, dtype=torch.float32).to(x.device)
+        acc = torch.empty(C_ * T, max(1, self.n_steps), dtype=torch.float32, device=temps.device)
+        pos, lp = kernel.sample_tempered(rng_key, x, logpdf, temps, data, self.n_steps, noise=noise,
                                          log_u=log_u, accepts=acc)
-        return pos, lp, acc[:, :self.n_steps].reshape(C_, T, self.n_steps) > 0
+        acc = acc[:, :self.n_steps].reshape(C_, T, self.n_steps) > 0
+        return (pos[0], lp[0], acc[0]) if single else (pos, lp, acc)
 
     def _exchange(self, rng_key, positions: torch.Tensor, logpdf: TemperedPDF, temperatures: torch.Tensor, data,
                   log_u=None):
         """One sweep of adjacent swaps per chain (parallel_tempering.py:291-398)
-        on the untempered log-likelihoods (``logpdf(x, data)``). Returns the
-        positions [C, T, d], their log-likelihoods [C, T] and the swap decisions
-        [C, T-1]."""
-        C_, T, d = positions.shape
-        x = positions.contiguous()
+        on the untempered log-likelihoods (``logpdf(x, data)``). positions
+        [C, T, d], or [T, d] for one chain. Returns the positions, their
+        log-likelihoods [(C,) T] and the swap decisions [(C,) T-1]."""
+        single = positions.ndim == 2
+        x = torch.as_tensor(positions, dtype=torch.float32)
+        x = (x if x.is_cuda else x.cuda())
+        x = (x[None] if single else x).contiguous()
+        C_, T, d = x.shape
+        temps = torch.as_tensor(temperatures, dtype=torch.float32).to(x.device).contiguous()
         loglik = _eval_logpdf(logpdf, x.reshape(C_ * T, d), data).reshape(C_, T).to(torch.float32).contiguous()
         loglik = loglik.clone()
         out = torch.empty_like(x)
@@ -105,10 +139,11 @@ class ParallelTempering(Strategy):
         k0, k1 = frandom.key_words(rng_key)
         lu = log_u.to(device=x.device, dtype=torch.float32).contiguous() if log_u is not None else None
         _native.call("fmc_pt_exchange", _native.stream_ptr(), _native.dptr(x, name="positions"),
-                     _native.dptr(out), _native.dptr(loglik), _native.dptr(temperatures, name="temperatures", numel=T),
+                     _native.dptr(out), _native.dptr(loglik), _native.dptr(temps, name="temperatures", numel=T),
                      C_, T, d, k0, k1, _native.dptr(lu, name="log_u", numel=C_ * (T - 1) if lu is not None else None),
                      _native.dptr(acc))
-        return out, loglik, acc[:, :T - 1] > 0
+        acc = acc[:, :T - 1] > 0
+        return (out[0], loglik[0], acc[0]) if single else (out, loglik, acc)
 
     def _adapt_temperature(self, temperatures: torch.Tensor, do_accept: torch.Tensor) -> torch.Tensor:
         """Temperature adaptation (parallel_tempering.py:400-436) from the swap

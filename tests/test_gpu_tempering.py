@@ -252,6 +252,34 @@ def test_parallel_tempering_helpers_shapes(gpu):
     assert torch.allclose(new, temps)
 
 
+def test_per_walker_helpers_follow_the_reference_tests(gpu):
+    # test_strategies.py:395-470: one walker's step body, its n_steps scan, one
+    # chain's ensemble step, and the exchange / adaptation of one chain
+    resources, strat, x0 = _pt_resources(False)
+    mala, logpdf = resources["MALA"], resources["logpdf"]
+    data = {"data": torch.arange(3, dtype=torch.float32, device="cuda")}
+    key = fr.PRNGKey(42)
+    position = x0[0]
+    log_prob = logpdf(position, data)
+    carry, extras = strat._individual_step_body(mala, (key, position, log_prob, logpdf, torch.tensor(1.0), data), None)
+    assert carry[1].shape == (3,) and extras[0].shape == (3,)
+    assert torch.allclose(carry[2], logpdf.tempered_log_pdf(1.0, carry[1], data), atol=1e-5)
+    walkers = torch.cat([x0.cuda()[:, None], resources["tempered_positions"].data], 1)
+    pos, lp, acc = strat._individal_step(mala, key, walkers[0, 0], logpdf, torch.tensor(1.0), data)
+    assert pos.shape == (3,) and acc.shape == (4,)
+    temps = torch.arange(5, dtype=torch.float32) + 1.0
+    p1, l1, a1 = strat._ensemble_step(mala, key, walkers[0], logpdf, temps, data)
+    assert p1.shape == (5, 3) and l1.shape == (5,) and a1.shape == (5, 4)
+    # the one-chain call is the batched call restricted to that chain
+    pb, lb, ab = strat._ensemble_step(mala, key, walkers[:1], logpdf, temps, data)
+    assert torch.equal(p1, pb[0]) and torch.equal(a1, ab[0])
+    y, L, sw = strat._exchange(fr.PRNGKey(7), p1, logpdf, torch.arange(5) * 0.3 + 1, data)
+    assert y.shape == (5, 3) and L.shape == (5,) and sw.shape == (4,)
+    assert torch.allclose(y.sort(0).values.sum(), p1.sort(0).values.sum())
+    new = strat._adapt_temperature(torch.arange(5) * 0.3 + 1, torch.ones(7, 5))
+    assert new.shape == (5,)
+
+
 def test_tempered_sampler_recovers_bimodal_weights(gpu):
     # two well-separated modes: the cold chains alone stay stuck, tempering lets them hop
     d, C_, T = 2, 512, 6
