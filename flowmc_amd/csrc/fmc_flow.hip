@@ -206,6 +206,8 @@ __device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, co
 }
 
 constexpr int PF_DEPTH = 4;  // B-fragment prefetch depth (k-steps); the kernels run 2-3 waves per SIMD
+// the pass kernel fits an 8-deep ring in its three-waves-per-SIMD budget (167 VGPRs)
+constexpr int PASS_PF = 8;
 
 // --------------------------------------------------------------- flow ----
 struct FlowDev {
@@ -832,7 +834,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       for (int c0 = wave; c0 < NBt; c0 += 8) {
         const int ncb = (c0 + 4 < NBt) ? 2 : 1;
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH, true>(a0, sm.la0, W + dm.pk_w1(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+        mfma_panel<PT, MB, 2, PASS_PF, true>(a0, sm.la0, W + dm.pk_w1(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
         for (int j = 0; j < 2; ++j)
           if (j < ncb) hidden_epilogue_tr<PT, MB>(hb1, sm.lh1, bl, c0 + 4 * j, acc[j], lane);
@@ -845,7 +847,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       for (int c0 = wave; c0 < NBt; c0 += 8) {
         const int ncb = (c0 + 4 < NBt) ? 2 : 1;
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH, true>(hb1, sm.lh1, W + dm.pk_w2(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+        mfma_panel<PT, MB, 2, PASS_PF, true>(hb1, sm.lh1, W + dm.pk_w2(l), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
         for (int j = 0; j < 2; ++j)
           if (j < ncb) hidden_epilogue_tr<PT, MB>(hb2, sm.lh2, bl + h1p, c0 + 4 * j, acc[j], lane);
@@ -864,7 +866,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       const int hf = lane >> 5;
       for (int q = wave; q < npair; q += 4) {
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH, true>(hb2, sm.lh2, W3r, KB, 2 * q, 1, 2, 0, KB, 0, acc, lane);
+        mfma_panel<PT, MB, 2, PASS_PF, true>(hb2, sm.lh2, W3r, KB, 2 * q, 1, 2, 0, KB, 0, acc, lane);
         const int k = 2 * q + hf;
         if (k < dtl) {
           const int j = dm.trans(l, k);
@@ -915,7 +917,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       for (int c0 = wave; c0 < NBt; c0 += 8) {
         const int ncb = (c0 + 4 < NBt) ? 2 : 1;
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH>(hb2, sm.lh2, W + dm.pk_w3(l), KB, coff / 32 + c0, 4, ncb, 0, KB, 0, acc,
+        mfma_panel<PT, MB, 2, PASS_PF>(hb2, sm.lh2, W + dm.pk_w3(l), KB, coff / 32 + c0, 4, ncb, 0, KB, 0, acc,
                                         lane);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
