@@ -1060,8 +1060,18 @@ __device__ __forceinline__ void store_fm(typename PT::T* st, int64_t ld, int row
     const int f = rbl * 32 + (lane & 31);
     const int mb = sl * KS + (lane >> 5) * EPL;
     T* dst = st + (((int64_t)(row0 / 32 + rbl) * ns + m0 / KS + sl) * 64 + lane) * EPL;
+    if constexpr (EPL * sizeof(T) == 16) {  // one 16-byte store per lane chunk
+      union {
+        T t[EPL];
+        uint4 u;
+      } pk;
 #pragma unroll
-    for (int e = 0; e < EPL; ++e) dst[e] = (f < nvalid) ? src[(mb + e) * lds + f] : PT::cvt(0.f);
+      for (int e = 0; e < EPL; ++e) pk.t[e] = (f < nvalid) ? src[(mb + e) * lds + f] : PT::cvt(0.f);
+      *reinterpret_cast<uint4*>(dst) = pk.u;
+    } else {
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) dst[e] = (f < nvalid) ? src[(mb + e) * lds + f] : PT::cvt(0.f);
+    }
   }
 }
 
@@ -1234,9 +1244,20 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
           // slot 8t+e of this lane's dim -> gradient-order column 64 wave + 16t + 8hf + e
           T* dst = gpb + m * sm.lg + 64 * wave + 8 * hf;
 #pragma unroll
-          for (int t4 = 0; t4 < 4; ++t4)
+          for (int t4 = 0; t4 < 4; ++t4) {
+            if constexpr (sizeof(T) == 2) {  // 8 BF16 slots: one 16-byte LDS store
+              union {
+                T t[8];
+                uint4 u;
+              } pk;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) dst[16 * t4 + e] = PT::cvt(go[8 * t4 + e]);
+              for (int e = 0; e < 8; ++e) pk.t[e] = PT::cvt(go[8 * t4 + e]);
+              *reinterpret_cast<uint4*>(dst + 16 * t4) = pk.u;
+            } else {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) dst[16 * t4 + e] = PT::cvt(go[8 * t4 + e]);
+            }
+          }
         }
       }
       __syncthreads();
