@@ -73,11 +73,16 @@ class Buffer(Resource):
         return np.histogram(self.data.detach().float().cpu().numpy().flatten(), bins=n_bins)
 
     def save_resource(self, path: str):
+        """buffers.py:52-57: ``np.savez(path + name, name=name, data=data)`` (an
+        ``.npz`` with a 0-d string array and the data array; no pickle)."""
         np.savez(path + self.name, name=self.name, data=self.data.detach().cpu().numpy())
 
     def load_resource(self, path: str) -> "Buffer":
-        f = np.load(path)
+        """buffers.py:59-64: a new Buffer named and shaped by the file, with
+        the default cursor (0) and cursor_dim (0), as the reference's
+        ``Buffer(data["name"], buffer.shape)``. Pickled files are refused."""
+        f = np.load(path, allow_pickle=False)
         arr = f["data"]
-        result = Buffer(str(f["name"]), arr.shape, self.cursor_dim, device=self.data.device)
+        result = Buffer(str(f["name"]), arr.shape, device=self.data.device)
         result.data = torch.as_tensor(arr, dtype=self.data.dtype, device=self.data.device)
         return result

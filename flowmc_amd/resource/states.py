@@ -31,7 +31,10 @@ class State(Resource):
         print(f"State: {self.name} with shape {len(self.data)} and data {self.data}")
 
     def save_resource(self, path: str):
-        # the dict is stored as parallel string arrays so it loads without pickle
+        # states.py:53-58 stores the dict itself, which numpy can only keep as a
+        # pickled object array (and states.py:60-63 then cannot load it without
+        # allow_pickle); here the dict is stored as parallel string arrays so it
+        # loads without pickle
         keys = np.array(list(self.data.keys()), dtype=str)
         vals = np.array([repr(v) for v in self.data.values()], dtype=str)
         np.savez(path + self.name, name=self.name, keys=keys, values=vals)
@@ -39,6 +42,6 @@ class State(Resource):
     def load_resource(self, path: str) -> "State":
         import ast
 
-        f = np.load(path)
+        f = np.load(path, allow_pickle=False)
         data = {str(k): ast.literal_eval(str(v)) for k, v in zip(f["keys"], f["values"])}
         return State(data, str(f["name"]))
