@@ -103,11 +103,19 @@ class BatchedHMC:
         nv, block, lpcs = layout
         st = _native.stream_ptr()
         maxd = max(self.dims)
+        # without injected noise the slot / work-item tables are the same for
+        # every launch of this call (the step offset is a kernel argument):
+        # build and upload them once
+        tables = self._tables(layout, key, fused_steps, outputs, None, None) if noise is None and log_u is None \
+            else None
         for s0 in range(0, n_steps, fused_steps):
             ns = min(fused_steps, n_steps - s0)
-            nz = [n[s0:s0 + ns] for n in noise] if noise is not None else None
-            lu = [u[s0:s0 + ns] for u in log_u] if log_u is not None else None
-            slot_dev, it_dev, n_items = self._tables(layout, key, ns, outputs, nz, lu)
+            if tables is None:
+                nz = [n[s0:s0 + ns] for n in noise] if noise is not None else None
+                lu = [u[s0:s0 + ns] for u in log_u] if log_u is not None else None
+                slot_dev, it_dev, n_items = self._tables(layout, key, ns, outputs, nz, lu)
+            else:
+                slot_dev, it_dev, n_items = tables
             _native.call("fmc_hmc_run_batched", st, slot_dev.data_ptr(), len(self.targets), it_dev.data_ptr(), n_items,
                          maxd, ns, s0, self.step_size, self.n_leapfrog, thinning, nv, block)
             self._last_tables = (slot_dev, it_dev)  # keep alive until the stream passes them
