@@ -702,6 +702,14 @@ def cpu_baseline_pt(args, tgt, T, S, dt):
 
 # --------------------------------------------------------------- C4 -------
 def bench_hmc_funnels(args):
+    """Batched uneven-dimension HMC: 32 Neal's funnels, d log-uniform in
+    [8, 256], 256 chains each, 10 leapfrog steps of 0.05, 16 steps per launch.
+
+    The step is FP32-vector bound, not HBM bound: per chain-step it computes
+    (L+2) gradients (3d each), (L+1) drift+kick pairs (4d) and the kinetic
+    terms (6d) = 86d FLOPs for L = 10, against 4d+8 bytes of row stores. The
+    roofline is that FLOP count over the HIP-event time of the launches on
+    the launching stream; the HBM figure of the same launches is in extra."""
     import torch
 
     from flowmc_amd import random as fr
@@ -709,36 +717,78 @@ def bench_hmc_funnels(args):
     from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
 
     dims = TG.funnel_dims(seed=0, n_targets=32, lo=8, hi=256)
-    C_ = args.chains or 256
-    bh = BatchedHMC([TG.Funnel(d) for d in dims], C_, step_size=0.05, n_leapfrog=10)
+    C_, L, K = args.chains or 256, 10, 16
+    bh = BatchedHMC([TG.Funnel(d) for d in dims], C_, step_size=0.05, n_leapfrog=L)
     bh.init_positions(fr.PRNGKey(0))
     T = args.steps
     outs = [(torch.empty((C_, T, d), dtype=torch.float32, device="cuda"),
              torch.empty((C_, T), dtype=torch.float32, device="cuda"),
              torch.empty((C_, T), dtype=torch.float32, device="cuda"), 0) for d in dims]
     layout = bh.tune(fr.PRNGKey(9))
-    bh.run(fr.PRNGKey(1), min(T, 16), outputs=outs)
+    bh.run(fr.PRNGKey(1), min(T, K), outputs=outs)
     torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    bh.run(fr.PRNGKey(2), T, outputs=outs, fused_steps=16)
+    ev0.record(stream)
+    bh.run(fr.PRNGKey(2), T, outputs=outs, fused_steps=K)
+    ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    total_steps = C_ * len(dims) * T
-    bytes_total = sum(C_ * T * (4 * d + 8) + C_ * (T // 16 + 1) * (8 * d + 8) for d in dims)
-    achieved = bytes_total / wall / 1e9
+    gpu_s = ev0.elapsed_time(ev1) * 1e-3
+    n_launch = (T + K - 1) // K
+    sum_d = sum(dims)
+    flops_total = (L + 2) * 3 * sum_d + 4 * (L + 1) * sum_d + 6 * sum_d
+    flops_total *= C_ * T
+    bytes_total = sum(C_ * T * (4 * d + 8) + C_ * n_launch * (8 * d + 8) for d in dims)
+    achieved = flops_total / gpu_s / 1e12
     v_all = torch.cat([o[0][..., 0].reshape(-1) for o in outs])
     out = {
-        "value": total_steps / wall,
+        "value": C_ * len(dims) * T / wall,
         "ms_per_step": wall * 1e3 / T,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "hmc_batched_kernel"},
-        "extra": {"acceptance": float(torch.cat([o[2].reshape(-1) for o in outs]).mean()),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "hmc_batched_kernel (FP32 vector ALU)", "kernel_ms": gpu_s * 1e3 / n_launch,
+                     "flops_per_launch": flops_total / n_launch},
+        "extra": {"hbm_gbs": bytes_total / gpu_s / 1e9, "hbm_frac": bytes_total / gpu_s / 1e9 / HBM_PEAK_GBS,
+                  "acceptance": float(torch.cat([o[2].reshape(-1) for o in outs]).mean()),
                   "v_mean": float(v_all.mean()), "v_var": float(v_all.var()), "layout": [layout[0], layout[1]],
                   "dims": dims},
-        "config": {"workload": "hmc_funnels", "targets": 32, "chains_per_target": C_, "n_leapfrog": 10,
-                   "step_size": 0.05, "fused_steps_per_launch": 16},
+        "config": {"workload": "hmc_funnels", "targets": 32, "chains_per_target": C_, "n_leapfrog": L,
+                   "step_size": 0.05, "fused_steps_per_launch": K},
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_hmc_funnels(args, dims, C_, L)
     return out
+
+
+def cpu_baseline_hmc_funnels(args, dims, C_, L):
+    """Oracle HMC steps (autograd gradients, FP32 torch on the host cores) on
+    every funnel of the batch in turn, all chains of a target at once, for
+    about --cpu-seconds; the rate counts chain-steps over all targets."""
+    import torch
+
+    from oracle import chains as OC
+    from oracle import targets as OT
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    g = torch.Generator().manual_seed(0)
+    xs = [torch.randn(C_, d, generator=g) for d in dims]
+    lps = [OT.funnel(x) for x in xs]
+    ones = [torch.ones(d) for d in dims]
+    steps = sweeps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        for i, d in enumerate(dims):
+            xs[i], lps[i], _, _ = OC.hmc_step(xs[i], lps[i], OT.funnel, 0.05, L, ones[i],
+                                              torch.randn(C_, d, generator=g),
+                                              torch.log(torch.rand(C_, generator=g)))
+            steps += C_
+        sweeps += 1
+    el = time.perf_counter() - t0
+    return {"value": steps / el, "unit": "chain-steps/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"{sweeps} sweeps of one HMC step over all {len(dims)} funnels x {C_} chains (L={L}), "
+                      f"FP32 torch-CPU oracle restatement with autograd gradients ({cpu_model()})"}
 
 
 WORKLOADS = {

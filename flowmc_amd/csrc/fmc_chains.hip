@@ -131,7 +131,7 @@ __device__ __forceinline__ float temper(const ChainIO& io, float beta, float lp,
 // previous step instead of being re-evaluated: MALA.py:59 recomputes it at the
 // same point, which for a deterministic target is the same value.
 template <int NV>
-__device__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
+__device__ __forceinline__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
                            int r, int lpc, int gbase) {
   const int d = t.d;
   float4 x[NV], g[NV];
@@ -189,7 +189,7 @@ __device__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg&
 // once (take_steps.py:177) and carried; the target's gradient output is dead
 // and removed by the compiler.
 template <int NV>
-__device__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
+__device__ __forceinline__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
                           int r, int lpc, int gbase) {
   const int d = t.d;
   float4 x[NV], g[NV];
@@ -224,13 +224,17 @@ __device__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
 // coefficient table of HMC.py:47-50: row 0 = (0, 1/2), rows 1..L = (1, 1),
 // row L+1 = (1, 1/2): L+1 drifts x += eps*c0*metric*p and L+2 kicks
 // p += eps*c1*grad logp(x). H0 - H1 is accumulated in FP64.
-template <int NV>
-__device__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
+//
+// UNIT_M: identity metric (io.metric == nullptr). Padding elements of p, x and
+// the gradient are 0 throughout, so that form needs no per-element index
+// test and no metric multiply.
+template <int NV, bool UNIT_M>
+__device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
                           int r, int lpc, int gbase) {
   const int d = t.d;
   float4 x[NV], g[NV], m[NV];
   load_row<NV>(io.x + c * d, d, r, lpc, x);
-  if (io.metric) {
+  if (!UNIT_M) {
     load_row<NV>(io.metric, d, r, lpc, m);
   } else {
 #pragma unroll
@@ -249,11 +253,16 @@ __device__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
     for (int v = 0; v < NV; ++v)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        int j = dim_of(v, e, r, lpc);
-        float mj = f4c(m[v], e);
-        float pj = (j < d) ? f4c(p[v], e) / sqrtf(mj) : 0.f;
-        f4(p[v], e) = pj;
-        k0 += 0.5 * (double)mj * (double)pj * (double)pj;
+        if (UNIT_M) {
+          const double pj = (double)f4c(p[v], e);
+          k0 += 0.5 * pj * pj;
+        } else {
+          int j = dim_of(v, e, r, lpc);
+          float mj = f4c(m[v], e);
+          float pj = (j < d) ? f4c(p[v], e) / sqrtf(mj) : 0.f;
+          f4(p[v], e) = pj;
+          k0 += 0.5 * (double)mj * (double)pj * (double)pj;
+        }
       }
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -272,7 +281,7 @@ __device__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
 #pragma unroll
       for (int v = 0; v < NV; ++v)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) f4(xl[v], e) += eps * (f4c(m[v], e) * f4c(p[v], e));
+        for (int e = 0; e < 4; ++e) f4(xl[v], e) += eps * (UNIT_M ? f4c(p[v], e) : f4c(m[v], e) * f4c(p[v], e));
       lpl = temper<NV>(io, beta, target_vg<NV>(t, xl, gl, r, lpc, gbase), gl);
 #pragma unroll
       for (int v = 0; v < NV; ++v)
@@ -284,8 +293,13 @@ __device__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& 
     for (int v = 0; v < NV; ++v)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        int j = dim_of(v, e, r, lpc);
-        if (j < d) k1 += 0.5 * (double)f4c(m[v], e) * (double)f4c(p[v], e) * (double)f4c(p[v], e);
+        if (UNIT_M) {
+          const double pj = (double)f4c(p[v], e);
+          k1 += 0.5 * pj * pj;
+        } else {
+          int j = dim_of(v, e, r, lpc);
+          if (j < d) k1 += 0.5 * (double)f4c(m[v], e) * (double)f4c(p[v], e) * (double)f4c(p[v], e);
+        }
       }
     k1 = group_sum(k1, lpc);
     double h0 = -(double)lp + k0;
@@ -335,7 +349,7 @@ __global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, 
   else if (KIND == KK_GRW)
     grw_group<NV>(t, io, cfg, c, r, lpc, gbase);
   else
-    hmc_group<NV>(t, io, cfg, c, r, lpc, gbase);
+    io.metric ? hmc_group<NV, false>(t, io, cfg, c, r, lpc, gbase) : hmc_group<NV, true>(t, io, cfg, c, r, lpc, gbase);
 }
 
 // Many HMC problems of uneven dimension in one launch: one work item per block.
@@ -355,7 +369,10 @@ __global__ void __launch_bounds__(1024) hmc_batched_kernel(const fmc_hmc_slot* _
   TargetDev t{sl.target.kind, sl.target.n_dim, sl.target.n_modes, sl.target.params};
   ChainIO io{sl.x,        sl.logp,     sl.metric,  sl.noise,  sl.log_u,  sl.out_pos, sl.out_logp,
              sl.out_acc,  sl.n_chains, sl.buf_len, sl.cursor, sl.key0,   sl.key1};
-  hmc_group<NV>(t, io, cfg, c, r, lpc, gbase);
+  if (io.metric)
+    hmc_group<NV, false>(t, io, cfg, c, r, lpc, gbase);
+  else
+    hmc_group<NV, true>(t, io, cfg, c, r, lpc, gbase);
 }
 
 // ------------------------------------------------------- target eval ------

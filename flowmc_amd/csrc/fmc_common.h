@@ -104,20 +104,112 @@ __device__ __forceinline__ float philox_log_uniform(uint32_t k0, uint32_t k1, ui
 }
 
 // ------------------------------------------------------- lane-group sums ---
-// A chain owns `lpc` consecutive lanes (power of two, aligned); a butterfly
-// over xor offsets < lpc stays inside the group and leaves the bitwise same
-// sum in every lane of the group.
+// A chain owns `lpc` consecutive lanes (power of two, aligned, the same for
+// the whole wave). The collectives below move data on the VALU, not through
+// the LDS crossbar (ds_bpermute): DPP quad permutations pair lanes i^1, i^2;
+// row half-mirror / mirror pair lane i with the other quad of its 8 / the
+// other 8 of its 16-lane row; v_permlane16_swap / v_permlane32_swap exchange
+// odd and even 16-lane rows / the two 32-lane halves. Every step combines
+// two lanes of the group, both lanes compute a + b with the same operands
+// (addition commutes), so the whole group ends with the bitwise same value.
+template <int CTRL, int BANK = 0xF>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, BANK, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int2 w = *reinterpret_cast<const int2*>(&v);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp(w.x, w.x, CTRL, 0xF, 0xF, false);
+  r.y = __builtin_amdgcn_update_dpp(w.y, w.y, CTRL, 0xF, 0xF, false);
+  return *reinterpret_cast<const double*>(&r);
+}
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8
+constexpr int DPP_MIRROR = 0x140;      // lane i <-> 15-i within 16
+constexpr int DPP_QUAD_BCAST0 = 0x00;  // quad_perm [0,0,0,0]
+constexpr int DPP_PAIR_BCAST0 = 0xA0;  // quad_perm [0,0,2,2]
+constexpr int DPP_ROW_SHR4 = 0x114;
+constexpr int DPP_ROW_BCAST0 = 0x150;  // row_newbcast:0
+
+// (a, b) after swapping the odd 16-lane rows of a with the even rows of b,
+// called with a = b = v: first = rows (0,0,2,2), second = rows (1,1,3,3).
+__device__ __forceinline__ void swap16(float v, float& lo, float& hi) {
+  auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  lo = __uint_as_float(r[0]);
+  hi = __uint_as_float(r[1]);
+}
+// The same over 32-lane halves: first = lanes 0-31 twice, second = 32-63 twice.
+__device__ __forceinline__ void swap32(float v, float& lo, float& hi) {
+  auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  lo = __uint_as_float(r[0]);
+  hi = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap16(double v, double& lo, double& hi) {
+  float a, b, c, e;
+  const float2 w = *reinterpret_cast<const float2*>(&v);
+  swap16(w.x, a, b);
+  swap16(w.y, c, e);
+  float2 l{a, c}, h{b, e};
+  lo = *reinterpret_cast<const double*>(&l);
+  hi = *reinterpret_cast<const double*>(&h);
+}
+__device__ __forceinline__ void swap32(double v, double& lo, double& hi) {
+  float a, b, c, e;
+  const float2 w = *reinterpret_cast<const float2*>(&v);
+  swap32(w.x, a, b);
+  swap32(w.y, c, e);
+  float2 l{a, c}, h{b, e};
+  lo = *reinterpret_cast<const double*>(&l);
+  hi = *reinterpret_cast<const double*>(&h);
+}
+
 __device__ __forceinline__ float group_sum(float v, int lpc) {
-  for (int o = lpc >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lpc >= 2) v += dpp_f<DPP_XOR1>(v, v);
+  if (lpc >= 4) v += dpp_f<DPP_XOR2>(v, v);
+  if (lpc >= 8) v += dpp_f<DPP_HALF_MIRROR>(v, v);
+  if (lpc >= 16) v += dpp_f<DPP_MIRROR>(v, v);
+  float lo, hi;
+  if (lpc >= 32) { swap16(v, lo, hi); v = lo + hi; }
+  if (lpc >= 64) { swap32(v, lo, hi); v = lo + hi; }
   return v;
 }
 __device__ __forceinline__ double group_sum(double v, int lpc) {
-  for (int o = lpc >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lpc >= 2) v += dpp_d<DPP_XOR1>(v);
+  if (lpc >= 4) v += dpp_d<DPP_XOR2>(v);
+  if (lpc >= 8) v += dpp_d<DPP_HALF_MIRROR>(v);
+  if (lpc >= 16) v += dpp_d<DPP_MIRROR>(v);
+  double lo, hi;
+  if (lpc >= 32) { swap16(v, lo, hi); v = lo + hi; }
+  if (lpc >= 64) { swap32(v, lo, hi); v = lo + hi; }
   return v;
 }
 __device__ __forceinline__ float group_max(float v, int lpc) {
-  for (int o = lpc >> 1; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if (lpc >= 2) v = fmaxf(v, dpp_f<DPP_XOR1>(v, v));
+  if (lpc >= 4) v = fmaxf(v, dpp_f<DPP_XOR2>(v, v));
+  if (lpc >= 8) v = fmaxf(v, dpp_f<DPP_HALF_MIRROR>(v, v));
+  if (lpc >= 16) v = fmaxf(v, dpp_f<DPP_MIRROR>(v, v));
+  float lo, hi;
+  if (lpc >= 32) { swap16(v, lo, hi); v = fmaxf(lo, hi); }
+  if (lpc >= 64) { swap32(v, lo, hi); v = fmaxf(lo, hi); }
   return v;
+}
+// The value of the group's first lane, in every lane of the group.
+__device__ __forceinline__ float group_bcast0(float v, int lpc) {
+  if (lpc == 1) return v;
+  if (lpc == 2) return dpp_f<DPP_PAIR_BCAST0>(v, v);
+  if (lpc == 4) return dpp_f<DPP_QUAD_BCAST0>(v, v);
+  if (lpc == 8) {
+    const float q = dpp_f<DPP_QUAD_BCAST0>(v, v);
+    return dpp_f<DPP_ROW_SHR4, 0xA>(q, q);  // banks 1 and 3 take lane i-4
+  }
+  if (lpc == 64) return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));  // int builtin: bitcast
+  const float rb = dpp_f<DPP_ROW_BCAST0>(v, v);
+  if (lpc == 16) return rb;
+  float lo, hi;
+  swap16(rb, lo, hi);
+  return lo;
 }
 
 __device__ __forceinline__ float& f4(float4& v, int e) { return (&v.x)[e]; }

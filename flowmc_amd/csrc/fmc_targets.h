@@ -102,7 +102,7 @@ __device__ __forceinline__ float tgt_rosenbrock(const TargetDev& t, const float4
   for (int v = 0; v < NV; ++v) {
     // x_{j+1} for element 3 and x_{j-1} for element 0 live in neighbouring lanes.
     float nx_same = __shfl(x[v].x, gbase + ((r + 1) & (lpc - 1)), 64);
-    float nx_next = __shfl(x[(v + 1 < NV) ? v + 1 : v].x, gbase, 64);
+    float nx_next = group_bcast0(x[(v + 1 < NV) ? v + 1 : v].x, lpc);
     float pw_same = __shfl(x[v].w, gbase + ((r - 1) & (lpc - 1)), 64);
     float pw_prev = __shfl(x[(v > 0) ? v - 1 : 0].w, gbase + lpc - 1, 64);
     float nxt3 = (r < lpc - 1) ? nx_same : nx_next;
@@ -128,35 +128,30 @@ __device__ __forceinline__ float tgt_rosenbrock(const TargetDev& t, const float4
 }
 
 // Neal's funnel: x_0 = v ~ N(0, 3^2), x_i | v ~ N(0, e^v) (unnormalised).
+// Padding elements (j >= d) hold 0 on entry, so they add nothing to the sum
+// and get a zero gradient without a per-element index test; only element 0
+// of the group's first lane (v itself) is special. exp(-v) is capped at
+// FLT_MAX so that 0 * exp(-v) stays 0 on padding when e^-v overflows.
 template <int NV>
 __device__ __forceinline__ float tgt_funnel(const TargetDev& t, const float4 (&x)[NV],
                                             float4 (&g)[NV], int r, int lpc, int gbase) {
   const int d = t.d;
-  float v0 = __shfl(x[0].x, gbase, 64);
-  float s = 0.f;
+  const float v0 = group_bcast0(x[0].x, lpc);
+  const float x00 = (r == 0) ? 0.f : x[0].x;
+  float s = x00 * x00;
 #pragma unroll
   for (int v = 0; v < NV; ++v)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      int j = dim_of(v, e, r, lpc);
-      float xi = f4c(x[v], e);
-      if (j >= 1 && j < d) s += xi * xi;
-    }
+    for (int e = (v == 0) ? 1 : 0; e < 4; ++e) s += f4c(x[v], e) * f4c(x[v], e);
   s = group_sum(s, lpc);
-  float em = expf(-v0);
-  float nm1 = (float)(d - 1);
+  const float em = fminf(expf(-v0), 3.402823466e38f);
+  const float nm1 = (float)(d - 1);
 #pragma unroll
   for (int v = 0; v < NV; ++v)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      int j = dim_of(v, e, r, lpc);
-      float gj = 0.f;
-      if (j == 0)
-        gj = -v0 * (1.f / 9.f) - 0.5f * nm1 + 0.5f * em * s;
-      else if (j < d)
-        gj = -f4c(x[v], e) * em;
-      f4(g[v], e) = gj;
-    }
+    for (int e = 0; e < 4; ++e) f4(g[v], e) = -f4c(x[v], e) * em;
+  const float g0 = -v0 * (1.f / 9.f) - 0.5f * nm1 + 0.5f * em * s;
+  if (r == 0) g[0].x = g0;
   return -v0 * v0 * (1.f / 18.f) - 0.5f * nm1 * v0 - 0.5f * em * s;
 }
 
@@ -236,8 +231,8 @@ __device__ __forceinline__ float tgt_dual_moon(const TargetDev& t, const float4 
     }
   n2 = group_sum(n2, lpc);
   float n = sqrtf(n2);
-  float x0 = __shfl(x[0].x, gbase, 64);
-  float x1 = __shfl(x[0].y, gbase, 64);
+  float x0 = group_bcast0(x[0].x, lpc);
+  float x1 = group_bcast0(x[0].y, lpc);
   // logsumexp pieces
   float a0 = -0.5f * ((x0 - 3.f) / 0.8f) * ((x0 - 3.f) / 0.8f);
   float b0 = -0.5f * ((x0 + 3.f) / 0.8f) * ((x0 + 3.f) / 0.8f);

@@ -50,7 +50,7 @@ class BatchedHMC:
 
     # ------------------------------------------------------------- tables --
     def default_layout(self, nv: int = 1, block: int = 256, spread: int = 1):
-        lpcs = [min(64, _pow2ceil(max(1, -(-d // (4 * nv)))) * spread) for d in self.dims]
+        lpcs = [min(64, spread * _pow2ceil(max(1, -(-d // (4 * nv))))) for d in self.dims]
         for d, l in zip(self.dims, lpcs):
             if 4 * nv * l < d:
                 raise ValueError(f"vec_per_lane={nv} cannot hold d={d} in 64 lanes")
@@ -58,6 +58,10 @@ class BatchedHMC:
 
     def _tables(self, layout, key, n_steps_noise, outputs, noise, log_u):
         nv, block, lpcs = layout
+        for d, l in zip(self.dims, lpcs):
+            # a chain's lanes must sit in one wavefront: power of two <= 64
+            if l < 1 or l > 64 or l & (l - 1) or block % l or 4 * nv * l < d:
+                raise ValueError(f"lanes_per_chain={l} (vec_per_lane={nv}, block={block}) cannot hold d={d}")
         slots = (_native.fmc_hmc_slot * len(self.targets))()
         items = []
         self._keep = []
@@ -129,9 +133,11 @@ class BatchedHMC:
             for block in (128, 256, 512):
                 for spread in (1, 2):
                     try:
-                        cands.append(self.default_layout(nv, block, spread))
+                        lay = self.default_layout(nv, block, spread)
                     except ValueError:
-                        pass
+                        continue
+                    if lay not in cands:
+                        cands.append(lay)
 
         def go(layout):
             for x, s in zip(self.x, saved):

@@ -108,7 +108,12 @@ def test_target_value_and_grad(gpu, name, d, layout):
     scale_v = 1 + v_ref.abs()
     assert ((lp.cpu().double() - v_ref).abs() <= 2e-5 * scale_v).all()
     scale_g = 1 + g_ref.abs().amax(-1, keepdim=True)
-    assert ((gr.cpu().double() - g_ref).abs() <= 2e-5 * scale_g).all()
+    # The mixture gradient sum_k w_k g_k weighs per-mode gradients by softmax(q): an FP32
+    # error u*|q_k| in the mode log-densities (|q_k| ~ d/2 + |c_k| ~ 40 at d = 64) moves the
+    # weights by that much relative, and the gradient by that times the spread of the g_k
+    # (~10 here): ~1e-4 relative to max|g| where two modes compete. Other targets: 2e-5.
+    tol_g = 1e-4 if name == "gmm" else 2e-5
+    assert ((gr.cpu().double() - g_ref).abs() <= tol_g * scale_g).all()
 
 
 def _compare_traj(kind, name, d, C_, S, dt, thinning=1, n_leapfrog=5, seed=0):
