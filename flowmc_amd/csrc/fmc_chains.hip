@@ -130,14 +130,14 @@ __device__ __forceinline__ float temper(const ChainIO& io, float beta, float lp,
 // MALA.py:26-89). The gradient at the current point is carried from the
 // previous step instead of being re-evaluated: MALA.py:59 recomputes it at the
 // same point, which for a deterministic target is the same value.
-template <int NV>
+template <int NV, int TK>
 __device__ __forceinline__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
                            int r, int lpc, int gbase) {
   const int d = t.d;
   float4 x[NV], g[NV];
   load_row<NV>(io.x + c * d, d, r, lpc, x);
   const float beta = row_beta(io, c);
-  float lp = temper<NV>(io, beta, target_vg<NV>(t, x, g, r, lpc, gbase), g);
+  float lp = temper<NV>(io, beta, target_vg<NV, TK>(t, x, g, r, lpc, gbase), g);
   const float dt = cfg.dt;
   const float h = 0.5f * dt * dt;
   const float inv2dt2 = 1.f / (2.f * dt * dt);
@@ -154,7 +154,7 @@ __device__ __forceinline__ void mala_group(const TargetDev& t, const ChainIO& io
         f4(prop[v], e) = f4c(x[v], e) + h * f4c(g[v], e) + n;
         s1 += n * n;
       }
-    float lp1 = temper<NV>(io, beta, target_vg<NV>(t, prop, g1, r, lpc, gbase), g1);
+    float lp1 = temper<NV>(io, beta, target_vg<NV, TK>(t, prop, g1, r, lpc, gbase), g1);
     float s2 = 0.f;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
@@ -188,14 +188,14 @@ __device__ __forceinline__ void mala_group(const TargetDev& t, const ChainIO& io
 // log u < logp(x') - logp(x). The log-density of the entry state is evaluated
 // once (take_steps.py:177) and carried; the target's gradient output is dead
 // and removed by the compiler.
-template <int NV>
+template <int NV, int TK>
 __device__ __forceinline__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
                           int r, int lpc, int gbase) {
   const int d = t.d;
   float4 x[NV], g[NV];
   load_row<NV>(io.x + c * d, d, r, lpc, x);
   const float beta = row_beta(io, c);
-  float lp = temper<NV>(io, beta, target_vg<NV>(t, x, g, r, lpc, gbase), g);
+  float lp = temper<NV>(io, beta, target_vg<NV, TK>(t, x, g, r, lpc, gbase), g);
   const float dt = cfg.dt;
   for (int s = 0; s < cfg.n_steps; ++s) {
     const int tg = cfg.step0 + s;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void grw_group(const TargetDev& t, const ChainIO& io,
     for (int v = 0; v < NV; ++v)
 #pragma unroll
       for (int e = 0; e < 4; ++e) f4(prop[v], e) = f4c(x[v], e) + dt * f4c(xi[v], e);
-    const float lp1 = temper<NV>(io, beta, target_vg<NV>(t, prop, g, r, lpc, gbase), g);
+    const float lp1 = temper<NV>(io, beta, target_vg<NV, TK>(t, prop, g, r, lpc, gbase), g);
     const float lu = draw_log_u(io, c, s, tg, RNG_GRW_UNIFORM);
     const bool acc = lu < lp1 - lp;
     if (acc) {
@@ -228,7 +228,7 @@ __device__ __forceinline__ void grw_group(const TargetDev& t, const ChainIO& io,
 // UNIT_M: identity metric (io.metric == nullptr). Padding elements of p, x and
 // the gradient are 0 throughout, so that form needs no per-element index
 // test and no metric multiply.
-template <int NV, bool UNIT_M>
+template <int NV, bool UNIT_M, int TK>
 __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
                           int r, int lpc, int gbase) {
   const int d = t.d;
@@ -241,7 +241,7 @@ __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io,
     for (int v = 0; v < NV; ++v) m[v] = make_float4(1.f, 1.f, 1.f, 1.f);
   }
   const float beta = row_beta(io, c);
-  float lp = temper<NV>(io, beta, target_vg<NV>(t, x, g, r, lpc, gbase), g);
+  float lp = temper<NV>(io, beta, target_vg<NV, TK>(t, x, g, r, lpc, gbase), g);
   const float eps = cfg.dt;
   const int L = cfg.n_leapfrog;
   for (int s = 0; s < cfg.n_steps; ++s) {
@@ -282,7 +282,7 @@ __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io,
       for (int v = 0; v < NV; ++v)
 #pragma unroll
         for (int e = 0; e < 4; ++e) f4(xl[v], e) += eps * (UNIT_M ? f4c(p[v], e) : f4c(m[v], e) * f4c(p[v], e));
-      lpl = temper<NV>(io, beta, target_vg<NV>(t, xl, gl, r, lpc, gbase), gl);
+      lpl = temper<NV>(io, beta, target_vg<NV, TK>(t, xl, gl, r, lpc, gbase), gl);
 #pragma unroll
       for (int v = 0; v < NV; ++v)
 #pragma unroll
@@ -334,7 +334,9 @@ __device__ __forceinline__ TargetDev stage_target(const fmc_target& tg, float* s
 
 enum KernelKind { KK_MALA = 0, KK_HMC = 1, KK_GRW = 2 };
 
-template <int NV, int KIND>
+// One kernel per (lane layout NV, step kind, target kind TK): the target's
+// value and gradient are inlined without a run-time dispatch.
+template <int NV, int KIND, int TK>
 __global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, StepCfg cfg,
                                                      int lpc, int n_lds) {
   extern __shared__ float smem[];
@@ -345,11 +347,13 @@ __global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, 
   const int64_t c = (int64_t)blockIdx.x * (blockDim.x / lpc) + threadIdx.x / lpc;
   if (c >= io.C) return;  // whole group leaves together
   if (KIND == KK_MALA)
-    mala_group<NV>(t, io, cfg, c, r, lpc, gbase);
+    mala_group<NV, TK>(t, io, cfg, c, r, lpc, gbase);
   else if (KIND == KK_GRW)
-    grw_group<NV>(t, io, cfg, c, r, lpc, gbase);
+    grw_group<NV, TK>(t, io, cfg, c, r, lpc, gbase);
+  else if (io.metric)
+    hmc_group<NV, false, TK>(t, io, cfg, c, r, lpc, gbase);
   else
-    io.metric ? hmc_group<NV, false>(t, io, cfg, c, r, lpc, gbase) : hmc_group<NV, true>(t, io, cfg, c, r, lpc, gbase);
+    hmc_group<NV, true, TK>(t, io, cfg, c, r, lpc, gbase);
 }
 
 // Many HMC problems of uneven dimension in one launch: one work item per block.
@@ -370,9 +374,9 @@ __global__ void __launch_bounds__(1024) hmc_batched_kernel(const fmc_hmc_slot* _
   ChainIO io{sl.x,        sl.logp,     sl.metric,  sl.noise,  sl.log_u,  sl.out_pos, sl.out_logp,
              sl.out_acc,  sl.n_chains, sl.buf_len, sl.cursor, sl.key0,   sl.key1};
   if (io.metric)
-    hmc_group<NV, false>(t, io, cfg, c, r, lpc, gbase);
+    hmc_group<NV, false, -1>(t, io, cfg, c, r, lpc, gbase);
   else
-    hmc_group<NV, true>(t, io, cfg, c, r, lpc, gbase);
+    hmc_group<NV, true, -1>(t, io, cfg, c, r, lpc, gbase);
 }
 
 // ------------------------------------------------------- target eval ------
@@ -843,6 +847,17 @@ static int lds_floats(const fmc_target* t) {
   return (n > 0 && n * 4 <= 48 * 1024) ? (int)n : 0;
 }
 
+template <int KIND, int TK>
+static void launch_chain_nv(hipStream_t st, const fmc_target* t, const ChainIO& io, const StepCfg& cfg,
+                            const Layout& L, dim3 g, dim3 b, size_t shm, int n_lds) {
+  switch (L.nv) {
+    case 1: hipLaunchKernelGGL((chain_kernel<1, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+    case 2: hipLaunchKernelGGL((chain_kernel<2, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+    case 4: hipLaunchKernelGGL((chain_kernel<4, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+    default: hipLaunchKernelGGL((chain_kernel<8, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+  }
+}
+
 template <int KIND>
 static int launch_chain(hipStream_t st, const fmc_target* t, const ChainIO& io, const StepCfg& cfg,
                         const Layout& L) {
@@ -852,11 +867,13 @@ static int launch_chain(hipStream_t st, const fmc_target* t, const ChainIO& io, 
   const int n_lds = lds_floats(t);
   const size_t shm = (size_t)n_lds * sizeof(float);
   dim3 g((unsigned)grid), b((unsigned)L.block);
-  switch (L.nv) {
-    case 1: hipLaunchKernelGGL((chain_kernel<1, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
-    case 2: hipLaunchKernelGGL((chain_kernel<2, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
-    case 4: hipLaunchKernelGGL((chain_kernel<4, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
-    default: hipLaunchKernelGGL((chain_kernel<8, KIND>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
+  switch (t->kind) {  // check_target has accepted the kind
+    case FMC_TARGET_GAUSS_DIAG: launch_chain_nv<KIND, FMC_TARGET_GAUSS_DIAG>(st, t, io, cfg, L, g, b, shm, n_lds); break;
+    case FMC_TARGET_GAUSS_DENSE: launch_chain_nv<KIND, FMC_TARGET_GAUSS_DENSE>(st, t, io, cfg, L, g, b, shm, n_lds); break;
+    case FMC_TARGET_ROSENBROCK: launch_chain_nv<KIND, FMC_TARGET_ROSENBROCK>(st, t, io, cfg, L, g, b, shm, n_lds); break;
+    case FMC_TARGET_FUNNEL: launch_chain_nv<KIND, FMC_TARGET_FUNNEL>(st, t, io, cfg, L, g, b, shm, n_lds); break;
+    case FMC_TARGET_GMM_DIAG: launch_chain_nv<KIND, FMC_TARGET_GMM_DIAG>(st, t, io, cfg, L, g, b, shm, n_lds); break;
+    default: launch_chain_nv<KIND, FMC_TARGET_DUAL_MOON>(st, t, io, cfg, L, g, b, shm, n_lds); break;
   }
   return check_launch(KIND == KK_MALA ? "mala chain kernel" : KIND == KK_GRW ? "grw chain kernel" : "hmc chain kernel");
 }

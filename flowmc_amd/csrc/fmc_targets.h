@@ -264,10 +264,12 @@ __device__ __forceinline__ float tgt_dual_moon(const TargetDev& t, const float4 
   return -(0.5f * t1 * t1 - l0 - l1);
 }
 
-template <int NV>
+// TK >= 0: the kind is fixed at compile time (one target per kernel);
+// TK = -1: dispatch on t.kind at run time.
+template <int NV, int TK = -1>
 __device__ __forceinline__ float target_vg(const TargetDev& t, const float4 (&x)[NV],
                                            float4 (&g)[NV], int r, int lpc, int gbase) {
-  switch (t.kind) {
+  switch (TK >= 0 ? TK : t.kind) {
     case FMC_TARGET_GAUSS_DIAG:
       return tgt_gauss_diag<NV>(t, x, g, r, lpc);
     case FMC_TARGET_GAUSS_DENSE:
