@@ -227,6 +227,9 @@ def bench_nfmcmc(args):
     if not args.no_plain:
         import bench_plain as BP
 
+        print(f"bench.py: fused path {value:.0f} min-ESS/s in {wall:.2f} s; running the plain FP32 per-step path",
+              file=sys.stderr, flush=True)
+
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ppos, tr, pr = BP.run_plain_nfmcmc(theta0, cfg, target, x0.clone(), time_budget_s=args.plain_budget)

@@ -16,6 +16,7 @@ It is used only by bench.py; the product path is flowmc_amd.
 
 from __future__ import annotations
 
+import sys
 import math
 
 import torch
@@ -246,6 +247,8 @@ def run_plain_nfmcmc(theta, cfg, target, x0, seed=0, time_budget_s=None):
         x = global_step(x, train_pos, cur)
         cur += n_global
         rounds += 1
+        print(f"plain path: training round {rounds}/{n_rounds} done at {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
         if time_budget_s is not None and time.perf_counter() - t0 > time_budget_s:
             break
     cur = 0
@@ -256,6 +259,8 @@ def run_plain_nfmcmc(theta, cfg, target, x0, seed=0, time_budget_s=None):
         x = global_step(x, prod_pos, cur)
         cur += n_global
         prod_rounds += 1
+        print(f"plain path: production round {prod_rounds}/{n_rounds} done at {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
         if time_budget_s is not None and time.perf_counter() - t0 > 2 * time_budget_s:
             break
     return prod_pos[:, :cur], rounds, prod_rounds
