@@ -7,330 +7,9 @@
 // NV float4 chunks of position / gradient in registers for the whole launch,
 // so the K fused steps touch HBM only for the thinned buffer rows, the noise
 // (injected-noise mode) and the state at entry/exit.
-#include "fmc_common.h"
-#include "fmc_targets.h"
+#include "fmc_chain_steps.h"
 
 namespace fmc {
-
-struct ChainIO {
-  float* x;
-  float* logp;
-  const float* metric;
-  const float* noise;
-  const float* log_u;
-  float* out_pos;
-  float* out_logp;
-  float* out_acc;
-  int64_t C;
-  int64_t buf_len;
-  int64_t cursor;
-  uint32_t key0, key1;
-  const float* temps;  // tempered rows: row c targets log p / temps[c % n_temps] (NULL: untempered)
-  int n_temps;
-};
-
-struct StepCfg {
-  int n_steps;
-  int step0;
-  float dt;
-  int n_leapfrog;
-  int thinning;
-};
-
-template <int NV>
-__device__ __forceinline__ void load_row(const float* __restrict__ src, int d, int r, int lpc,
-                                         float4 (&x)[NV]) {
-  const bool vec = (d & 3) == 0;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    int j0 = (v * lpc + r) * 4;
-    if (vec) {
-      x[v] = (j0 < d) ? *reinterpret_cast<const float4*>(src + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) f4(x[v], e) = (j0 + e < d) ? src[j0 + e] : 0.f;
-    }
-  }
-}
-
-template <int NV>
-__device__ __forceinline__ void store_row(float* __restrict__ dst, int d, int r, int lpc,
-                                          const float4 (&x)[NV]) {
-  const bool vec = (d & 3) == 0;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    int j0 = (v * lpc + r) * 4;
-    if (vec) {
-      if (j0 < d) *reinterpret_cast<float4*>(dst + j0) = x[v];
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (j0 + e < d) dst[j0 + e] = f4c(x[v], e);
-    }
-  }
-}
-
-// Standard normals of (chain, step) for this lane's chunks; padding zeroed.
-template <int NV>
-__device__ __forceinline__ void draw_normals(const ChainIO& io, int d, int r, int lpc, int64_t c,
-                                             int s, int t_glob, uint32_t sid, float4 (&xi)[NV]) {
-  if (io.noise) {
-    load_row<NV>(io.noise + ((int64_t)s * io.C + c) * d, d, r, lpc, xi);
-  } else {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      int chunk = v * lpc + r;
-      xi[v] = philox_normal4(io.key0, io.key1, (uint32_t)chunk, (uint32_t)c, (uint32_t)t_glob, sid);
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (chunk * 4 + e >= d) f4(xi[v], e) = 0.f;
-    }
-  }
-}
-
-__device__ __forceinline__ float draw_log_u(const ChainIO& io, int64_t c, int s, int t_glob,
-                                            uint32_t sid) {
-  if (io.log_u) return io.log_u[(int64_t)s * io.C + c];
-  return philox_log_uniform(io.key0, io.key1, (uint32_t)c, (uint32_t)t_glob, sid);
-}
-
-template <int NV>
-__device__ __forceinline__ void write_rows(const ChainIO& io, int d, int r, int lpc, int64_t c,
-                                           int t_glob, int thinning, const float4 (&x)[NV], float lp,
-                                           bool acc) {
-  if (t_glob % thinning != 0) return;
-  int64_t row = io.cursor + t_glob / thinning;
-  if (io.out_pos) store_row<NV>(io.out_pos + (c * io.buf_len + row) * d, d, r, lpc, x);
-  if (r == 0) {
-    if (io.out_logp) io.out_logp[c * io.buf_len + row] = lp;
-    if (io.out_acc) io.out_acc[c * io.buf_len + row] = acc ? 1.f : 0.f;
-  }
-}
-
-// Tempered rows (parallel tempering): row c of a tempered launch targets
-// log p(x) / T with T = temps[c % n_temps] — TemperedPDF.tempered_log_pdf
-// (src/flowMC/resource/logPDF.py:104-106) with a zero log prior. The value and
-// the gradient of the target are scaled by beta = 1/T after each evaluation.
-__device__ __forceinline__ float row_beta(const ChainIO& io, int64_t c) {
-  return io.temps ? 1.f / io.temps[c % io.n_temps] : 1.f;
-}
-
-template <int NV>
-__device__ __forceinline__ float temper(const ChainIO& io, float beta, float lp, float4 (&g)[NV]) {
-  if (io.temps == nullptr) return lp;
-#pragma unroll
-  for (int v = 0; v < NV; ++v)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) f4(g[v], e) *= beta;
-  return beta * lp;
-}
-
-// ------------------------------------------------------------------ MALA ---
-// One chain group runs cfg.n_steps MALA steps (src/flowMC/resource/kernel/
-// MALA.py:26-89). The gradient at the current point is carried from the
-// previous step instead of being re-evaluated: MALA.py:59 recomputes it at the
-// same point, which for a deterministic target is the same value.
-template <int NV, int TK>
-__device__ __forceinline__ void mala_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
-                           int r, int lpc, int gbase) {
-  const int d = t.d;
-  float4 x[NV], g[NV];
-  load_row<NV>(io.x + c * d, d, r, lpc, x);
-  const float beta = row_beta(io, c);
-  float lp = temper<NV>(io, beta, target_vg<NV, TK>(t, x, g, r, lpc, gbase), g);
-  const float dt = cfg.dt;
-  const float h = 0.5f * dt * dt;
-  const float inv2dt2 = 1.f / (2.f * dt * dt);
-  for (int s = 0; s < cfg.n_steps; ++s) {
-    const int tg = cfg.step0 + s;
-    float4 xi[NV], prop[NV], g1[NV];
-    draw_normals<NV>(io, d, r, lpc, c, s, tg, RNG_MALA_NOISE, xi);
-    float s1 = 0.f;
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float n = dt * f4c(xi[v], e);
-        f4(prop[v], e) = f4c(x[v], e) + h * f4c(g[v], e) + n;
-        s1 += n * n;
-      }
-    float lp1 = temper<NV>(io, beta, target_vg<NV, TK>(t, prop, g1, r, lpc, gbase), g1);
-    float s2 = 0.f;
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float q = f4c(x[v], e) - f4c(prop[v], e) - h * f4c(g1[v], e);
-        s2 += q * q;
-      }
-    // padding lanes contribute exactly 0 to s1 and s2
-    float ds = group_sum(s1 - s2, lpc);
-    float ratio = (lp1 - lp) + ds * inv2dt2;
-    float lu = draw_log_u(io, c, s, tg, RNG_MALA_UNIFORM);
-    bool acc = lu < ratio;
-    if (acc) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        x[v] = prop[v];
-        g[v] = g1[v];
-      }
-      lp = lp1;
-    }
-    write_rows<NV>(io, d, r, lpc, c, tg, cfg.thinning, x, lp, acc);
-  }
-  store_row<NV>(io.x + c * d, d, r, lpc, x);
-  if (r == 0 && io.logp) io.logp[c] = lp;
-}
-
-// ------------------------------------------------------------------- GRW ---
-// GaussianRandomWalk.kernel (src/flowMC/resource/kernel/Gaussian_random_walk.py:
-// 25-61) scanned by TakeSerialSteps: x' = x + step * xi, accept iff
-// log u < logp(x') - logp(x). The log-density of the entry state is evaluated
-// once (take_steps.py:177) and carried; the target's gradient output is dead
-// and removed by the compiler.
-template <int NV, int TK>
-__device__ __forceinline__ void grw_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
-                          int r, int lpc, int gbase) {
-  const int d = t.d;
-  float4 x[NV], g[NV];
-  load_row<NV>(io.x + c * d, d, r, lpc, x);
-  const float beta = row_beta(io, c);
-  float lp = temper<NV>(io, beta, target_vg<NV, TK>(t, x, g, r, lpc, gbase), g);
-  const float dt = cfg.dt;
-  for (int s = 0; s < cfg.n_steps; ++s) {
-    const int tg = cfg.step0 + s;
-    float4 xi[NV], prop[NV];
-    draw_normals<NV>(io, d, r, lpc, c, s, tg, RNG_GRW_NOISE, xi);
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) f4(prop[v], e) = f4c(x[v], e) + dt * f4c(xi[v], e);
-    const float lp1 = temper<NV>(io, beta, target_vg<NV, TK>(t, prop, g, r, lpc, gbase), g);
-    const float lu = draw_log_u(io, c, s, tg, RNG_GRW_UNIFORM);
-    const bool acc = lu < lp1 - lp;
-    if (acc) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) x[v] = prop[v];
-      lp = lp1;
-    }
-    write_rows<NV>(io, d, r, lpc, c, tg, cfg.thinning, x, lp, acc);
-  }
-  store_row<NV>(io.x + c * d, d, r, lpc, x);
-  if (r == 0 && io.logp) io.logp[c] = lp;
-}
-
-// ------------------------------------------------------------------- HMC ---
-// HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) with the leapfrog
-// coefficient table of HMC.py:47-50: row 0 = (0, 1/2), rows 1..L = (1, 1),
-// row L+1 = (1, 1/2): L+1 drifts x += eps*c0*metric*p and L+2 kicks
-// p += eps*c1*grad logp(x). H0 - H1 is accumulated in FP64.
-//
-// UNIT_M: identity metric (io.metric == nullptr). Padding elements of p, x and
-// the gradient are 0 throughout, so that form needs no per-element index
-// test and no metric multiply.
-template <int NV, bool UNIT_M, int TK>
-__device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
-                          int r, int lpc, int gbase) {
-  const int d = t.d;
-  float4 x[NV], g[NV], m[NV];
-  load_row<NV>(io.x + c * d, d, r, lpc, x);
-  if (!UNIT_M) {
-    load_row<NV>(io.metric, d, r, lpc, m);
-  } else {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) m[v] = make_float4(1.f, 1.f, 1.f, 1.f);
-  }
-  const float beta = row_beta(io, c);
-  float lp = temper<NV>(io, beta, target_vg<NV, TK>(t, x, g, r, lpc, gbase), g);
-  const float eps = cfg.dt;
-  const int L = cfg.n_leapfrog;
-  for (int s = 0; s < cfg.n_steps; ++s) {
-    const int tg = cfg.step0 + s;
-    float4 p[NV], xl[NV], gl[NV];
-    draw_normals<NV>(io, d, r, lpc, c, s, tg, RNG_HMC_MOMENTUM, p);
-    double k0 = 0.0;
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (UNIT_M) {
-          const double pj = (double)f4c(p[v], e);
-          k0 += 0.5 * pj * pj;
-        } else {
-          int j = dim_of(v, e, r, lpc);
-          float mj = f4c(m[v], e);
-          float pj = (j < d) ? f4c(p[v], e) / sqrtf(mj) : 0.f;
-          f4(p[v], e) = pj;
-          k0 += 0.5 * (double)mj * (double)pj * (double)pj;
-        }
-      }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      xl[v] = x[v];
-      gl[v] = g[v];
-    }
-    k0 = group_sum(k0, lpc);
-    // row 0: drift coefficient 0, kick 1/2
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) f4(p[v], e) += eps * 0.5f * f4c(gl[v], e);
-    float lpl = lp;
-    for (int i = 1; i <= L + 1; ++i) {
-      const float kick = (i <= L) ? 1.f : 0.5f;
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) f4(xl[v], e) += eps * (UNIT_M ? f4c(p[v], e) : f4c(m[v], e) * f4c(p[v], e));
-      lpl = temper<NV>(io, beta, target_vg<NV, TK>(t, xl, gl, r, lpc, gbase), gl);
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) f4(p[v], e) += eps * kick * f4c(gl[v], e);
-    }
-    double k1 = 0.0;
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (UNIT_M) {
-          const double pj = (double)f4c(p[v], e);
-          k1 += 0.5 * pj * pj;
-        } else {
-          int j = dim_of(v, e, r, lpc);
-          if (j < d) k1 += 0.5 * (double)f4c(m[v], e) * (double)f4c(p[v], e) * (double)f4c(p[v], e);
-        }
-      }
-    k1 = group_sum(k1, lpc);
-    double h0 = -(double)lp + k0;
-    double h1 = -(double)lpl + k1;
-    double log_acc = h0 - h1;
-    float lu = draw_log_u(io, c, s, tg, RNG_HMC_UNIFORM);
-    bool acc = (double)lu < log_acc;
-    if (acc) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        x[v] = xl[v];
-        g[v] = gl[v];
-      }
-      lp = lpl;
-    }
-    write_rows<NV>(io, d, r, lpc, c, tg, cfg.thinning, x, lp, acc);
-  }
-  store_row<NV>(io.x + c * d, d, r, lpc, x);
-  if (r == 0 && io.logp) io.logp[c] = lp;
-}
-
-// Copies the target parameter block to LDS when it fits (n_lds > 0).
-__device__ __forceinline__ TargetDev stage_target(const fmc_target& tg, float* smem, int n_lds) {
-  TargetDev t{tg.kind, tg.n_dim, tg.n_modes, tg.params};
-  if (n_lds > 0) {
-    for (int i = threadIdx.x; i < n_lds; i += blockDim.x) smem[i] = tg.params[i];
-    __syncthreads();
-    t.p = smem;
-  }
-  return t;
-}
 
 enum KernelKind { KK_MALA = 0, KK_HMC = 1, KK_GRW = 2 };
 
@@ -354,29 +33,6 @@ __global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, 
     hmc_group<NV, false, TK>(t, io, cfg, c, r, lpc, gbase);
   else
     hmc_group<NV, true, TK>(t, io, cfg, c, r, lpc, gbase);
-}
-
-// Many HMC problems of uneven dimension in one launch: one work item per block.
-template <int NV>
-__global__ void __launch_bounds__(1024) hmc_batched_kernel(const fmc_hmc_slot* __restrict__ slots,
-                                                           const fmc_work_item* __restrict__ items,
-                                                           StepCfg cfg) {
-  const fmc_work_item it = items[blockIdx.x];
-  const fmc_hmc_slot sl = slots[it.slot];
-  const int lpc = sl.lanes_per_chain;
-  const int lane = threadIdx.x & 63;
-  const int r = lane & (lpc - 1);
-  const int gbase = lane - r;
-  const int local = threadIdx.x / lpc;
-  if (local >= it.n_chains) return;
-  const int64_t c = (int64_t)it.chain0 + local;
-  TargetDev t{sl.target.kind, sl.target.n_dim, sl.target.n_modes, sl.target.params};
-  ChainIO io{sl.x,        sl.logp,     sl.metric,  sl.noise,  sl.log_u,  sl.out_pos, sl.out_logp,
-             sl.out_acc,  sl.n_chains, sl.buf_len, sl.cursor, sl.key0,   sl.key1};
-  if (io.metric)
-    hmc_group<NV, false, -1>(t, io, cfg, c, r, lpc, gbase);
-  else
-    hmc_group<NV, true, -1>(t, io, cfg, c, r, lpc, gbase);
 }
 
 // ------------------------------------------------------- target eval ------
@@ -1079,33 +735,6 @@ extern "C" int fmc_hmc_run(void* stream, const fmc_target* target, float* x, flo
   StepCfg sc{n_steps, step0, step_size, n_leapfrog, thinning};
   return launch_chain<KK_HMC>((hipStream_t)stream, target, io, sc, L);
 }
-
-extern "C" int fmc_hmc_run_batched(void* stream, const fmc_hmc_slot* slots, int32_t n_slots,
-                                   const fmc_work_item* items, int32_t n_items, int32_t max_dim,
-                                   int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
-                                   int32_t thinning, int32_t vec_per_lane, int32_t block_size) {
-  FMC_ARG(slots != nullptr && items != nullptr, "slots/items are NULL");
-  FMC_ARG(n_slots >= 1 && n_items >= 0, "bad n_slots / n_items");
-  FMC_ARG(n_leapfrog >= 0 && n_steps >= 0 && step0 >= 0 && thinning >= 1, "bad step arguments");
-  FMC_ARG(vec_per_lane == 1 || vec_per_lane == 2 || vec_per_lane == 4 || vec_per_lane == 8,
-          "vec_per_lane must be 1, 2, 4 or 8");
-  FMC_ARG(block_size >= 64 && block_size <= 1024 && block_size % 64 == 0, "bad block_size");
-  FMC_ARG(max_dim >= 1, "max_dim must be >= 1");
-  // slot / item tables live in device memory; the host-side builder
-  // (flowmc_amd.kernels.hmc_batched) validates lanes_per_chain, dims and tiles.
-  if (n_items == 0) return FMC_OK;
-  StepCfg sc{n_steps, step0, step_size, n_leapfrog, thinning};
-  hipStream_t st = (hipStream_t)stream;
-  dim3 g((unsigned)n_items), b((unsigned)block_size);
-  switch (vec_per_lane) {
-    case 1: hipLaunchKernelGGL(hmc_batched_kernel<1>, g, b, 0, st, slots, items, sc); break;
-    case 2: hipLaunchKernelGGL(hmc_batched_kernel<2>, g, b, 0, st, slots, items, sc); break;
-    case 4: hipLaunchKernelGGL(hmc_batched_kernel<4>, g, b, 0, st, slots, items, sc); break;
-    default: hipLaunchKernelGGL(hmc_batched_kernel<8>, g, b, 0, st, slots, items, sc); break;
-  }
-  return check_launch("batched hmc kernel");
-}
-
 
 extern "C" int fmc_mala_propose(void* stream, const float* x, const float* grad, float* prop, int64_t n_chains,
                                 int32_t n_dim, float step_size, uint32_t key0, uint32_t key1, int32_t step,

@@ -1,0 +1,73 @@
+// Many HMC problems of uneven dimension advanced in one launch (gfx950).
+//
+// Each work item (slot, first chain, n chains) is one workgroup; a chain is
+// held by the slot's lanes_per_chain lanes exactly as in the single-target
+// fused HMC kernel, and runs HMC.kernel (src/flowMC/resource/kernel/HMC.py:
+// 98-151) for cfg.n_steps steps. Slots may mix target kinds, so the target
+// is dispatched at run time (a workgroup-uniform branch). Specialising each
+// workgroup's whole HMC loop on its target kind instead (one loop body per
+// kind in one kernel) raised the register demand to the 128-VGPR cap with
+// spills at two float4 chunks per lane, so the single loop with the uniform
+// dispatch is kept. Its own translation unit: it builds in parallel with the
+// other chain kernels.
+#include "fmc_chain_steps.h"
+
+namespace fmc {
+
+// At most 512 threads per workgroup: the register budget is then 256 VGPRs,
+// which the run-time target dispatch needs at two or more float4 chunks per
+// lane (under a 1024-thread bound the 128-VGPR cap spilled). The batch has
+// about 1.4 wavefronts per SIMD, so the bound costs no occupancy.
+template <int NV>
+__global__ void __launch_bounds__(512) hmc_batched_kernel(const fmc_hmc_slot* __restrict__ slots,
+                                                           const fmc_work_item* __restrict__ items,
+                                                           StepCfg cfg) {
+  const fmc_work_item it = items[blockIdx.x];
+  const fmc_hmc_slot sl = slots[it.slot];
+  const int lpc = sl.lanes_per_chain;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (lpc - 1);
+  const int gbase = lane - r;
+  const int local = threadIdx.x / lpc;
+  if (local >= it.n_chains) return;  // whole chain groups leave together
+  const int64_t c = (int64_t)it.chain0 + local;
+  TargetDev t{sl.target.kind, sl.target.n_dim, sl.target.n_modes, sl.target.params};
+  ChainIO io{sl.x,        sl.logp,     sl.metric,  sl.noise,  sl.log_u,  sl.out_pos, sl.out_logp,
+             sl.out_acc,  sl.n_chains, sl.buf_len, sl.cursor, sl.key0,   sl.key1};
+  if (io.metric)
+    hmc_group<NV, false, -1>(t, io, cfg, c, r, lpc, gbase);
+  else
+    hmc_group<NV, true, -1>(t, io, cfg, c, r, lpc, gbase);
+}
+
+}  // namespace fmc
+
+using namespace fmc;
+
+extern "C" int fmc_hmc_run_batched(void* stream, const fmc_hmc_slot* slots, int32_t n_slots,
+                                   const fmc_work_item* items, int32_t n_items, int32_t max_dim,
+                                   int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
+                                   int32_t thinning, int32_t vec_per_lane, int32_t block_size) {
+  FMC_ARG(slots != nullptr && items != nullptr, "slots/items are NULL");
+  FMC_ARG(n_slots >= 1 && n_items >= 0, "bad n_slots / n_items");
+  FMC_ARG(n_leapfrog >= 0 && n_steps >= 0 && step0 >= 0 && thinning >= 1, "bad step arguments");
+  FMC_ARG(vec_per_lane == 1 || vec_per_lane == 2 || vec_per_lane == 4 || vec_per_lane == 8,
+          "vec_per_lane must be 1, 2, 4 or 8");
+  FMC_ARG(block_size >= 64 && block_size <= 512 && block_size % 64 == 0,
+          "block_size must be a multiple of 64 in [64, 512] (got %d)", block_size);
+  FMC_ARG(max_dim >= 1, "max_dim must be >= 1");
+  // The slot / item tables live in device memory; the host-side builder
+  // (flowmc_amd.resource.kernel.batched_HMC.BatchedHMC) validates
+  // lanes_per_chain (a power of two <= 64 dividing the block), dims and tiles.
+  if (n_items == 0) return FMC_OK;
+  StepCfg sc{n_steps, step0, step_size, n_leapfrog, thinning};
+  hipStream_t st = (hipStream_t)stream;
+  dim3 g((unsigned)n_items), b((unsigned)block_size);
+  switch (vec_per_lane) {
+    case 1: hipLaunchKernelGGL(hmc_batched_kernel<1>, g, b, 0, st, slots, items, sc); break;
+    case 2: hipLaunchKernelGGL(hmc_batched_kernel<2>, g, b, 0, st, slots, items, sc); break;
+    case 4: hipLaunchKernelGGL(hmc_batched_kernel<4>, g, b, 0, st, slots, items, sc); break;
+    default: hipLaunchKernelGGL(hmc_batched_kernel<8>, g, b, 0, st, slots, items, sc); break;
+  }
+  return check_launch("batched hmc kernel");
+}

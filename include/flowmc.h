@@ -218,6 +218,8 @@ typedef struct fmc_work_item {
   int32_t reserved;
 } fmc_work_item;
 
+/* block_size: a multiple of 64 in [64, 512]; every slot's lanes_per_chain   */
+/* divides it.                                                               */
 int fmc_hmc_run_batched(void* stream, const fmc_hmc_slot* slots, int32_t n_slots,
                         const fmc_work_item* items, int32_t n_items, int32_t max_dim,
                         int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
