@@ -325,25 +325,35 @@ def _bench_local_rosenbrock(args, kind):
     lps = torch.empty((C_, T), dtype=torch.float32, device=dev)
     acc = torch.empty((C_, T), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
     cfg = N.fmc_launch_cfg(args.lpc, args.nv, args.block, 0) if args.lpc else None
     cfgp = C.byref(cfg) if cfg is not None else None
 
     def run(n_steps, key):
         k0, k1 = fr.key_words(key)
+        st = torch.cuda.current_stream().cuda_stream  # the capture stream while a graph is recorded
         for s0 in range(0, n_steps, K):
             ns = min(K, n_steps - s0)
-            N.call("fmc_mala_run" if kind == "mala" else "fmc_grw_run", sp, C.byref(desc), x.data_ptr(),
+            N.call("fmc_mala_run" if kind == "mala" else "fmc_grw_run", st, C.byref(desc), x.data_ptr(),
                    lp.data_ptr(), C_, ns, s0, dt, k0, k1, None, None, pos.data_ptr(), lps.data_ptr(), acc.data_ptr(),
                    T, 0, 1, cfgp)
 
     run(args.warmup, fr.PRNGKey(1))
     torch.cuda.synchronize()
+    graph = None
+    if not args.no_graph:
+        # the whole timed sequence of fused launches (64 steps each) as one HIP graph
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            run(steps, fr.PRNGKey(2))
+        torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
-    run(steps, fr.PRNGKey(2))
+    if graph is not None:
+        graph.replay()
+    else:
+        run(steps, fr.PRNGKey(2))
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -361,7 +371,8 @@ def _bench_local_rosenbrock(args, kind):
                      "kernel_ms": gpu_ms / n_launch, "bytes_per_launch": bytes_total / n_launch},
         "extra": {"fp32_tflops": flops / (gpu_ms * 1e-3) / 1e12, "acceptance": float(acc[:, :steps].mean())},
         "config": {"workload": f"{kind}_rosenbrock_128d", "chains": C_, "dim": d, "step_size": dt,
-                   "fused_steps_per_launch": K, "thinning": 1},
+                   "fused_steps_per_launch": K, "thinning": 1,
+                   "launch": "eager" if graph is None else f"one HIP graph of {n_launch} fused launches"},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_mala_rosenbrock(args, kind)
@@ -841,6 +852,8 @@ def main():
     ap.add_argument("--plain-budget", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="Rosenbrock MALA / GRW: launch the fused steps eagerly instead of replaying a HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)

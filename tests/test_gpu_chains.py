@@ -438,3 +438,45 @@ def test_batched_hmc_matches_single_target_runs(gpu):
     # the autotuner returns a valid layout and the tuned launch reproduces the default-layout run
     layout = bh.tune(fr.PRNGKey(1))
     assert layout[0] in (1, 2) and all(4 * layout[0] * l >= d for l, d in zip(layout[2], dims))
+
+
+@pytest.mark.parametrize("entry", ["fmc_mala_run", "fmc_grw_run"])
+def test_fused_launches_replay_in_a_hip_graph(gpu, entry):
+    # A sequence of fused launches captured as one HIP graph replays to the eager
+    # result bit for bit, replay after replay, with eager work and host
+    # synchronisation between the replays (the launches record no memsets).
+    C_, d, K, S = 1024, 128, 16, 64
+    tgt = TG.Rosenbrock(d, device="cuda")
+    desc = tgt.descriptor()
+    x0 = fr.normal(fr.PRNGKey(11), (C_, d), device="cuda")
+    x = x0.clone()
+    lp = torch.empty(C_, device="cuda")
+    pos = torch.empty((C_, S, d), device="cuda")
+    lps = torch.empty((C_, S), device="cuda")
+    acc = torch.empty((C_, S), device="cuda")
+    k0, k1 = fr.key_words(fr.PRNGKey(12))
+
+    def run():
+        st = torch.cuda.current_stream().cuda_stream
+        for s0 in range(0, S, K):
+            N.call(entry, st, C.byref(desc), x.data_ptr(), lp.data_ptr(), C_, K, s0, 0.01, k0, k1, None, None,
+                   pos.data_ptr(), lps.data_ptr(), acc.data_ptr(), S, 0, 1, None)
+
+    run()
+    torch.cuda.synchronize()
+    want = [t.clone() for t in (x, lp, pos, lps, acc)]
+    x.copy_(x0)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        run()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        x.copy_(x0)
+        for t in (lp, pos, lps, acc):
+            t.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = (x, lp, pos, lps, acc)
+        assert all(torch.equal(a, b) for a, b in zip(got, want)), f"replay {rep} differs from the eager run"
+        _ = (x0 * 2).sum().item()  # eager work between replays
