@@ -741,11 +741,22 @@ def bench_hmc_funnels(args):
     layout = bh.tune(fr.PRNGKey(9))
     bh.run(fr.PRNGKey(1), min(T, K), outputs=outs)
     torch.cuda.synchronize()
+    launch = bh.prepare(fr.PRNGKey(2), T, outputs=outs, fused_steps=K)  # tables uploaded here
+    graph = None
+    if not args.no_graph:
+        # the timed launches (16 steps each, all 32 targets per launch) as one HIP graph
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            launch()
+        torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
-    bh.run(fr.PRNGKey(2), T, outputs=outs, fused_steps=K)
+    if graph is not None:
+        graph.replay()
+    else:
+        launch()
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -769,7 +780,8 @@ def bench_hmc_funnels(args):
                   "v_mean": float(v_all.mean()), "v_var": float(v_all.var()), "layout": [layout[0], layout[1]],
                   "dims": dims},
         "config": {"workload": "hmc_funnels", "targets": 32, "chains_per_target": C_, "n_leapfrog": L,
-                   "step_size": 0.05, "fused_steps_per_launch": K},
+                   "step_size": 0.05, "fused_steps_per_launch": K,
+                   "launch": "eager" if graph is None else f"one HIP graph of {n_launch} batched launches"},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_hmc_funnels(args, dims, C_, L)
@@ -853,7 +865,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
-                    help="Rosenbrock MALA / GRW: launch the fused steps eagerly instead of replaying a HIP graph")
+                    help="Rosenbrock MALA / GRW, batched HMC: launch the fused steps eagerly instead of replaying a HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)

@@ -103,26 +103,49 @@ class BatchedHMC:
         """Advance every problem by ``n_steps`` HMC steps. ``outputs[s]`` is
         (positions [C,T,d], log_probs [C,T], accepts [C,T], cursor) or None;
         ``noise[s]`` [n_steps, C, d] / ``log_u[s]`` [n_steps, C] inject randomness."""
+        if noise is None and log_u is None:
+            # the slot / work-item tables are the same for every launch of this
+            # call (the step offset is a kernel argument): upload them once
+            launch = self.prepare(key, n_steps, outputs, thinning, fused_steps, layout)
+            launch()
+            self._last_tables = launch.tables  # keep alive until the stream passes them
+            return
         layout = layout or self.layout or self.default_layout()
         nv, block, lpcs = layout
         st = _native.stream_ptr()
         maxd = max(self.dims)
-        # without injected noise the slot / work-item tables are the same for
-        # every launch of this call (the step offset is a kernel argument):
-        # build and upload them once
-        tables = self._tables(layout, key, fused_steps, outputs, None, None) if noise is None and log_u is None \
-            else None
         for s0 in range(0, n_steps, fused_steps):
+            # injected noise: each launch's slots point at its own noise window
             ns = min(fused_steps, n_steps - s0)
-            if tables is None:
-                nz = [n[s0:s0 + ns] for n in noise] if noise is not None else None
-                lu = [u[s0:s0 + ns] for u in log_u] if log_u is not None else None
-                slot_dev, it_dev, n_items = self._tables(layout, key, ns, outputs, nz, lu)
-            else:
-                slot_dev, it_dev, n_items = tables
+            nz = [n[s0:s0 + ns] for n in noise] if noise is not None else None
+            lu = [u[s0:s0 + ns] for u in log_u] if log_u is not None else None
+            slot_dev, it_dev, n_items = self._tables(layout, key, ns, outputs, nz, lu)
             _native.call("fmc_hmc_run_batched", st, slot_dev.data_ptr(), len(self.targets), it_dev.data_ptr(), n_items,
                          maxd, ns, s0, self.step_size, self.n_leapfrog, thinning, nv, block)
             self._last_tables = (slot_dev, it_dev)  # keep alive until the stream passes them
+
+    def prepare(self, key, n_steps: int, outputs=None, thinning: int = 1, fused_steps: int = 16, layout=None):
+        """Upload the slot / work-item tables of ``run(key, n_steps, outputs,
+        thinning, fused_steps=fused_steps)`` (no injected noise) and return a
+        callable that issues its launches on the current stream. The callable
+        allocates and copies nothing, so it can be recorded in a HIP graph
+        (``torch.cuda.graph``) and replayed; each call or replay advances the
+        chains from their current state with the same keys and step offsets."""
+        layout = layout or self.layout or self.default_layout()
+        nv, block, lpcs = layout
+        maxd = max(self.dims)
+        slot_dev, it_dev, n_items = self._tables(layout, key, fused_steps, outputs, None, None)
+        n_slots, eps, n_lf = len(self.targets), self.step_size, self.n_leapfrog
+
+        def launch():
+            st = _native.stream_ptr()
+            for s0 in range(0, n_steps, fused_steps):
+                ns = min(fused_steps, n_steps - s0)
+                _native.call("fmc_hmc_run_batched", st, slot_dev.data_ptr(), n_slots, it_dev.data_ptr(), n_items,
+                             maxd, ns, s0, eps, n_lf, thinning, nv, block)
+
+        launch.tables = (slot_dev, it_dev)
+        return launch
 
     def tune(self, key, n_steps: int = 8):
         """Pick (vec_per_lane, block size, lane spread) by timing short runs."""

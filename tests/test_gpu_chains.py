@@ -480,3 +480,39 @@ def test_fused_launches_replay_in_a_hip_graph(gpu, entry):
         got = (x, lp, pos, lps, acc)
         assert all(torch.equal(a, b) for a, b in zip(got, want)), f"replay {rep} differs from the eager run"
         _ = (x0 * 2).sum().item()  # eager work between replays
+
+
+def test_batched_hmc_replays_in_a_hip_graph(gpu):
+    # BatchedHMC.prepare uploads the tables once; its launches recorded as one HIP
+    # graph replay to the eager result bit for bit, replay after replay.
+    from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
+
+    dims = TG.funnel_dims(5, n_targets=5, lo=8, hi=200)
+    C_, S = 96, 32
+    bh = BatchedHMC([TG.Funnel(d) for d in dims], C_, step_size=0.05, n_leapfrog=10)
+    bh.init_positions(fr.PRNGKey(3))
+    x0 = [x.clone() for x in bh.x]
+    outs = [(torch.empty(C_, S, d, device=gpu), torch.empty(C_, S, device=gpu), torch.empty(C_, S, device=gpu), 0)
+            for d in dims]
+    launch = bh.prepare(fr.PRNGKey(4), S, outputs=outs, fused_steps=8)
+    launch()
+    torch.cuda.synchronize()
+    want = [t.clone() for o in outs for t in o[:3]] + [x.clone() for x in bh.x] + [lp.clone() for lp in bh.logp]
+    for x, x_init in zip(bh.x, x0):
+        x.copy_(x_init)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        launch()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for x, x_init in zip(bh.x, x0):
+            x.copy_(x_init)
+        for o in outs:
+            for t in o[:3]:
+                t.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = [t for o in outs for t in o[:3]] + list(bh.x) + list(bh.logp)
+        assert all(torch.equal(a, b) for a, b in zip(got, want)), f"replay {rep} differs from the eager run"
+        _ = sum(float(x.sum()) for x in x0)  # eager work between replays
