@@ -337,6 +337,12 @@ def _bench_local_rosenbrock(args, kind):
                    lp.data_ptr(), C_, ns, s0, dt, k0, k1, None, None, pos.data_ptr(), lps.data_ptr(), acc.data_ptr(),
                    T, 0, 1, cfgp)
 
+    # calibration copy of known traffic (256 MiB read + 256 MiB written) for
+    # the FETCH_SIZE / WRITE_SIZE counter runs of this workload
+    ca = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+    cb = torch.empty_like(ca)
+    cb.copy_(ca)
+    del ca, cb
     run(args.warmup, fr.PRNGKey(1))
     torch.cuda.synchronize()
     graph = None
@@ -363,11 +369,16 @@ def _bench_local_rosenbrock(args, kind):
     achieved = bytes_total / (gpu_ms * 1e-3) / 1e9
     flops = C_ * steps * ((17 * d + 12 * d) if kind == "mala" else (8 * d + 8 * d))
     kname = "chain_kernel<1,0> (fused MALA)" if kind == "mala" else "chain_kernel<1,2> (fused GRW)"
+    traffic = None  # HBM bytes per 64-step launch from the counter runs of this workload
+    tf = os.path.join(ROOT, "profiles", f"chain_kernel_traffic__{kind}_rosenbrock_128d.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
     out = {
         "value": C_ * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
                      "kernel_ms": gpu_ms / n_launch, "bytes_per_launch": bytes_total / n_launch},
         "extra": {"fp32_tflops": flops / (gpu_ms * 1e-3) / 1e12, "acceptance": float(acc[:, :steps].mean())},
         "config": {"workload": f"{kind}_rosenbrock_128d", "chains": C_, "dim": d, "step_size": dt,
