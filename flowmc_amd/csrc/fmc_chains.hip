@@ -14,9 +14,12 @@ namespace fmc {
 enum KernelKind { KK_MALA = 0, KK_HMC = 1, KK_GRW = 2 };
 
 // One kernel per (lane layout NV, step kind, target kind TK): the target's
-// value and gradient are inlined without a run-time dispatch.
+// value and gradient are inlined without a run-time dispatch. HMC keeps
+// position, momentum, gradient, metric and the leapfrog copies in registers:
+// its workgroups are bounded at 512 threads, which gives it 256 VGPRs
+// instead of the 128 of a 1024-thread bound (spills at NV >= 2).
 template <int NV, int KIND, int TK>
-__global__ void __launch_bounds__(1024) chain_kernel(fmc_target tg, ChainIO io, StepCfg cfg,
+__global__ void __launch_bounds__(KIND == KK_HMC ? 512 : 1024) chain_kernel(fmc_target tg, ChainIO io, StepCfg cfg,
                                                      int lpc, int n_lds) {
   extern __shared__ float smem[];
   TargetDev t = stage_target(tg, smem, n_lds);
@@ -517,6 +520,8 @@ static void launch_chain_nv(hipStream_t st, const fmc_target* t, const ChainIO& 
 template <int KIND>
 static int launch_chain(hipStream_t st, const fmc_target* t, const ChainIO& io, const StepCfg& cfg,
                         const Layout& L) {
+  FMC_ARG(KIND != KK_HMC || L.block <= 512, "fused HMC launches take at most 512 threads per workgroup (got %d)",
+          L.block);
   const int per_block = L.block / L.lpc;
   const int64_t grid = (io.C + per_block - 1) / per_block;
   FMC_ARG(grid <= 0x7fffffffLL, "too many chains for one launch");

@@ -70,7 +70,8 @@ typedef struct fmc_target {
 typedef struct fmc_launch_cfg {
   int32_t lanes_per_chain;
   int32_t vec_per_lane;
-  int32_t block_size; /* threads per workgroup (64..1024, multiple of 64) */
+  int32_t block_size; /* threads per workgroup (64..1024, multiple of 64;  */
+                      /* fused HMC launches: at most 512)                */
   int32_t reserved;
 } fmc_launch_cfg;
 
