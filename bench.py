@@ -368,7 +368,8 @@ def _bench_local_rosenbrock(args, kind):
     bytes_total = C_ * (steps * (4 * d + 8) + n_launch * (8 * d + 8))
     achieved = bytes_total / (gpu_ms * 1e-3) / 1e9
     flops = C_ * steps * ((17 * d + 12 * d) if kind == "mala" else (8 * d + 8 * d))
-    kname = "chain_kernel<1,0> (fused MALA)" if kind == "mala" else "chain_kernel<1,2> (fused GRW)"
+    kname = ("chain_kernel<1,MALA,ROSENBROCK> (fused MALA)" if kind == "mala"
+             else "chain_kernel<1,GRW,ROSENBROCK> (fused GRW)")
     traffic = None  # HBM bytes per 64-step launch from the counter runs of this workload
     tf = os.path.join(ROOT, "profiles", f"chain_kernel_traffic__{kind}_rosenbrock_128d.json")
     if os.path.exists(tf):
