@@ -368,8 +368,9 @@ def _bench_local_rosenbrock(args, kind):
     bytes_total = C_ * (steps * (4 * d + 8) + n_launch * (8 * d + 8))
     achieved = bytes_total / (gpu_ms * 1e-3) / 1e9
     flops = C_ * steps * ((17 * d + 12 * d) if kind == "mala" else (8 * d + 8 * d))
-    kname = ("chain_kernel<1,MALA,ROSENBROCK> (fused MALA)" if kind == "mala"
-             else "chain_kernel<1,GRW,ROSENBROCK> (fused GRW)")
+    lay = f"{args.lpc} lanes x {args.nv} float4 per chain" if args.lpc else "default layout: 16 lanes x 2 float4 per chain"
+    kname = (f"chain_kernel<{args.nv if args.lpc else 2},{kind.upper()},ROSENBROCK> "
+             f"(fused {kind.upper()}, {lay})")
     traffic = None  # HBM bytes per 64-step launch from the counter runs of this workload
     tf = os.path.join(ROOT, "profiles", f"chain_kernel_traffic__{kind}_rosenbrock_128d.json")
     if os.path.exists(tf):
