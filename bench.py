@@ -753,6 +753,12 @@ def bench_hmc_funnels(args):
              torch.empty((C_, T), dtype=torch.float32, device="cuda"),
              torch.empty((C_, T), dtype=torch.float32, device="cuda"), 0) for d in dims]
     layout = bh.tune(fr.PRNGKey(9))
+    # calibration copy of known traffic (256 MiB read + 256 MiB written) for
+    # the FETCH_SIZE / WRITE_SIZE counter runs of this workload
+    ca = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+    cb = torch.empty_like(ca)
+    cb.copy_(ca)
+    del ca, cb
     bh.run(fr.PRNGKey(1), min(T, K), outputs=outs)
     torch.cuda.synchronize()
     launch = bh.prepare(fr.PRNGKey(2), T, outputs=outs, fused_steps=K)  # tables uploaded here
@@ -781,12 +787,17 @@ def bench_hmc_funnels(args):
     flops_total *= C_ * T
     bytes_total = sum(C_ * T * (4 * d + 8) + C_ * n_launch * (8 * d + 8) for d in dims)
     achieved = flops_total / gpu_s / 1e12
+    traffic = None  # HBM bytes per 16-step launch from the counter runs of this workload
+    tf = os.path.join(ROOT, "profiles", "hmc_batched_kernel_traffic__hmc_funnels.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
     v_all = torch.cat([o[0][..., 0].reshape(-1) for o in outs])
     out = {
         "value": C_ * len(dims) * T / wall,
         "ms_per_step": wall * 1e3 / T,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "hmc_batched_kernel (FP32 vector ALU)", "kernel_ms": gpu_s * 1e3 / n_launch,
                      "flops_per_launch": flops_total / n_launch},
         "extra": {"hbm_gbs": bytes_total / gpu_s / 1e9, "hbm_frac": bytes_total / gpu_s / 1e9 / HBM_PEAK_GBS,
