@@ -122,6 +122,7 @@ _SIGNATURES: dict[str, list] = {
     "fmc_abi_version": [],
     "fmc_last_error": [],
     "fmc_device_query": [C.c_int, P, P, P, P, P],
+    "fmc_target_check": [C.POINTER(fmc_target)],
     "fmc_target_eval": [P, C.POINTER(fmc_target), P, I64, P, P, C.POINTER(fmc_launch_cfg)],
     "fmc_mala_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, U32, U32, P, P, P, P, P, I64, I64, I32,
                      C.POINTER(fmc_launch_cfg)],

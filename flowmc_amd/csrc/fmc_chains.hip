@@ -228,71 +228,6 @@ __global__ void hmc_accept_kernel(float* x, float* logp, const float* __restrict
   }
 }
 
-// --------------------------------------------------------- NF MH scan -----
-// NF_proposal.py:91-126: sequential independence MH over the S proposals of
-// each chain. Phase 1 (one thread per chain) decides and records which
-// proposal the chain sits on after every step; phase 2 gathers the rows.
-__global__ void nf_mh_scan_kernel(const float* __restrict__ lp0, const float* __restrict__ q0,
-                                  const float* __restrict__ lpp, const float* __restrict__ qp,
-                                  int64_t C, int S, uint32_t k0, uint32_t k1, const float* log_u,
-                                  int* out_idx, float* lp_last, float* out_logp, float* out_acc,
-                                  int64_t buf_len, int64_t cursor, int thinning) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
-  float lpc = lp0[c], qc = q0[c], lp_keep = lp0[c];
-  int idx = -1;
-  const int s_last = ((S - 1) / thinning) * thinning;
-  for (int s = 0; s < S; ++s) {
-    const int64_t k = c * S + s;
-    float lpv = lpp[k], qv = qp[k];
-    float ratio = (lpv - lpc) - (qv - qc);
-    float lu = log_u ? log_u[k] : philox_log_uniform(k0, k1, (uint32_t)c, (uint32_t)s, RNG_NF_UNIFORM);
-    bool acc = lu < ratio;
-    if (acc) {
-      idx = s;
-      lpc = lpv;
-      qc = qv;
-    }
-    out_idx[k] = idx;
-    if (s == s_last) lp_keep = lpc;
-    if (s % thinning == 0) {
-      int64_t row = cursor + s / thinning;
-      if (out_logp) out_logp[c * buf_len + row] = lpc;
-      if (out_acc) out_acc[c * buf_len + row] = acc ? 1.f : 0.f;
-    }
-  }
-  if (lp_last) lp_last[c] = lp_keep;
-}
-
-// One (chain, thinned step) row per 64 lanes.
-__global__ void nf_mh_gather_kernel(const float* __restrict__ x0, const float* __restrict__ prop,
-                                    const int* __restrict__ idx, int64_t C, int S, int d,
-                                    float* out_pos, int64_t buf_len, int64_t cursor, int thinning,
-                                    int n_rows) {
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= C * n_rows) return;
-  const int64_t c = w / n_rows;
-  const int rr = (int)(w - c * n_rows);
-  const int s = rr * thinning;
-  const int id = idx[c * S + s];
-  const float* src = (id < 0) ? x0 + c * d : prop + (c * S + id) * d;
-  float* dst = out_pos + (c * buf_len + cursor + rr) * d;
-  for (int j = lane; j < d; j += 64) dst[j] = src[j];
-}
-
-// Final position of each chain after the scan.
-__global__ void nf_mh_last_kernel(const float* __restrict__ x0, const float* __restrict__ prop,
-                                  const int* __restrict__ idx, int64_t C, int S, int d, int s_last,
-                                  float* x_last) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C * d) return;
-  const int64_t c = i / d;
-  const int j = (int)(i - c * d);
-  const int id = idx[c * S + s_last];
-  x_last[i] = (id < 0) ? x0[i] : prop[(c * S + id) * d + j];
-}
-
 // -------------------------------------------------------------- RNG -------
 // ---------------------------------------------------- finite-row mask ---
 // out[r] = 1 iff every entry of row r is finite (jnp.isfinite(data).all(-1)
@@ -543,27 +478,13 @@ static int launch_chain(hipStream_t st, const fmc_target* t, const ChainIO& io, 
   return check_launch(KIND == KK_MALA ? "mala chain kernel" : KIND == KK_GRW ? "grw chain kernel" : "hmc chain kernel");
 }
 
-static int check_buffers(const float* out_pos, const float* out_logp, const float* out_acc,
-                         int64_t buf_len, int64_t cursor, int thinning, int step0, int n_steps) {
-  FMC_ARG(thinning >= 1, "thinning must be >= 1");
-  FMC_ARG(step0 >= 0 && n_steps >= 0, "step0 and n_steps must be >= 0");
-  if (out_pos || out_logp || out_acc) {
-    FMC_ARG(buf_len >= 1 && cursor >= 0, "bad buffer length / cursor");
-    if (n_steps > 0) {
-      int64_t last_t = step0 + n_steps - 1;
-      int64_t last_row = cursor + last_t / thinning;
-      FMC_ARG(last_row < buf_len, "buffer row %lld out of range (buf_len %lld)", (long long)last_row,
-              (long long)buf_len);
-    }
-  }
-  return FMC_OK;
-}
-
 }  // namespace fmc
 
 using namespace fmc;
 
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" int fmc_target_check(const fmc_target* target) { return check_target(target); }
 
 extern "C" int fmc_target_eval(void* stream, const fmc_target* target, const float* x, int64_t n_points,
                                float* logp, float* grad, const fmc_launch_cfg* cfg) {
@@ -820,36 +741,6 @@ extern "C" int fmc_hmc_accept(void* stream, float* x, float* logp, const float* 
                      x_new, logp_new, p_new, kinetic0, metric, n_chains, n_dim, key0, key1, step, log_u, out_pos,
                      out_logp, out_acc, buf_len, cursor, thinning);
   return check_launch("hmc accept");
-}
-
-extern "C" int fmc_nf_mh(void* stream, const float* x0, const float* lp0, const float* q0, const float* prop,
-                         const float* lp_prop, const float* q_prop, int64_t n_chains, int32_t n_steps,
-                         int32_t n_dim, uint32_t key0, uint32_t key1, const float* log_u, int32_t* out_idx,
-                         float* x_last, float* lp_last, float* out_pos, float* out_logp, float* out_acc,
-                         int64_t buf_len, int64_t cursor, int32_t thinning) {
-  FMC_ARG(x0 && lp0 && q0 && prop && lp_prop && q_prop && out_idx, "NULL pointer");
-  FMC_ARG(n_chains >= 0 && n_steps >= 1 && n_dim >= 1, "bad sizes");
-  int rc = check_buffers(out_pos, out_logp, out_acc, buf_len, cursor, thinning, 0, n_steps);
-  if (rc) return rc;
-  if (n_chains == 0) return FMC_OK;
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(nf_mh_scan_kernel, dim3(grid1(n_chains, 256)), dim3(256), 0, st, lp0, q0, lp_prop, q_prop,
-                     n_chains, n_steps, key0, key1, log_u, out_idx, lp_last, out_logp, out_acc, buf_len, cursor,
-                     thinning);
-  if ((rc = check_launch("nf mh scan"))) return rc;
-  const int n_rows = (n_steps + thinning - 1) / thinning;
-  if (out_pos) {
-    hipLaunchKernelGGL(nf_mh_gather_kernel, dim3(grid1(n_chains * n_rows, 4)), dim3(256), 0, st, x0, prop,
-                       out_idx, n_chains, n_steps, n_dim, out_pos, buf_len, cursor, thinning, n_rows);
-    if ((rc = check_launch("nf mh gather"))) return rc;
-  }
-  if (x_last) {
-    const int s_last = (n_rows - 1) * thinning;
-    hipLaunchKernelGGL(nf_mh_last_kernel, dim3(grid1(n_chains * n_dim, 256)), dim3(256), 0, st, x0, prop,
-                       out_idx, n_chains, n_steps, n_dim, s_last, x_last);
-    if ((rc = check_launch("nf mh last"))) return rc;
-  }
-  return FMC_OK;
 }
 
 extern "C" int fmc_philox_blocks(void* stream, uint32_t key0, uint32_t key1, uint32_t ctr0, uint32_t ctr1,

@@ -26,6 +26,24 @@ int check_launch(const char* what);
 
 #define FMC_ARG(cond, ...) FMC_REQUIRE(cond, FMC_ERR_INVALID_ARG, __VA_ARGS__)
 
+// Row range of the thinned buffer writes of steps step0 .. step0+n_steps-1:
+// global step t with t % thinning == 0 lands on row cursor + t / thinning.
+static inline int check_buffers(const float* out_pos, const float* out_logp, const float* out_acc,
+                                int64_t buf_len, int64_t cursor, int thinning, int step0, int n_steps) {
+  FMC_ARG(thinning >= 1, "thinning must be >= 1");
+  FMC_ARG(step0 >= 0 && n_steps >= 0, "step0 and n_steps must be >= 0");
+  if (out_pos || out_logp || out_acc) {
+    FMC_ARG(buf_len >= 1 && cursor >= 0, "bad buffer length / cursor");
+    if (n_steps > 0) {
+      int64_t last_t = step0 + n_steps - 1;
+      int64_t last_row = cursor + last_t / thinning;
+      FMC_ARG(last_row < buf_len, "buffer row %lld out of range (buf_len %lld)", (long long)last_row,
+              (long long)buf_len);
+    }
+  }
+  return FMC_OK;
+}
+
 // ---------------------------------------------------------------- Philox ---
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11): 10 rounds of the 4x32 S-box with the published

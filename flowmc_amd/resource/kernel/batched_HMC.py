@@ -56,23 +56,87 @@ class BatchedHMC:
                 raise ValueError(f"vec_per_lane={nv} cannot hold d={d} in 64 lanes")
         return nv, block, lpcs
 
-    def _tables(self, layout, key, n_steps_noise, outputs, noise, log_u):
+    def _check_slot(self, s, out, noise, log_u, n_steps, step0, thinning, n_run):
+        """Host-side checks of slot ``s`` (the kernel reads the slot table from
+        device memory, so nothing on the device side can validate it): every
+        tensor the kernel will touch must have the shape the slot implies and
+        every buffer row it writes must exist. Shapes are checked first, then
+        dtype / device / contiguity (by the caller, once every slot's shapes
+        have passed). Returns (buf_len, cursor, [(name, tensor)])."""
+        t, C_, d = self.targets[s], self.n_chains[s], self.dims[s]
+        # the descriptor's kind / dims / parameter count (host-side check only)
+        desc = _native.fmc_target(t.kind, t.n_dim, t.n_modes, 0, t.params.data_ptr(), t._n_params)
+        _native.call("fmc_target_check", C.byref(desc))
+        if t.n_dim != d:
+            raise ValueError(f"slot {s}: target n_dim {t.n_dim} != {d}")
+
+        def shape_is(name, ten, want):
+            got = tuple(ten.shape)
+            if len(got) != len(want) or any(w is not None and g != w for g, w in zip(got, want)):
+                txt = "[" + ", ".join("T" if w is None else str(w) for w in want) + "]"
+                raise ValueError(f"slot {s}: {name} must be {txt}, got {list(got)}")
+
+        shape_is("state", self.x[s], (C_, d))
+        shape_is("log_prob state", self.logp[s], (C_,))
+        m = self.metrics[s]
+        if m is not None:
+            shape_is("metric", m, (d,))
+        if (noise is None) != (log_u is None):
+            raise ValueError(f"slot {s}: noise and log_u are injected together")
+        if noise is not None:
+            shape_is("noise", noise, (n_steps, C_, d))
+            shape_is("log_u", log_u, (n_steps, C_))
+        pos, lps, accs, cursor = out if out is not None else (None, None, None, 0)
+        bufs = [(n, b) for n, b in (("position buffer", pos), ("log_prob buffer", lps), ("acceptance buffer", accs))
+                if b is not None]
+        lens = set()
+        for name, buf in bufs:
+            shape_is(name, buf, (C_, None, d) if name.startswith("position") else (C_, None))
+            lens.add(int(buf.shape[1]))
+        if len(lens) > 1:
+            raise ValueError(f"slot {s}: output buffers disagree on their length {sorted(lens)}")
+        buf_len, cursor = (lens.pop(), int(cursor)) if lens else (1, 0)
+        if bufs and n_run > 0:
+            last_row = cursor + (step0 + n_run - 1) // thinning
+            if cursor < 0 or last_row >= buf_len:
+                raise ValueError(f"slot {s}: rows {cursor}..{last_row} do not fit a buffer of length {buf_len}")
+        tensors = [(f"slot {s} {name}", ten) for name, ten in
+                   [("state", self.x[s]), ("log_prob state", self.logp[s]), ("metric", m), ("noise", noise),
+                    ("log_u", log_u)] + bufs]
+        return buf_len, cursor, tensors
+
+    def _tables(self, layout, key, n_steps_noise, outputs, noise, log_u, thinning=1, step0=0, n_run=None):
         nv, block, lpcs = layout
+        if thinning < 1:
+            raise ValueError("thinning must be >= 1")
+        if outputs is not None and len(outputs) != len(self.targets):
+            raise ValueError(f"outputs must have one entry per target ({len(self.targets)})")
         for d, l in zip(self.dims, lpcs):
             # a chain's lanes must sit in one wavefront: power of two <= 64
             if l < 1 or l > 64 or l & (l - 1) or block % l or 4 * nv * l < d:
                 raise ValueError(f"lanes_per_chain={l} (vec_per_lane={nv}, block={block}) cannot hold d={d}")
+        n_run = n_steps_noise if n_run is None else n_run
+        checked = []
+        for s in range(len(self.targets)):
+            checked.append(self._check_slot(s, outputs[s] if outputs is not None else None,
+                                            noise[s] if noise is not None else None,
+                                            log_u[s] if log_u is not None else None,
+                                            n_steps_noise, step0, thinning, n_run))
+        for _, _, tensors in checked:
+            for name, ten in tensors:
+                _native.dptr(ten, name=name)
         slots = (_native.fmc_hmc_slot * len(self.targets))()
         items = []
         self._keep = []
         for s, t in enumerate(self.targets):
             k0, k1 = frandom.key_words(frandom.fold_in(key, s))
-            out = outputs[s] if outputs is not None else (None, None, None, 0)
+            out = outputs[s] if outputs is not None else None
             m = self.metrics[s]
             nz = noise[s] if noise is not None else None
             lu = log_u[s] if log_u is not None else None
+            buf_len, cursor, _ = checked[s]
+            out = out if out is not None else (None, None, None, 0)
             self._keep += [m, nz, lu]
-            buf_len = int(out[0].shape[1]) if out[0] is not None else 1
             slots[s].target = t.descriptor()
             slots[s].x = self.x[s].data_ptr()
             slots[s].logp = self.logp[s].data_ptr()
@@ -84,7 +148,7 @@ class BatchedHMC:
             slots[s].out_acc = _native.dptr(out[2], name="acceptance buffer")
             slots[s].n_chains = self.n_chains[s]
             slots[s].buf_len = buf_len
-            slots[s].cursor = int(out[3])
+            slots[s].cursor = cursor
             slots[s].key0, slots[s].key1 = k0, k1
             slots[s].lanes_per_chain = lpcs[s]
             cpb = block // lpcs[s]
@@ -119,7 +183,7 @@ class BatchedHMC:
             ns = min(fused_steps, n_steps - s0)
             nz = [n[s0:s0 + ns] for n in noise] if noise is not None else None
             lu = [u[s0:s0 + ns] for u in log_u] if log_u is not None else None
-            slot_dev, it_dev, n_items = self._tables(layout, key, ns, outputs, nz, lu)
+            slot_dev, it_dev, n_items = self._tables(layout, key, ns, outputs, nz, lu, thinning, s0, ns)
             _native.call("fmc_hmc_run_batched", st, slot_dev.data_ptr(), len(self.targets), it_dev.data_ptr(), n_items,
                          maxd, ns, s0, self.step_size, self.n_leapfrog, thinning, nv, block)
             self._last_tables = (slot_dev, it_dev)  # keep alive until the stream passes them
@@ -134,7 +198,7 @@ class BatchedHMC:
         layout = layout or self.layout or self.default_layout()
         nv, block, lpcs = layout
         maxd = max(self.dims)
-        slot_dev, it_dev, n_items = self._tables(layout, key, fused_steps, outputs, None, None)
+        slot_dev, it_dev, n_items = self._tables(layout, key, fused_steps, outputs, None, None, thinning, 0, n_steps)
         n_slots, eps, n_lf = len(self.targets), self.step_size, self.n_leapfrog
 
         def launch():

@@ -75,6 +75,12 @@ typedef struct fmc_launch_cfg {
   int32_t reserved;
 } fmc_launch_cfg;
 
+/* Host-side validation of a target descriptor (kind, n_dim, n_modes and the
+ * parameter count the kind needs), the check every chain entry point makes
+ * first; no device work. For callers that place descriptors in device
+ * memory (fmc_hmc_run_batched's slots), where the library cannot read them. */
+int fmc_target_check(const fmc_target* target);
+
 /* Value (and optionally gradient) of a built-in target at C points.
  * x [C,d] -> logp [C], grad [C,d] (grad may be NULL). */
 int fmc_target_eval(void* stream, const fmc_target* target, const float* x,

@@ -43,24 +43,34 @@ def mala_step(x, fn, dt, xi, log_u):
 def hmc_step(x, lp, fn, eps, n_leapfrog, metric, xi, log_u):
     """One HMC step — src/flowMC/resource/kernel/HMC.py:98-151.
 
-    metric is the diagonal of the condition matrix (ones for eye);
-    momentum = xi @ chol(inv(M)).T = xi / sqrt(m) (HMC.py:133-136);
-    kinetic = 1/2 sum(p^2 * M) (HMC.py:118-121); leapfrog coefficient table
-    HMC.py:47-50 scanned n_leapfrog+2 times (HMC.py:91-95); H0 uses the
-    passed-in log_prob (HMC.py:137).
+    metric is the condition matrix M: a vector (its diagonal; ones for eye)
+    or a dense [d, d] matrix. Momentum = xi @ chol(inv(M)).T (HMC.py:133-136),
+    i.e. xi / sqrt(m) for a diagonal M; kinetic = 1/2 sum(p^2 * M) with NumPy
+    broadcasting (HMC.py:118-121), which for a dense M is
+    1/2 sum_j p_j^2 * colsum_j(M), so its p-gradient (the drift direction,
+    HMC.py:74-76) is p * colsum(M); leapfrog coefficient table HMC.py:47-50
+    scanned n_leapfrog+2 times (HMC.py:91-95); H0 uses the passed-in
+    log_prob (HMC.py:137).
     """
     coefs = torch.ones(n_leapfrog + 2, 2, dtype=x.dtype)
     coefs[0] = torch.tensor([0.0, 0.5])
     coefs[-1] = torch.tensor([1.0, 0.5])
-    p = xi / torch.sqrt(metric)
+    metric = torch.as_tensor(metric, dtype=x.dtype)
+    if metric.ndim == 2:
+        chol = torch.linalg.cholesky(torch.linalg.inv(metric))
+        p = xi @ chol.T
+        kin_w = metric.sum(0)  # (p**2 * M).sum() = sum_j p_j^2 sum_i M_ij
+    else:
+        p = xi / torch.sqrt(metric)
+        kin_w = metric
 
     def kinetic(p):
-        return 0.5 * (p * p * metric).sum(-1)
+        return 0.5 * (p * p * kin_w).sum(-1)
 
     H = -lp + kinetic(p)
     xl, pl = x, p
     for i in range(n_leapfrog + 2):
-        xl = xl + eps * coefs[i, 0] * (pl * metric)
+        xl = xl + eps * coefs[i, 0] * (pl * kin_w)
         _, g = value_and_grad(fn, xl)
         pl = pl - eps * coefs[i, 1] * (-g)
     lp_prop, _ = value_and_grad(fn, xl)

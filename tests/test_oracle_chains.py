@@ -191,3 +191,89 @@ def test_grw_moments_standard_normal():
     pos, _, _, _, _ = chains.take_serial_steps("grw", x, fn, noise, lu, step_size=1.0)
     tail = pos[:, 100:]
     assert abs(float(tail.mean())) < 3e-2 and abs(float(tail.var()) - 1.0) < 3e-2
+
+
+# ---------------------------------------------------------------------------
+# Non-unit condition matrices (HMC.py:118-136). Two independent pins:
+#   (1) convention: on a flat target no kick moves p, so the L+1 drifts
+#       displace x by eps (L+1) * dK/dp, whose covariance is fixed in closed
+#       form by the momentum law p ~ N(0, M^-1) (the momentum draw
+#       xi @ chol(inv(M)).T) and the broadcast kinetic 1/2 sum(p^2 * M);
+#   (2) consistency: for a diagonal M, one HMC step started from exact draws
+#       of a Gaussian target keeps its mean and variance (the joint
+#       exp(-U - K) has the momentum marginal the step draws from).
+# Each is checked to reject the planted mistake p = xi * sqrt(m).
+
+
+def _hmc_displacement_cov(step, metric, C=200_000, d=4, eps=0.1, L=4, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    x = torch.zeros(C, d)
+    xi = torch.randn(C, d, generator=g)
+    flat = lambda z: 0.0 * z.sum(-1)
+    xn, _, acc, log_acc = step(x, flat(x), flat, eps, L, metric, xi, torch.full((C,), -1e300))
+    assert bool(acc.all()) and float(log_acc.abs().max()) < 1e-12  # p is unchanged: H0 == H1
+    disp = (xn - x) / (eps * (L + 1))
+    return torch.cov(disp.T)
+
+
+def _swapped_momentum_hmc_step(x, lp, fn, eps, n_leapfrog, metric, xi, log_u):
+    # the planted mistake, on the CPU: momentum xi * sqrt(m) instead of xi / sqrt(m)
+    return chains.hmc_step(x, lp, fn, eps, n_leapfrog, metric, xi * metric, log_u)
+
+
+def test_hmc_diagonal_metric_displacement_covariance():
+    m = torch.tensor([0.3, 2.5, 1.0, 4.0])
+    want = torch.diag(m)  # dK/dp = m p with p ~ N(0, diag(1/m)): Cov = M M^-1 M = M
+    got = _hmc_displacement_cov(chains.hmc_step, m)
+    assert torch.allclose(got, want, atol=0.02 * float(m.max()), rtol=0.02)
+    bad = _hmc_displacement_cov(_swapped_momentum_hmc_step, m)
+    assert not torch.allclose(bad, want, atol=0.02 * float(m.max()), rtol=0.02)
+
+
+def test_hmc_dense_metric_displacement_covariance():
+    # dense SPD M: p ~ N(0, M^-1) (its Cholesky factor from numpy, not torch);
+    # the broadcast kinetic's p-gradient is p * colsum(M)
+    rng = np.random.default_rng(4)
+    A = rng.normal(size=(4, 4))
+    M = A @ A.T / 4 + np.diag([0.5, 1.0, 2.0, 3.0])
+    c = M.sum(0)
+    want = torch.tensor(np.diag(c) @ np.linalg.inv(M) @ np.diag(c))
+    Lnp = np.linalg.cholesky(np.linalg.inv(M))
+    assert np.allclose(Lnp @ Lnp.T, np.linalg.inv(M))
+    got = _hmc_displacement_cov(chains.hmc_step, torch.tensor(M))
+    tol = 0.02 * float(want.abs().max())
+    assert torch.allclose(got, want, atol=tol, rtol=0.02)
+    # a diagonal matrix given densely is the diagonal path, bit for bit
+    g = torch.Generator().manual_seed(6)
+    fn = lambda z: T.gauss_diag(z, torch.zeros(4), torch.tensor([1.0, 2.0, 0.5, 1.5]))
+    x = torch.randn(16, 4, generator=g)
+    xi = torch.randn(16, 4, generator=g)
+    lu = torch.log(torch.rand(16, generator=g))
+    md = torch.tensor([0.7, 1.3, 2.0, 0.4])
+    a = chains.hmc_step(x, fn(x), fn, 0.2, 3, md, xi, lu)
+    b = chains.hmc_step(x, fn(x), fn, 0.2, 3, torch.diag(md), xi, lu)
+    assert torch.allclose(a[0], b[0], atol=1e-14) and torch.allclose(a[3], b[3], atol=1e-12)
+
+
+def _one_step_moments(step, m, C=40_000, seed=1):
+    g = torch.Generator().manual_seed(seed)
+    mu = torch.tensor([1.0, -2.0, 0.5, 0.0])
+    sd = torch.tensor([0.5, 2.0, 1.0, 1.5])
+    fn = lambda z: T.gauss_diag(z, mu, 1.0 / sd ** 2)
+    x = mu + sd * torch.randn(C, 4, generator=g)  # exact draws from the target
+    xi = torch.randn(C, 4, generator=g)
+    lu = torch.log(torch.rand(C, generator=g))
+    xn, _, acc, _ = step(x, fn(x), fn, 0.35, 4, m, xi, lu)
+    z = (xn - mu) / sd
+    return z.mean(0), z.var(0), float(acc.float().mean())
+
+
+def test_hmc_diagonal_metric_keeps_gaussian_moments():
+    m = torch.tensor([3.0, 0.4, 2.0, 0.25])
+    mean, var, acc = _one_step_moments(chains.hmc_step, m)
+    # N(0, 1) after standardisation: 5 sigma of the sampling error at C = 40k
+    assert acc > 0.5
+    assert float(mean.abs().max()) < 5 / 40_000 ** 0.5 * 1.5
+    assert float((var - 1).abs().max()) < 5 * (2 / 40_000) ** 0.5 * 1.5
+    _, bvar, _ = _one_step_moments(_swapped_momentum_hmc_step, m)
+    assert float((bvar - 1).abs().max()) > 0.1

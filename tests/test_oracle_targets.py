@@ -83,3 +83,36 @@ def test_autograd_gradient_matches_finite_differences(name):
     fn = fns[name]
     _, ga = T.value_and_grad(fn, x)
     assert torch.allclose(ga, fd_grad(fn, x), rtol=1e-6, atol=1e-6)
+
+
+def _mixture_reference(x, means, sigmas, log_w):
+    # torch.distributions: categorical over modes of independent normals
+    mix = torch.distributions.MixtureSameFamily(
+        torch.distributions.Categorical(logits=log_w),
+        torch.distributions.Independent(torch.distributions.Normal(means, sigmas), 1))
+    return mix.log_prob(x) + 0.5 * x.shape[-1] * math.log(2 * math.pi)
+
+
+def test_gmm_several_modes_matches_mixture_same_family():
+    # K = 5 modes, unequal per-mode / per-dim sigmas, non-uniform weights: pins
+    # the per-mode -sum log sigma_k and the log w_k the K = 1 case cannot see
+    g = torch.Generator().manual_seed(7)
+    K, d = 5, 6
+    means = 3 * torch.randn(K, d, generator=g)
+    sigmas = 0.4 + 1.5 * torch.rand(K, d, generator=g)
+    w = torch.tensor([0.05, 0.4, 0.1, 0.3, 0.15])
+    log_w = torch.log(w)
+    x = means[torch.randint(0, K, (400,), generator=g)] + 1.3 * torch.randn(400, d, generator=g)
+    want = _mixture_reference(x, means, sigmas, log_w)
+    assert torch.allclose(T.gmm_diag(x, means, sigmas, log_w), want, atol=1e-10, rtol=1e-12)
+
+    # planted mistakes (restated on the CPU) must fail the same comparison
+    def dropped_log_sigma(z):
+        zz = (z[..., None, :] - means) / sigmas
+        return torch.logsumexp(log_w - 0.5 * (zz * zz).sum(-1), dim=-1)
+
+    def uniform_weights(z):
+        return T.gmm_diag(z, means, sigmas)
+
+    for bad in (dropped_log_sigma, uniform_weights):
+        assert not torch.allclose(bad(x), want, atol=1e-3)
