@@ -253,11 +253,11 @@ def loss(params, x, K):
     return -log_prob(params, x, K).mean()
 
 
-def init_params(d, L, K, h1, h2, seed=0, dtype=torch.float64, scale=0.0, hidden_var=1.0):
+def init_params(d, L, K, h1, h2, seed=0, dtype=torch.float64, scale=0.0, hidden_var=1.0, out_var=1.0):
     """Random parameters for tests (not the reference's init draws). ``scale``
     sets the ScalarAffine scale/shift magnitude (0 = the reference's init);
     hidden weights are N(0, hidden_var / fan_in) (the reference uses 1e-2,
-    rqSpline.py:431 with common.py:98-100)."""
+    rqSpline.py:431 with common.py:98-100); output weights N(0, out_var / h2)."""
     g = torch.Generator().manual_seed(seed)
     P = 3 * K + 1
     layers = []
@@ -269,7 +269,7 @@ def init_params(d, L, K, h1, h2, seed=0, dtype=torch.float64, scale=0.0, hidden_
             "b1": (torch.rand(h1, generator=g, dtype=dtype) - 0.5) * 0.2,
             "W2": torch.randn(h2, h1, generator=g, dtype=dtype) * math.sqrt(hidden_var / h1),
             "b2": (torch.rand(h2, generator=g, dtype=dtype) - 0.5) * 0.2,
-            "W3": torch.randn(d * P, h2, generator=g, dtype=dtype) * math.sqrt(1.0 / h2),
+            "W3": torch.randn(d * P, h2, generator=g, dtype=dtype) * math.sqrt(out_var / h2),
             "b3": (torch.rand(d * P, generator=g, dtype=dtype) - 0.5) * 0.5,
         })
     return {"layers": layers, "data_mean": torch.zeros(d, dtype=dtype), "data_cov": torch.eye(d, dtype=dtype),

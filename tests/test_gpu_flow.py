@@ -1,13 +1,25 @@
 """GPU parity of the RQ-spline flow kernels against the FP64 oracle.
 
-Tolerances (stated per precision of the conditioner GEMMs):
-  FP32 conditioner: log_prob / log-det / positions within 2e-4 * (1 + |ref|);
-                    parameter gradients within 2e-3 * max|grad| (FP32
-                    accumulation over the batch vs FP64).
-  BF16 conditioner: log_prob within 3e-2 * (1 + |ref|) and positions within
-                    3e-2 * (1 + |ref|) (8-bit mantissa operands, FP32
-                    accumulation, FP32 spline); gradients: cosine similarity
-                    with the FP64 gradient >= 0.999 and relative L2 error <= 5e-2.
+Tolerances (stated per precision of the conditioner GEMMs; errors scaled as
+|a - b| / (1 + |b|)):
+  FP32 conditioner: log_prob / log-det / positions within 2e-4, every
+                    element, at every depth; parameter gradients within
+                    2e-3 * max|grad| (FP32 accumulation over the batch vs FP64).
+  BF16 conditioner, against the oracle that rounds GEMM operands to BF16 as
+                    the kernels do: 99 % of elements within 2e-3 (up to 4
+                    layers) or 1e-2 (8-10 layers: one rounding tie per layer
+                    adds up along the depth), and every element within 3e-2
+                    at full depth and 0.15 for the shallow cases' wilder
+                    parameters (output logits of unit variance: a rounding tie
+                    in a hidden activation moves a steep or nearly flat spline
+                    segment, and the inverse divides by the local slope);
+                    gradients: relative L2 error <= 1e-2 against that oracle,
+                    and cosine similarity >= 0.999 with relative L2 error
+                    <= 5e-2 against the exact FP64 gradient.
+Full-depth cases (8 and 10 layers) draw moderate per-layer parameters
+(ScalarAffine within +-5 %, output weights N(0, 0.1 / h2)): with the shallow
+cases' per-layer magnitudes, ten layers compound into a flow whose FP32
+forward overflows on 3-sigma inputs.
 """
 
 import ctypes as C
@@ -38,12 +50,29 @@ CASES = [  # d, L, K, h1, h2
     (7, 3, 4, 16, 48),
     (6, 2, 12, 32, 32),  # K > 10: chunked spline path
     (6, 2, 8, 320, 64),  # hidden > 256: the wide-hidden adjoint kernel
+    (32, 8, 10, 128, 128),  # the 32-D training flow at full depth
+    (64, 10, 8, 256, 256),  # the 64-D NF-MCMC flow at full depth
 ]
-ADJ_CASES = CASES[:3] + CASES[-2:]
+ADJ_CASES = CASES[:3] + CASES[4:]
 
 
-def make(d, L, K, h1, h2, prec, seed=0, scale=0.3, hidden_var=1.0):
-    p = F.init_params(d, L, K, h1, h2, seed=seed, scale=scale, hidden_var=hidden_var)
+def full_depth(case):
+    return case[1] >= 8
+
+
+def bf16_q99(case):
+    return 1e-2 if full_depth(case) else TOL["bfloat16"]
+
+
+def bf16_max(case):
+    return 3e-2 if full_depth(case) else 0.15
+
+
+def make(d, L, K, h1, h2, prec, seed=0, scale=None, hidden_var=1.0, out_var=None):
+    moderate = full_depth((d, L, K, h1, h2))
+    scale = (0.05 if moderate else 0.3) if scale is None else scale
+    out_var = (0.1 if moderate else 1.0) if out_var is None else out_var
+    p = F.init_params(d, L, K, h1, h2, seed=seed, scale=scale, hidden_var=hidden_var, out_var=out_var)
     g = torch.Generator().manual_seed(seed + 100)
     p["data_mean"] = torch.randn(d, generator=g, dtype=torch.float64) * 0.5
     A = torch.rand(d, generator=g, dtype=torch.float64) + 0.5
@@ -71,15 +100,14 @@ def within(a, b, tol, what="", quantile=1.0):
 @pytest.mark.parametrize("case", CASES)
 def test_flow_passes_match_oracle(gpu, prec, case):
     """FP32: every element within 2e-4 of the FP64 oracle. BF16: against the
-    oracle that rounds the GEMM operands to BF16 as the kernels do, 99% of
-    elements within 2e-3 (the rest are points where a rounding tie or a
-    steep spline segment amplifies a last-bit difference)."""
+    oracle that rounds the GEMM operands to BF16 as the kernels do, the 99 %
+    quantile and every element within the bounds of the module docstring."""
     d, L, K, h1, h2 = case
     m, p = make(d, L, K, h1, h2, prec)
     g = torch.Generator().manual_seed(1)
     x = torch.randn(300, d, generator=g, dtype=torch.float64) * 3  # some samples land in the linear tails
     xf = x.float()
-    tol = TOL[prec]
+    tol = TOL[prec] if prec == "float32" else bf16_q99(case)
     q = 1.0 if prec == "float32" else 0.99
     y, ld = m.forward(xf.cuda())
     xi, ldi = m.inverse(xf.cuda())
@@ -94,18 +122,17 @@ def test_flow_passes_match_oracle(gpu, prec, case):
         rxi, rldi = F.inverse(p, xf.double(), K)
         rlp = F.log_prob(p, xf.double(), K)
         rsmp = F.sample_from_noise(p, z.double(), K)
-    checks = [
-        within(lp, rlp, tol, "log_prob", q),
-        within(y, ry, tol, "forward y", q), within(ld, rld, tol, "forward logdet", q),
-        within(xi, rxi, tol, "inverse x", q), within(ldi, rldi, tol, "inverse logdet", q),
-        within(xe, xf.double(), 10 * tol, "exact inverse round trip", q),
-        within(smp, rsmp, tol, "sample", q),
-    ]
+    pairs = [("log_prob", lp, rlp, tol), ("forward y", y, ry, tol), ("forward logdet", ld, rld, tol),
+             ("inverse x", xi, rxi, tol), ("inverse logdet", ldi, rldi, tol),
+             ("exact inverse round trip", xe, xf.double(), 10 * tol), ("sample", smp, rsmp, tol)]
+    checks = [within(a, b, t, what, q) for what, a, b, t in pairs]
+    if prec == "bfloat16":
+        checks += [within(a, b, bf16_max(case), what + " (all elements)") for what, a, b, _ in pairs]
     assert all(checks), checks
 
 
 @pytest.mark.parametrize("prec", ["float32", "bfloat16"])
-@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("case", CASES[:3] + CASES[-2:])
 def test_exact_inverse_samples_carry_their_log_prob(gpu, prec, case):
     """One exact-inverse pass returns samples and log N(z) - inverse log-det;
     that equals the forward log_prob of the same samples (change of
@@ -119,7 +146,7 @@ def test_exact_inverse_samples_carry_their_log_prob(gpu, prec, case):
     xs = m.sample_from_noise(z.cuda())
     assert float((x - xs).abs().max()) == 0.0
     lf = m.log_prob(x)
-    tol = TOL[prec]
+    tol = TOL[prec] if prec == "float32" else bf16_q99(case)
     q = 1.0 if prec == "float32" else 0.99
     assert within(lq, lf.double().cpu(), 5 * tol, "inverse-pass log q vs forward log_prob", q)
     with (F.emulate_bf16() if prec == "bfloat16" else _null()):
@@ -138,7 +165,7 @@ class _null:
         return False
 
 
-@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("case", CASES[:3] + CASES[-2:])
 def test_flow_bf16_error_vs_fp64_reference_init(gpu, case):
     """BF16 conditioner against the exact FP64 flow for parameters drawn like
     the reference's initialisation (hidden weights N(0, 1e-2/fan_in)): every
@@ -228,7 +255,7 @@ def test_flow_gradient_independent_of_the_backward_schedule(gpu, hidden):
         assert torch.equal(g1, g2) and torch.equal(gx1, gx2)
 
 
-@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("case", CASES[:3] + CASES[-2:])
 def test_flow_bf16_gradient_vs_fp64_reference_init(gpu, case):
     """BF16 adjoint against the exact FP64 gradient at reference-like init."""
     d, L, K, h1, h2 = case
