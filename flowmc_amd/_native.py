@@ -18,6 +18,7 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libflowmc_hip.so"
 
 ABI_VERSION = 1
+NF_MH_NOALLOC_MAX = 1 << 22  # FMC_NF_MH_NOALLOC_MAX of include/flowmc.h
 
 
 class FlowMCNativeError(RuntimeError):
@@ -150,6 +151,7 @@ _SIGNATURES: dict[str, list] = {
     "fmc_hmc_kick": [P, P, P, I64, I32, F32],
     "fmc_hmc_accept": [P, P, P, P, P, P, P, P, I64, I32, U32, U32, I32, P, P, P, P, I64, I64, I32],
     "fmc_nf_mh": [P, P, P, P, P, P, P, I64, I32, I32, U32, U32, P, P, P, P, P, P, P, I64, I64, I32],
+    "fmc_nf_mh_noalloc": [P, P, P, P, P, P, P, I64, I32, I32, U32, U32, P, P, P, P, P, P, I64, I64, I32],
     "fmc_philox_blocks": [P, U32, U32, U32, U32, U32, U32, I64, P],
     "fmc_rng_normal": [P, U32, U32, I64, I32, I32, I32, I32, P],
     "fmc_rng_log_uniform": [P, U32, U32, I64, I32, I32, I32, P],

@@ -1,6 +1,10 @@
 // Independence Metropolis-Hastings over normalizing-flow proposals
 // (NFProposal.kernel's scan, src/flowMC/resource/kernel/NF_proposal.py:91-126)
-// and the C-ABI entry point fmc_nf_mh that runs it.
+// and the two C-ABI entry points that run it: fmc_nf_mh, whose caller passes
+// the [C, S] proposal-index buffer, and fmc_nf_mh_noalloc, which keeps that
+// buffer in a fixed device array of this library instead.
+#include <mutex>
+
 #include "fmc_common.h"
 
 namespace fmc {
@@ -70,6 +74,37 @@ __global__ void nf_mh_last_kernel(const float* __restrict__ x0, const float* __r
   x_last[i] = (id < 0) ? x0[i] : prop[(c * S + id) * d + j];
 }
 
+// Per-(chain, step) proposal indices of fmc_nf_mh_noalloc: a fixed array of
+// FMC_NF_MH_NOALLOC_MAX int32 (16 MiB) on every device the library runs on,
+// held until the process ends. The scan writes every entry it later reads,
+// so nothing needs clearing between calls; calls that overlap on one device
+// share it and race.
+static __device__ __align__(256) int32_t scratch[FMC_NF_MH_NOALLOC_MAX];
+
+// Device address of `scratch` on the current device, taken once per device
+// with hipGetSymbolAddress (the array's name is not a device address in host
+// code).
+static int scratch_address(int32_t** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static int32_t* addr[kMaxDevices] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  FMC_REQUIRE(e == hipSuccess, FMC_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  FMC_REQUIRE(dev >= 0 && dev < kMaxDevices, FMC_ERR_UNSUPPORTED, "device %d beyond the %d this library tracks", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    FMC_REQUIRE(e == hipSuccess && p != nullptr, FMC_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                hipGetErrorString(e));
+    addr[dev] = static_cast<int32_t*>(p);
+  }
+  *out = addr[dev];
+  return FMC_OK;
+}
+
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // Argument checks and the scan / gather / last-row launches shared by both
@@ -124,4 +159,22 @@ extern "C" int fmc_nf_mh(void* stream, const float* x0, const float* lp0, const 
   if (n_chains == 0) return FMC_OK;
   return nf_mh_run((hipStream_t)stream, x0, lp0, q0, prop, lp_prop, q_prop, n_chains, n_steps, n_dim, key0, key1,
                    log_u, out_idx, x_last, lp_last, out_pos, out_logp, out_acc, buf_len, cursor, thinning);
+}
+
+extern "C" int fmc_nf_mh_noalloc(void* stream, const float* x0, const float* lp0, const float* q0,
+                                 const float* prop, const float* lp_prop, const float* q_prop, int64_t n_chains,
+                                 int32_t n_steps, int32_t n_dim, uint32_t key0, uint32_t key1, const float* log_u,
+                                 float* x_last, float* lp_last, float* out_pos, float* out_logp, float* out_acc,
+                                 int64_t buf_len, int64_t cursor, int32_t thinning) {
+  int rc = nf_mh_check(x0, lp0, q0, prop, lp_prop, q_prop, n_chains, n_steps, n_dim, out_pos, out_logp, out_acc,
+                       buf_len, cursor, thinning);
+  if (rc) return rc;
+  FMC_ARG(n_chains <= (int64_t)FMC_NF_MH_NOALLOC_MAX / n_steps,
+          "n_chains * n_steps = %lld exceeds the %d proposal slots of the no-allocation scan",
+          (long long)n_chains * n_steps, (int)FMC_NF_MH_NOALLOC_MAX);
+  int32_t* idx = nullptr;
+  if ((rc = scratch_address(&idx))) return rc;
+  if (n_chains == 0) return FMC_OK;
+  return nf_mh_run((hipStream_t)stream, x0, lp0, q0, prop, lp_prop, q_prop, n_chains, n_steps, n_dim, key0, key1,
+                   log_u, idx, x_last, lp_last, out_pos, out_logp, out_acc, buf_len, cursor, thinning);
 }

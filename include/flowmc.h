@@ -289,6 +289,24 @@ int fmc_nf_mh(void* stream, const float* x0, const float* lp0, const float* q0,
               float* lp_last, float* out_pos, float* out_logp, float* out_acc,
               int64_t buf_len, int64_t cursor, int32_t thinning);
 
+/* fmc_nf_mh without the out_idx argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation). No counterpart in the reference. The [C, S]
+ * proposal indices live in a fixed array of FMC_NF_MH_NOALLOC_MAX int32
+ * (16 MiB) that the library holds on each device it runs on until the
+ * process ends; n_chains * n_steps above that is refused with
+ * FMC_ERR_INVALID_ARG. The call allocates nothing. Calls that overlap on one
+ * device share the array and race: order them on one stream. All other
+ * arguments and outputs are those of fmc_nf_mh; the stream must belong to
+ * the current device. */
+#define FMC_NF_MH_NOALLOC_MAX (1 << 22)
+int fmc_nf_mh_noalloc(void* stream, const float* x0, const float* lp0, const float* q0,
+                      const float* prop, const float* lp_prop, const float* q_prop,
+                      int64_t n_chains, int32_t n_steps, int32_t n_dim, uint32_t key0,
+                      uint32_t key1, const float* log_u, float* x_last, float* lp_last,
+                      float* out_pos, float* out_logp, float* out_acc, int64_t buf_len,
+                      int64_t cursor, int32_t thinning);
+
 /* ------------------------------------------------------------------------ */
 /* Random numbers (the exact streams the fused kernels use).                */
 /* ------------------------------------------------------------------------ */
