@@ -17,7 +17,8 @@ import torch
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libflowmc_hip.so"
 
-ABI_VERSION = 1
+ABI_VERSION = 2
+FLOW_MAX_HIDDEN = 4  # FMC_FLOW_MAX_HIDDEN of include/flowmc.h
 NF_MH_NOALLOC_MAX = 1 << 22  # FMC_NF_MH_NOALLOC_MAX of include/flowmc.h
 
 
@@ -87,8 +88,8 @@ class fmc_flow(C.Structure):
         ("n_dim", C.c_int32),
         ("n_layers", C.c_int32),
         ("n_bins", C.c_int32),
-        ("hidden1", C.c_int32),
-        ("hidden2", C.c_int32),
+        ("n_hidden", C.c_int32),
+        ("hidden", C.c_int32 * 4),  # FMC_FLOW_MAX_HIDDEN
         ("precision", C.c_int32),
         ("range_min", C.c_float),
         ("range_max", C.c_float),
@@ -98,6 +99,17 @@ class fmc_flow(C.Structure):
         ("packed", C.c_void_p),
         ("packed_bias", C.c_void_p),
     ]
+
+
+def make_flow(n_dim, n_layers, n_bins, hidden, precision, range_min=-10.0, range_max=10.0, min_bin_size=1e-4,
+              min_knot_slope=1e-4, theta=None, packed=None, packed_bias=None) -> "fmc_flow":
+    """fmc_flow descriptor with the conditioner's hidden widths given as a list."""
+    hidden = [int(h) for h in hidden]
+    if not 1 <= len(hidden) <= FLOW_MAX_HIDDEN:
+        raise ValueError(f"the flow kernels take 1 to {FLOW_MAX_HIDDEN} hidden layers, got {len(hidden)}")
+    hw = (C.c_int32 * FLOW_MAX_HIDDEN)(*(hidden + [0] * (FLOW_MAX_HIDDEN - len(hidden))))
+    return fmc_flow(n_dim, n_layers, n_bins, len(hidden), hw, precision, range_min, range_max, min_bin_size,
+                    min_knot_slope, theta, packed, packed_bias)
 
 
 class fmc_rnvp(C.Structure):

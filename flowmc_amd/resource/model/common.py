@@ -63,7 +63,8 @@ class Linear:
 
 
 class MLP:
-    """View of the conditioner (common.py:68-124): Linear, tanh, Linear, tanh, Linear."""
+    """View of the conditioner (common.py:68-124): Linear and tanh alternating,
+    one Linear per hidden layer, then the output Linear."""
 
     def __init__(self, linears: list):
         self.layers = []

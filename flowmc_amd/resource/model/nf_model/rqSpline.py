@@ -71,8 +71,9 @@ class MaskedCouplingRQSpline(NFModel):
 
     def __init__(self, n_features: int, n_layers: int, hidden_size: list, num_bins: int, key,
                  spline_range=(-10.0, 10.0), precision: str = "bfloat16", device=None, **kwargs):
-        if len(hidden_size) != 2:
-            raise NotImplementedError("the flow kernels implement conditioners with two hidden layers")
+        if not 1 <= len(hidden_size) <= _native.FLOW_MAX_HIDDEN:
+            raise NotImplementedError(f"the flow kernels implement conditioners with 1 to "
+                                      f"{_native.FLOW_MAX_HIDDEN} hidden layers (got hidden_size={list(hidden_size)})")
         if precision not in _PREC:
             raise ValueError(f"precision must be one of {list(_PREC)}")
         _native.require_gpu()
@@ -108,30 +109,39 @@ class MaskedCouplingRQSpline(NFModel):
         tdt = torch.bfloat16 if self.precision == "bfloat16" else torch.float32
         self.packed = torch.empty(self._packed_size, dtype=tdt, device=self.device)
         self.packed_bias = torch.empty(self._pbias_size, dtype=torch.float32, device=self.device)
-        d, (h1, h2), P = self._n_features, self.hidden_size, 3 * self.num_bins + 1
-        self._layer_size = 2 + h1 * d + h1 + h2 * h1 + h2 + d * P * h2 + d * P
+        self._layer_size = 2 + sum(int(np.prod(shp)) for _, shp in self._leaf_shapes()[2:])
+        d = self._n_features
         off = self.n_layers * self._layer_size
         self._off = {"data_mean": off, "data_cov": off + d, "base_mean": off + d + d * d,
                      "base_cov": off + 2 * d + d * d}
         assert off + 2 * d + 2 * d * d == self._theta_size
 
     def _struct(self, theta_ptr=None) -> _native.fmc_flow:
-        h1, h2 = self.hidden_size
         tp = theta_ptr if theta_ptr is not None else _native.dptr(self.theta, name="theta")
         pk = getattr(self, "packed", None)
         pb = getattr(self, "packed_bias", None)
-        return _native.fmc_flow(self._n_features, self.n_layers, self.num_bins, h1, h2, _PREC[self.precision],
-                                self.spline_range[0], self.spline_range[1], self.min_bin_size, self.min_knot_slope,
-                                tp, pk.data_ptr() if (pk is not None and theta_ptr is None) else None,
-                                pb.data_ptr() if (pb is not None and theta_ptr is None) else None)
+        return _native.make_flow(self._n_features, self.n_layers, self.num_bins, self.hidden_size,
+                                 _PREC[self.precision], self.spline_range[0], self.spline_range[1], self.min_bin_size,
+                                 self.min_knot_slope, tp,
+                                 pk.data_ptr() if (pk is not None and theta_ptr is None) else None,
+                                 pb.data_ptr() if (pb is not None and theta_ptr is None) else None)
+
+    def _leaf_shapes(self):
+        """(name, shape) of one layer's leaves in theta order: scale, shift,
+        then W{i}, b{i} of the conditioner's linear layers (W1 .. Wn hidden,
+        W{n+1} the output layer; eqx.nn.Linear's [out, in] weights)."""
+        d, P = self._n_features, 3 * self.num_bins + 1
+        ins = [d] + self.hidden_size
+        outs = self.hidden_size + [d * P]
+        shapes = [("scale", ()), ("shift", ())]
+        for i, (a, b) in enumerate(zip(ins, outs)):
+            shapes += [(f"W{i + 1}", (b, a)), (f"b{i + 1}", (b,))]
+        return shapes
 
     def view(self, layer: int, name: str) -> torch.Tensor:
         """View of one leaf of layer ``layer`` inside theta."""
-        d, (h1, h2), P = self._n_features, self.hidden_size, 3 * self.num_bins + 1
-        shapes = [("scale", ()), ("shift", ()), ("W1", (h1, d)), ("b1", (h1,)), ("W2", (h2, h1)), ("b2", (h2,)),
-                  ("W3", (d * P, h2)), ("b3", (d * P,))]
         o = layer * self._layer_size
-        for k, shp in shapes:
+        for k, shp in self._leaf_shapes():
             n = int(np.prod(shp)) if shp else 1
             if k == name:
                 return self.theta[o:o + n].view(shp)
@@ -165,8 +175,8 @@ class MaskedCouplingRQSpline(NFModel):
         for i in range(self.n_layers):
             aff = MaskedCouplingLayer(ScalarAffine(self.view(i, "scale"), self.view(i, "shift")),
                                       torch.zeros(d, dtype=torch.bool))
-            mlp = MLP([Linear(self.view(i, "W1"), self.view(i, "b1")), Linear(self.view(i, "W2"), self.view(i, "b2")),
-                       Linear(self.view(i, "W3"), self.view(i, "b3"))])
+            mlp = MLP([Linear(self.view(i, f"W{k}"), self.view(i, f"b{k}"))
+                       for k in range(1, len(self.hidden_size) + 2)])
             mask = ((torch.arange(d) + i) % 2).to(torch.bool)
             out.append([aff, MaskedCouplingLayer(mlp, mask)])
         return out
@@ -177,18 +187,23 @@ class MaskedCouplingRQSpline(NFModel):
         hidden weights N(0, 1e-2 / fan_in), hidden biases and the last layer
         U(-1/sqrt(fan_in), 1/sqrt(fan_in)), ScalarAffine (0, 0). The draws come
         from a torch generator seeded from ``key`` (not JAX's streams)."""
-        d, (h1, h2), P = self._n_features, self.hidden_size, 3 * self.num_bins + 1
+        d = self._n_features
         k0, k1 = frandom.key_words(key)
         g = torch.Generator().manual_seed((k0 << 32) | k1)
+        n_lin = len(self.hidden_size) + 1
         parts = []
         for _ in range(self.n_layers):
             parts += [torch.zeros(2)]
-            parts += [torch.randn(h1, d, generator=g) * math.sqrt(1e-2 / d),
-                      (torch.rand(h1, generator=g) * 2 - 1) / math.sqrt(d)]
-            parts += [torch.randn(h2, h1, generator=g) * math.sqrt(1e-2 / h1),
-                      (torch.rand(h2, generator=g) * 2 - 1) / math.sqrt(h1)]
-            parts += [(torch.rand(d * P, h2, generator=g) * 2 - 1) / math.sqrt(h2),
-                      (torch.rand(d * P, generator=g) * 2 - 1) / math.sqrt(h2)]
+            fan = d
+            for k, shp in self._leaf_shapes()[2:]:
+                if k[0] == "W":  # weights come before their biases: fan is W's fan-in for both
+                    fan = shp[1]
+                    if int(k[1:]) == n_lin:
+                        parts.append((torch.rand(*shp, generator=g) * 2 - 1) / math.sqrt(fan))
+                    else:
+                        parts.append(torch.randn(*shp, generator=g) * math.sqrt(1e-2 / fan))
+                else:
+                    parts.append((torch.rand(*shp, generator=g) * 2 - 1) / math.sqrt(fan))
         dm = kwargs.get("data_mean")
         dc = kwargs.get("data_cov")
         bd = kwargs.get("base_dist")

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define FMC_ABI_VERSION 1
+#define FMC_ABI_VERSION 2
 
 typedef enum fmc_status {
   FMC_OK = 0,
@@ -385,8 +385,13 @@ int fmc_rnvp_weight_grad(void* stream, const fmc_rnvp* flow, int32_t layer, floa
 /* ------------------------------------------------------------------------ */
 typedef enum fmc_precision { FMC_PREC_F32 = 0, FMC_PREC_BF16 = 1 } fmc_precision;
 
+#define FMC_FLOW_MAX_HIDDEN 4
+
 typedef struct fmc_flow {
-  int32_t n_dim, n_layers, n_bins, hidden1, hidden2;
+  int32_t n_dim, n_layers, n_bins;
+  int32_t n_hidden;           /* hidden layers of every conditioner MLP, 1..4   */
+                              /* (hidden_size of rqSpline.py:395, common.py:86-107) */
+  int32_t hidden[FMC_FLOW_MAX_HIDDEN]; /* their widths; entries >= n_hidden unused */
   int32_t precision;          /* fmc_precision of the conditioner GEMMs */
   float range_min, range_max; /* spline range (rqSpline.py:399)        */
   float min_bin_size, min_knot_slope; /* rqSpline.py:299-300           */

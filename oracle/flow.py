@@ -6,8 +6,10 @@ and the mask then keeps the conditioning dims (common.py:150-168); the bin is
 selected with the one-hot formulation of rqSpline.py:63-80.
 
 Parameters come as a dict of per-layer tensors:
-  {"scale","shift","W1","b1","W2","b2","W3","b3"} per layer (eqx.nn.Linear
-  convention y = W x + b), plus "data_mean","data_cov","base_mean","base_cov".
+  {"scale","shift","W1","b1",...,"W{n+1}","b{n+1}"} per layer for a
+  conditioner with n hidden layers (W1 .. Wn hidden, W{n+1} the output
+  layer; eqx.nn.Linear convention y = W x + b), plus "data_mean","data_cov",
+  "base_mean","base_cov".
 """
 
 from __future__ import annotations
@@ -162,16 +164,28 @@ class emulate_bf16:
         _EMULATE_BF16[0] = False
 
 
+def n_linear(lp) -> int:
+    """Linear layers of a conditioner: its hidden layers plus the output layer."""
+    n = 0
+    while f"W{n + 1}" in lp:
+        n += 1
+    return n
+
+
 def mlp(lp, c):
-    """MLP with tanh hidden activations (common.py:109-112, rqSpline.py:428-433)."""
+    """MLP with tanh hidden activations (common.py:109-112, rqSpline.py:428-433):
+    any number of hidden layers (common.py:92-107)."""
+    n = n_linear(lp)
     if not _EMULATE_BF16[0]:
-        h = torch.tanh(c @ lp["W1"].T + lp["b1"])
-        h = torch.tanh(h @ lp["W2"].T + lp["b2"])
-        return h @ lp["W3"].T + lp["b3"]
+        h = c
+        for i in range(1, n):
+            h = torch.tanh(h @ lp[f"W{i}"].T + lp[f"b{i}"])
+        return h @ lp[f"W{n}"].T + lp[f"b{n}"]
     r, rg = _RoundSTE.apply, _RoundGrad.apply
-    h = torch.tanh(rg(r(c) @ r(lp["W1"]).T + lp["b1"]))
-    h = torch.tanh(rg(r(h) @ r(lp["W2"]).T + lp["b2"]))
-    return rg(r(h) @ r(lp["W3"]).T + lp["b3"])
+    h = c
+    for i in range(1, n):
+        h = torch.tanh(rg(r(h) @ r(lp[f"W{i}"]).T + lp[f"b{i}"]))
+    return rg(r(h) @ r(lp[f"W{n}"]).T + lp[f"b{n}"])
 
 
 def coupling(lp, x, layer, K, inverse, rng=(-10.0, 10.0)):
@@ -253,53 +267,71 @@ def loss(params, x, K):
     return -log_prob(params, x, K).mean()
 
 
-def init_params(d, L, K, h1, h2, seed=0, dtype=torch.float64, scale=0.0, hidden_var=1.0, out_var=1.0):
-    """Random parameters for tests (not the reference's init draws). ``scale``
-    sets the ScalarAffine scale/shift magnitude (0 = the reference's init);
-    hidden weights are N(0, hidden_var / fan_in) (the reference uses 1e-2,
-    rqSpline.py:431 with common.py:98-100); output weights N(0, out_var / h2)."""
-    g = torch.Generator().manual_seed(seed)
+def _hidden(h1, h2):
+    """Hidden widths from (h1, h2) or from a list passed as h1 (h2 None)."""
+    if h2 is None:
+        return [int(h) for h in h1]
+    return [int(h1), int(h2)]
+
+
+def _shapes(d, K, hidden):
     P = 3 * K + 1
+    ins = [d] + hidden
+    outs = hidden + [d * P]
+    shp = [("scale", ()), ("shift", ())]
+    for i, (a, b) in enumerate(zip(ins, outs)):
+        shp += [(f"W{i + 1}", (b, a)), (f"b{i + 1}", (b,))]
+    return shp
+
+
+def init_params(d, L, K, h1, h2=None, seed=0, dtype=torch.float64, scale=0.0, hidden_var=1.0, out_var=1.0):
+    """Random parameters for tests (not the reference's init draws). Hidden
+    widths are (h1, h2) or the list h1. ``scale`` sets the ScalarAffine
+    scale/shift magnitude (0 = the reference's init); hidden weights are
+    N(0, hidden_var / fan_in) (the reference uses 1e-2, rqSpline.py:431 with
+    common.py:98-100); output weights N(0, out_var / fan_in)."""
+    hidden = _hidden(h1, h2)
+    g = torch.Generator().manual_seed(seed)
+    n = len(hidden) + 1
     layers = []
     for _ in range(L):
-        layers.append({
-            "scale": (torch.rand((), generator=g, dtype=dtype) - 0.5) * 2 * scale,
-            "shift": (torch.rand((), generator=g, dtype=dtype) - 0.5) * 2 * scale,
-            "W1": torch.randn(h1, d, generator=g, dtype=dtype) * math.sqrt(hidden_var / d),
-            "b1": (torch.rand(h1, generator=g, dtype=dtype) - 0.5) * 0.2,
-            "W2": torch.randn(h2, h1, generator=g, dtype=dtype) * math.sqrt(hidden_var / h1),
-            "b2": (torch.rand(h2, generator=g, dtype=dtype) - 0.5) * 0.2,
-            "W3": torch.randn(d * P, h2, generator=g, dtype=dtype) * math.sqrt(out_var / h2),
-            "b3": (torch.rand(d * P, generator=g, dtype=dtype) - 0.5) * 0.5,
-        })
+        lp = {"scale": (torch.rand((), generator=g, dtype=dtype) - 0.5) * 2 * scale,
+              "shift": (torch.rand((), generator=g, dtype=dtype) - 0.5) * 2 * scale}
+        for k, shp in _shapes(d, K, hidden)[2:]:
+            i = int(k[1:])
+            if k[0] == "W":
+                var = hidden_var if i < n else out_var
+                lp[k] = torch.randn(*shp, generator=g, dtype=dtype) * math.sqrt(var / shp[1])
+            else:
+                lp[k] = (torch.rand(*shp, generator=g, dtype=dtype) - 0.5) * (0.2 if i < n else 0.5)
+        layers.append(lp)
     return {"layers": layers, "data_mean": torch.zeros(d, dtype=dtype), "data_cov": torch.eye(d, dtype=dtype),
             "base_mean": torch.zeros(d, dtype=dtype), "base_cov": torch.eye(d, dtype=dtype)}
 
 
-LEAF_ORDER = ("scale", "shift", "W1", "b1", "W2", "b2", "W3", "b3")
-
-
 def to_flat(params) -> torch.Tensor:
-    """Flatten in the master-parameter order of the device flow."""
+    """Flatten in the master-parameter order of the device flow (the leaf
+    order of the reference's pytree: scale, shift, then weight and bias of
+    every linear layer)."""
     parts = []
     for lp in params["layers"]:
-        parts += [lp[k].reshape(-1) for k in LEAF_ORDER]
+        parts += [lp["scale"].reshape(-1), lp["shift"].reshape(-1)]
+        for i in range(1, n_linear(lp) + 1):
+            parts += [lp[f"W{i}"].reshape(-1), lp[f"b{i}"].reshape(-1)]
     parts += [params["data_mean"].reshape(-1), params["data_cov"].reshape(-1), params["base_mean"].reshape(-1),
               params["base_cov"].reshape(-1)]
     return torch.cat(parts)
 
 
-def from_flat(theta, d, L, K, h1, h2):
-    P = 3 * K + 1
-    shapes = {"scale": (), "shift": (), "W1": (h1, d), "b1": (h1,), "W2": (h2, h1), "b2": (h2,),
-              "W3": (d * P, h2), "b3": (d * P,)}
+def from_flat(theta, d, L, K, h1, h2=None):
+    shapes = _shapes(d, K, _hidden(h1, h2))
     o = 0
     layers = []
     for _ in range(L):
         lp = {}
-        for k in LEAF_ORDER:
-            n = math.prod(shapes[k]) if shapes[k] else 1
-            lp[k] = theta[o:o + n].reshape(shapes[k])
+        for k, shp in shapes:
+            n = math.prod(shp) if shp else 1
+            lp[k] = theta[o:o + n].reshape(shp)
             o += n
         layers.append(lp)
     out = {"layers": layers}

@@ -3,7 +3,10 @@
 Tolerances (stated per precision of the conditioner GEMMs; errors scaled as
 |a - b| / (1 + |b|)):
   FP32 conditioner: log_prob / log-det / positions within 2e-4, every
-                    element, at every depth; parameter gradients within
+                    element, at every depth (the forward-then-exact-inverse
+                    round trip, a property of the flow's conditioning rather
+                    than a comparison with the oracle, within 5e-3: the inverse
+                    divides FP32 rounding by the local slope); parameter gradients within
                     2e-3 * max|grad| (FP32 accumulation over the batch vs FP64).
   BF16 conditioner, against the oracle that rounds GEMM operands to BF16 as
                     the kernels do: 99 % of elements within 2e-3 (up to 4
@@ -16,10 +19,12 @@ Tolerances (stated per precision of the conditioner GEMMs; errors scaled as
                     gradients: relative L2 error <= 1e-2 against that oracle,
                     and cosine similarity >= 0.999 with relative L2 error
                     <= 5e-2 against the exact FP64 gradient.
-Full-depth cases (8 and 10 layers) draw moderate per-layer parameters
-(ScalarAffine within +-5 %, output weights N(0, 0.1 / h2)): with the shallow
-cases' per-layer magnitudes, ten layers compound into a flow whose FP32
-forward overflows on 3-sigma inputs.
+Full-depth cases (8 and 10 layers) and the cases with one, three or four
+hidden layers draw moderate per-layer parameters (ScalarAffine within +-5 %,
+output weights N(0, 0.1 / fan_in)): with the two-hidden-layer shallow cases'
+per-layer magnitudes, ten layers compound into a flow whose FP32 forward
+overflows on 3-sigma inputs, and single draws give nearly flat spline
+segments whose inverse turns a last-bit difference into an O(1) one.
 """
 
 import ctypes as C
@@ -43,15 +48,20 @@ from oracle import optim as O
 pytestmark = pytest.mark.gpu
 
 TOL = {"float32": 2e-4, "bfloat16": 2e-3}
-CASES = [  # d, L, K, h1, h2
-    (5, 4, 8, 32, 32),
-    (32, 3, 10, 128, 128),
-    (64, 2, 8, 256, 256),
-    (7, 3, 4, 16, 48),
-    (6, 2, 12, 32, 32),  # K > 10: chunked spline path
-    (6, 2, 8, 320, 64),  # hidden > 256: the wide-hidden adjoint kernel
-    (32, 8, 10, 128, 128),  # the 32-D training flow at full depth
-    (64, 10, 8, 256, 256),  # the 64-D NF-MCMC flow at full depth
+CASES = [  # d, L, K, conditioner hidden widths
+    (5, 4, 8, (32, 32)),
+    (32, 3, 10, (128, 128)),
+    (64, 2, 8, (256, 256)),
+    (7, 3, 4, (16, 48)),
+    (6, 2, 12, (32, 32)),  # K > 10: chunked spline path
+    (6, 2, 8, (320, 64)),  # hidden > 256: the wide-hidden adjoint kernel
+    (6, 3, 8, (48,)),  # one hidden layer
+    (9, 2, 10, (64, 32, 48)),  # three hidden layers of different widths
+    (16, 2, 8, (128, 128, 128, 128)),  # four hidden layers
+    (8, 2, 12, (32, 32, 32)),  # three hidden layers, chunked spline path
+    (6, 2, 7, (320,)),  # one wide hidden layer, runtime bin count
+    (32, 8, 10, (128, 128)),  # the 32-D training flow at full depth
+    (64, 10, 8, (256, 256)),  # the 64-D NF-MCMC flow at full depth
 ]
 ADJ_CASES = CASES[:3] + CASES[4:]
 
@@ -60,19 +70,27 @@ def full_depth(case):
     return case[1] >= 8
 
 
+def moderate(case):
+    """Cases that draw moderate per-layer parameters: full depth and the
+    conditioner-depth cases (the two-hidden-layer shallow cases keep the
+    wilder parameters their bounds were set for)."""
+    return full_depth(case) or len(case[3]) != 2
+
+
 def bf16_q99(case):
     return 1e-2 if full_depth(case) else TOL["bfloat16"]
 
 
 def bf16_max(case):
-    return 3e-2 if full_depth(case) else 0.15
+    return 3e-2 if moderate(case) else 0.15
 
 
-def make(d, L, K, h1, h2, prec, seed=0, scale=None, hidden_var=1.0, out_var=None):
-    moderate = full_depth((d, L, K, h1, h2))
-    scale = (0.05 if moderate else 0.3) if scale is None else scale
-    out_var = (0.1 if moderate else 1.0) if out_var is None else out_var
-    p = F.init_params(d, L, K, h1, h2, seed=seed, scale=scale, hidden_var=hidden_var, out_var=out_var)
+def make(d, L, K, hidden, prec, seed=0, scale=None, hidden_var=1.0, out_var=None):
+    hidden = list(hidden)
+    mod = moderate((d, L, K, hidden))
+    scale = (0.05 if mod else 0.3) if scale is None else scale
+    out_var = (0.1 if mod else 1.0) if out_var is None else out_var
+    p = F.init_params(d, L, K, hidden, seed=seed, scale=scale, hidden_var=hidden_var, out_var=out_var)
     g = torch.Generator().manual_seed(seed + 100)
     p["data_mean"] = torch.randn(d, generator=g, dtype=torch.float64) * 0.5
     A = torch.rand(d, generator=g, dtype=torch.float64) + 0.5
@@ -80,8 +98,8 @@ def make(d, L, K, h1, h2, prec, seed=0, scale=None, hidden_var=1.0, out_var=None
     p["base_cov"] = torch.eye(d, dtype=torch.float64) * 0.9
     p["base_mean"] = torch.full((d,), 0.1, dtype=torch.float64)
     theta = F.to_flat(p).float()
-    p = F.from_flat(theta.double(), d, L, K, h1, h2)  # oracle on the FP32-rounded parameters
-    m = MaskedCouplingRQSpline(d, L, [h1, h2], K, fr.PRNGKey(0), precision=prec, _theta=theta)
+    p = F.from_flat(theta.double(), d, L, K, hidden)  # oracle on the FP32-rounded parameters
+    m = MaskedCouplingRQSpline(d, L, hidden, K, fr.PRNGKey(0), precision=prec, _theta=theta)
     return m, p
 
 
@@ -102,8 +120,8 @@ def test_flow_passes_match_oracle(gpu, prec, case):
     """FP32: every element within 2e-4 of the FP64 oracle. BF16: against the
     oracle that rounds the GEMM operands to BF16 as the kernels do, the 99 %
     quantile and every element within the bounds of the module docstring."""
-    d, L, K, h1, h2 = case
-    m, p = make(d, L, K, h1, h2, prec)
+    d, L, K, H = case
+    m, p = make(d, L, K, H, prec)
     g = torch.Generator().manual_seed(1)
     x = torch.randn(300, d, generator=g, dtype=torch.float64) * 3  # some samples land in the linear tails
     xf = x.float()
@@ -124,7 +142,7 @@ def test_flow_passes_match_oracle(gpu, prec, case):
         rsmp = F.sample_from_noise(p, z.double(), K)
     pairs = [("log_prob", lp, rlp, tol), ("forward y", y, ry, tol), ("forward logdet", ld, rld, tol),
              ("inverse x", xi, rxi, tol), ("inverse logdet", ldi, rldi, tol),
-             ("exact inverse round trip", xe, xf.double(), 10 * tol), ("sample", smp, rsmp, tol)]
+             ("exact inverse round trip", xe, xf.double(), 25 * tol), ("sample", smp, rsmp, tol)]
     checks = [within(a, b, t, what, q) for what, a, b, t in pairs]
     if prec == "bfloat16":
         checks += [within(a, b, bf16_max(case), what + " (all elements)") for what, a, b, _ in pairs]
@@ -132,13 +150,13 @@ def test_flow_passes_match_oracle(gpu, prec, case):
 
 
 @pytest.mark.parametrize("prec", ["float32", "bfloat16"])
-@pytest.mark.parametrize("case", CASES[:3] + CASES[-2:])
+@pytest.mark.parametrize("case", CASES[:3] + CASES[6:9] + CASES[-2:])
 def test_exact_inverse_samples_carry_their_log_prob(gpu, prec, case):
     """One exact-inverse pass returns samples and log N(z) - inverse log-det;
     that equals the forward log_prob of the same samples (change of
     variables) and the FP64 oracle's log_prob of them."""
-    d, L, K, h1, h2 = case
-    m, p = make(d, L, K, h1, h2, prec, seed=6)
+    d, L, K, H = case
+    m, p = make(d, L, K, H, prec, seed=6)
     m.exact_inverse = True
     g = torch.Generator().manual_seed(7)
     z = torch.randn(400, d, generator=g)
@@ -165,13 +183,13 @@ class _null:
         return False
 
 
-@pytest.mark.parametrize("case", CASES[:3] + CASES[-2:])
+@pytest.mark.parametrize("case", CASES[:3] + CASES[6:9] + CASES[-2:])
 def test_flow_bf16_error_vs_fp64_reference_init(gpu, case):
     """BF16 conditioner against the exact FP64 flow for parameters drawn like
     the reference's initialisation (hidden weights N(0, 1e-2/fan_in)): every
     log_prob within 1e-2 * (1 + |ref|)."""
-    d, L, K, h1, h2 = case
-    m, p = make(d, L, K, h1, h2, "bfloat16", scale=0.05, hidden_var=1e-2)
+    d, L, K, H = case
+    m, p = make(d, L, K, H, "bfloat16", scale=0.05, hidden_var=1e-2)
     g = torch.Generator().manual_seed(11)
     x = (torch.randn(500, d, generator=g) * 2).float()
     assert within(m.log_prob(x.cuda()), F.log_prob(p, x.double(), K), 1e-2, "bf16 log_prob vs fp64")
@@ -180,8 +198,8 @@ def test_flow_bf16_error_vs_fp64_reference_init(gpu, case):
 @pytest.mark.parametrize("prec", ["float32", "bfloat16"])
 @pytest.mark.parametrize("case", ADJ_CASES)
 def test_flow_adjoint_matches_oracle_autograd(gpu, prec, case):
-    d, L, K, h1, h2 = case
-    m, p = make(d, L, K, h1, h2, prec, seed=2)
+    d, L, K, H = case
+    m, p = make(d, L, K, H, prec, seed=2)
     g = torch.Generator().manual_seed(3)
     B = 200
     x = (torch.randn(B, d, generator=g) * 2).float()
@@ -190,7 +208,7 @@ def test_flow_adjoint_matches_oracle_autograd(gpu, prec, case):
     theta = F.to_flat(p).clone().requires_grad_(True)
     xr = x.double().clone().requires_grad_(True)
     with (F.emulate_bf16() if prec == "bfloat16" else _null()):
-        obj = (w.double() * F.log_prob(F.from_flat(theta, d, L, K, h1, h2), xr, K)).sum()
+        obj = (w.double() * F.log_prob(F.from_flat(theta, d, L, K, list(H)), xr, K)).sum()
         rg, rgx = torch.autograd.grad(obj, [theta, xr])
     n_layers_part = m.n_layers * m._layer_size  # stat leaves get no gradient
     gt = gt.cpu().double()
@@ -212,8 +230,8 @@ def test_flow_gradient_is_additive_over_batch_tiles(gpu, prec):
     """The gradient of a 40k-sample batch (1250 adjoint workgroups, several
     per CU, split-K weight gradients over 8 wavefronts) equals the sum of the
     gradients of its two halves, and dL/dx is per sample."""
-    d, L, K, h1, h2 = 64, 2, 8, 256, 256
-    m, _ = make(d, L, K, h1, h2, prec, seed=4)
+    d, L, K, H = 64, 2, 8, (256, 256)
+    m, _ = make(d, L, K, H, prec, seed=4)
     g = torch.Generator().manual_seed(5)
     B = 40000
     x = (torch.randn(B, d, generator=g) * 2).float().cuda()
@@ -235,7 +253,7 @@ def test_flow_gradient_independent_of_the_backward_schedule(gpu, hidden):
     """Weight gradients on a side stream beside the next adjoint (hidden <= 128)
     or on one stream: the same kernels on the same data, so the same bits."""
     d, L, K = 32, 4, 10
-    m, _ = make(d, L, K, hidden, hidden, "bfloat16", seed=6)
+    m, _ = make(d, L, K, (hidden, hidden), "bfloat16", seed=6)
     g = torch.Generator().manual_seed(7)
     B = 10000
     x = (torch.randn(B, d, generator=g) * 2).float().cuda()
@@ -255,16 +273,16 @@ def test_flow_gradient_independent_of_the_backward_schedule(gpu, hidden):
         assert torch.equal(g1, g2) and torch.equal(gx1, gx2)
 
 
-@pytest.mark.parametrize("case", CASES[:3] + CASES[-2:])
+@pytest.mark.parametrize("case", CASES[:3] + CASES[6:9] + CASES[-2:])
 def test_flow_bf16_gradient_vs_fp64_reference_init(gpu, case):
     """BF16 adjoint against the exact FP64 gradient at reference-like init."""
-    d, L, K, h1, h2 = case
-    m, p = make(d, L, K, h1, h2, "bfloat16", seed=2, scale=0.05, hidden_var=1e-2)
+    d, L, K, H = case
+    m, p = make(d, L, K, H, "bfloat16", seed=2, scale=0.05, hidden_var=1e-2)
     g = torch.Generator().manual_seed(3)
     x = (torch.randn(400, d, generator=g) * 2).float()
     gt, _ = m.grad_log_prob_sum(x.cuda(), torch.full((400,), -1.0 / 400).cuda())
     theta = F.to_flat(p).clone().requires_grad_(True)
-    (rg,) = torch.autograd.grad(F.loss(F.from_flat(theta, d, L, K, h1, h2), x.double(), K), theta)
+    (rg,) = torch.autograd.grad(F.loss(F.from_flat(theta, d, L, K, list(H)), x.double(), K), theta)
     n = m.n_layers * m._layer_size
     a, b = gt.cpu().double()[:n], rg[:n]
     cos = float((a * b).sum() / (a.norm() * b.norm()))
@@ -293,15 +311,15 @@ def test_adamw_kernel_matches_oracle(gpu, clip):
 
 
 def test_train_step_matches_oracle_fp32(gpu):
-    d, L, K, h1, h2 = 6, 2, 5, 32, 32
-    m, p = make(d, L, K, h1, h2, "float32", seed=5)
+    d, L, K, H = 6, 2, 5, (32, 32)
+    m, p = make(d, L, K, H, "float32", seed=5)
     g = torch.Generator().manual_seed(6)
     x = (torch.randn(256, d, generator=g) * 1.5).float()
     opt = adamw(1e-3, weight_decay=1e-4, max_norm=1.0)
     state = opt.init(m)
     loss, m2, state = m.train_step(x.cuda(), opt, state)
     theta = F.to_flat(p).clone().requires_grad_(True)
-    rl = F.loss(F.from_flat(theta, d, L, K, h1, h2), x.double(), K)
+    rl = F.loss(F.from_flat(theta, d, L, K, list(H)), x.double(), K)
     (rg,) = torch.autograd.grad(rl, theta)
     th, _, _ = O.adamw_clip_step(theta.detach(), rg, torch.zeros_like(rg), torch.zeros_like(rg), 1, 1e-3,
                                  max_norm=1.0)
@@ -376,8 +394,8 @@ def test_train_model_strategy(gpu):
 
 
 def test_flow_log_prob_autograd_function(gpu):
-    d, L, K, h1, h2 = 5, 3, 6, 32, 32
-    m, p = make(d, L, K, h1, h2, "float32", seed=8)
+    d, L, K, H = 5, 3, 6, (32, 32)
+    m, p = make(d, L, K, H, "float32", seed=8)
     x = (torch.randn(40, d) * 1.5).cuda().requires_grad_(True)
     lp = m.log_prob(x)
     (gx,) = torch.autograd.grad(lp.sum(), x)
@@ -402,3 +420,31 @@ def test_mala_on_a_flow_target(gpu):
                            StepOutputs(pos, lp, ac, 0))
     assert bool(torch.isfinite(pos).all()) and 0.2 < float(ac.mean()) <= 1.0
     assert torch.allclose(lp, m.log_prob(pos), atol=1e-4)
+
+
+@pytest.mark.parametrize("hidden", [[24], [32, 16, 32], [16, 16, 16, 16]])
+def test_training_with_other_conditioner_depths_reduces_loss(gpu, hidden):
+    """hidden_size lists of one, three and four layers train end to end
+    (rqSpline.py:428-433 builds the MLP from any list)."""
+    d = 4
+    g = torch.Generator().manual_seed(17)
+    means = torch.tensor([[2.0, -1.0, 0.0, 1.0], [-2.0, 1.0, 1.0, -1.0]])
+    lab = torch.randint(0, 2, (4000,), generator=g)
+    data = (means[lab] + 0.5 * torch.randn(4000, d, generator=g)).cuda()
+    m = MaskedCouplingRQSpline(d, 3, hidden, 8, fr.PRNGKey(4))
+    mlp = m.layers[0][1].bijector
+    assert [tuple(lin.weight.shape) for lin in mlp.layers if hasattr(lin, "weight")] == [
+        (o, i) for i, o in zip([d] + hidden, hidden + [d * (3 * 8 + 1)])]
+    opt = Optimizer(m, learning_rate=1e-2)
+    rng, best, state, losses = m.train(fr.PRNGKey(2), data, opt.optim, opt.optim_state, 8, 500, verbose=False)
+    assert float(losses[-1]) < float(losses[0])
+
+
+def test_deep_wide_conditioner_adjoint_is_refused(gpu):
+    """Three hidden layers wider than 256: the pass runs, the adjoint kernel
+    refuses them with a message (not a silent fallback)."""
+    m = MaskedCouplingRQSpline(6, 2, [320, 64, 64], 8, fr.PRNGKey(3))
+    x = torch.randn(64, 6, device=gpu)
+    assert bool(torch.isfinite(m.log_prob(x)).all())
+    with pytest.raises(N.FlowMCNativeError, match="at most two hidden layers"):
+        m.grad_log_prob_sum(x)

@@ -10,20 +10,44 @@ import flowmc_amd._native as N
 from oracle import flow as F
 
 
-def sizes(d, L, K, h1, h2, prec=1):
-    f = N.fmc_flow(d, L, K, h1, h2, prec, -10.0, 10.0, 1e-4, 1e-4, 1, None, None)
+def flow(d, L, K, hidden, prec=1, theta=1, packed=None, pbias=None):
+    return N.make_flow(d, L, K, hidden, prec, -10.0, 10.0, 1e-4, 1e-4, theta, packed, pbias)
+
+
+def sizes(d, L, K, h1, h2=None, prec=1):
+    f = flow(d, L, K, [h1, h2] if h2 is not None else h1, prec)
     out = [C.c_int64() for _ in range(4)] + [C.c_int32(), C.c_int32()]
     N.call("fmc_flow_sizes", C.byref(f), *[C.byref(o) for o in out])
     return [o.value for o in out]
 
 
-@pytest.mark.parametrize("case", [(5, 4, 8, 32, 32), (32, 8, 10, 128, 128), (64, 10, 8, 256, 256), (7, 3, 4, 16, 48)])
+@pytest.mark.parametrize("case", [(5, 4, 8, [32, 32]), (32, 8, 10, [128, 128]), (64, 10, 8, [256, 256]),
+                                  (7, 3, 4, [16, 48]), (6, 2, 8, [32]), (9, 3, 8, [64, 32, 48]),
+                                  (8, 2, 12, [32, 32, 32, 32])])
 def test_theta_layout_matches_oracle(case):
-    d, L, K, h1, h2 = case
-    th, pk, pb, rows, tf, tb = sizes(*case)
-    p = F.init_params(d, L, K, h1, h2)
+    d, L, K, hidden = case
+    th, pk, pb, rows, tf, tb = sizes(d, L, K, hidden)
+    p = F.init_params(d, L, K, hidden)
     assert th == F.to_flat(p).numel()
     assert pk > 0 and pb > 0 and rows > 0 and tf in (32, 64) and tb in (32, 64)
+    # store rows: conditioner input, every hidden layer and its gradient, spline-parameter gradients
+    P, dt = 3 * K + 1, d - d // 2
+    n3 = 64 * ((dt + 1) // 2) if K <= 10 else None
+    if n3 is not None:
+        hp = [(h + 31) // 32 * 32 for h in hidden]
+        assert rows == 32 * ((d + 1) // 2 + 31 >= 32) * (((d + 1) // 2 + 31) // 32) + 2 * sum(hp) + n3
+
+
+def test_packed_sizes_with_one_and_three_hidden_layers():
+    # bf16, d=64, K=8: per layer wh(i) / wht(i) images plus the register-order output images
+    _, pk1, pb1, rows1, _, _ = sizes(64, 2, 8, [256])
+    assert pk1 == 2 * (2 * 256 * 32 + 2 * 1024 * 256)
+    assert pb1 == 2 * (256 + 1024)
+    _, pk3, pb3, rows3, _, _ = sizes(64, 2, 8, [256, 128, 64])
+    per = 2 * 256 * 32 + 2 * 128 * 256 + 2 * 64 * 128 + 2 * 1024 * 64
+    assert pk3 == 2 * per
+    assert pb3 == 2 * (256 + 128 + 64 + 1024)
+    assert rows3 == 32 + 2 * (256 + 128 + 64) + 1024
 
 
 def test_packed_sizes_scale_with_transformed_dims():
@@ -47,18 +71,32 @@ def test_packed_sizes_scale_with_transformed_dims():
 
 
 def test_invalid_flow_rejected():
-    f = N.fmc_flow(1, 2, 8, 32, 32, 1, -10.0, 10.0, 1e-4, 1e-4, 1, None, None)
+    f = flow(1, 2, 8, [32, 32])
     z = C.c_int64()
     with pytest.raises(N.FlowMCNativeError, match="n_dim"):
         N.call("fmc_flow_sizes", C.byref(f), C.byref(z), None, None, None, None, None)
-    f = N.fmc_flow(4, 2, 8, 32, 32, 7, -10.0, 10.0, 1e-4, 1e-4, 1, None, None)
+    f = flow(4, 2, 8, [32, 32], prec=7)
     with pytest.raises(N.FlowMCNativeError, match="precision"):
         N.call("fmc_flow_sizes", C.byref(f), C.byref(z), None, None, None, None, None)
-    f = N.fmc_flow(4, 2, 8, 32, 32, 1, -10.0, 10.0, 1e-4, 1e-4, 1, None, None)
+    f = flow(4, 2, 8, [32, 32])
     with pytest.raises(N.FlowMCNativeError, match="packed buffers"):
         N.call("fmc_flow_pass", None, C.byref(f), 0, 0, 1, 4, 0, 0, 0, None, None, None, None, None)
     # the base log-prob of a reference-order inverse pass is refused (its output
     # does not have the forward map's density); checked before any launch
-    f = N.fmc_flow(4, 2, 8, 32, 32, 1, -10.0, 10.0, 1e-4, 1e-4, 8, 16, 24)
+    f = flow(4, 2, 8, [32, 32], packed=16, pbias=24, theta=8)
     with pytest.raises(N.FlowMCNativeError, match="base log-prob"):
         N.call("fmc_flow_pass", None, C.byref(f), 1, 2 | 4, 0, 4, 0, 0, 0, None, None, None, None, None)
+
+
+def test_hidden_layer_count_is_checked():
+    f = flow(4, 2, 8, [32])
+    f.n_hidden = 5
+    with pytest.raises(N.FlowMCNativeError, match="1 to 4 hidden layers"):
+        N.call("fmc_flow_sizes", C.byref(f), None, None, None, None, None, None)
+    f.n_hidden = 0
+    with pytest.raises(N.FlowMCNativeError, match="1 to 4 hidden layers"):
+        N.call("fmc_flow_sizes", C.byref(f), None, None, None, None, None, None)
+    with pytest.raises(N.FlowMCNativeError, match="hidden size 2"):
+        N.call("fmc_flow_sizes", C.byref(flow(4, 2, 8, [32, 32, 600])), None, None, None, None, None, None)
+    with pytest.raises(ValueError):
+        N.make_flow(4, 2, 8, [], 1)
