@@ -59,6 +59,7 @@ class fmc_hmc_slot(C.Structure):
         ("x", C.c_void_p),
         ("logp", C.c_void_p),
         ("metric", C.c_void_p),
+        ("momentum_chol", C.c_void_p),
         ("noise", C.c_void_p),
         ("log_u", C.c_void_p),
         ("out_pos", C.c_void_p),
@@ -141,11 +142,11 @@ _SIGNATURES: dict[str, list] = {
                      C.POINTER(fmc_launch_cfg)],
     "fmc_grw_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, U32, U32, P, P, P, P, P, I64, I64, I32,
                     C.POINTER(fmc_launch_cfg)],
-    "fmc_hmc_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, I32, P, U32, U32, P, P, P, P, P, I64, I64,
-                    I32, C.POINTER(fmc_launch_cfg)],
+    "fmc_hmc_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, I32, P, P, U32, U32, P, P, P, P, P, I64,
+                    I64, I32, C.POINTER(fmc_launch_cfg)],
     "fmc_hmc_run_batched": [P, P, I32, P, I32, I32, I32, I32, F32, I32, I32, I32, I32],
-    "fmc_tempered_run": [P, I32, C.POINTER(fmc_target), P, P, I64, I32, P, I32, I32, F32, I32, P, U32, U32, P, P, P,
-                         I64, C.POINTER(fmc_launch_cfg)],
+    "fmc_tempered_run": [P, I32, C.POINTER(fmc_target), P, P, I64, I32, P, I32, I32, F32, I32, P, P, U32, U32, P, P,
+                         P, I64, C.POINTER(fmc_launch_cfg)],
     "fmc_pt_exchange": [P, P, P, P, P, I64, I32, I32, U32, U32, P, P],
     "fmc_finite_rows": [P, P, I64, I32, P],
     "fmc_rnvp_sizes": [C.POINTER(fmc_rnvp), P, P, P, P],
@@ -158,7 +159,7 @@ _SIGNATURES: dict[str, list] = {
     "fmc_mala_accept": [P, P, P, P, P, P, P, I64, I32, F32, U32, U32, I32, P, P, P, P, I64, I64, I32],
     "fmc_grw_propose": [P, P, P, I64, I32, F32, U32, U32, I32, P],
     "fmc_grw_accept": [P, P, P, P, P, I64, I32, U32, U32, I32, P, P, P, P, I64, I64, I32],
-    "fmc_hmc_momentum": [P, P, P, I64, I32, P, U32, U32, I32, P],
+    "fmc_hmc_momentum": [P, P, P, I64, I32, P, P, U32, U32, I32, P],
     "fmc_hmc_drift": [P, P, P, P, I64, I32, F32],
     "fmc_hmc_kick": [P, P, P, I64, I32, F32],
     "fmc_hmc_accept": [P, P, P, P, P, P, P, P, I64, I32, U32, U32, I32, P, P, P, P, I64, I64, I32],

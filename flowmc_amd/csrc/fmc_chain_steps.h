@@ -26,6 +26,10 @@ struct ChainIO {
   uint32_t key0, key1;
   const float* temps;  // tempered rows: row c targets log p / temps[c % n_temps] (NULL: untempered)
   int n_temps;
+  // HMC with a dense condition matrix M: chol = the lower-triangular
+  // Cholesky factor of inv(M) ([d, d] row-major), and metric = the kinetic
+  // weights colsum(M) (NULL: diagonal or unit metric)
+  const float* chol = nullptr;
 };
 
 struct StepCfg {
@@ -221,15 +225,58 @@ __device__ __forceinline__ void grw_group(const TargetDev& t, const ChainIO& io,
 // ------------------------------------------------------------------- HMC ---
 // HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) with the leapfrog
 // coefficient table of HMC.py:47-50: row 0 = (0, 1/2), rows 1..L = (1, 1),
-// row L+1 = (1, 1/2): L+1 drifts x += eps*c0*metric*p and L+2 kicks
+// row L+1 = (1, 1/2): L+1 drifts x += eps*c0*w*p and L+2 kicks
 // p += eps*c1*grad logp(x). H0 - H1 is accumulated in FP64.
 //
-// UNIT_M: identity metric (io.metric == nullptr). Padding elements of p, x and
-// the gradient are 0 throughout, so that form needs no per-element index
-// test and no metric multiply.
-template <int NV, bool UNIT_M, int TK>
+// Metric kinds MK (the condition matrix M):
+//   MK_UNIT  identity (io.metric == nullptr): padding elements of p, x and
+//            the gradient are 0 throughout, so this form needs no
+//            per-element index test and no metric multiply;
+//   MK_DIAG  diagonal: p = xi / sqrt(m), kinetic 1/2 sum m p^2, drift m p;
+//   MK_DENSE dense: p = xi @ chol(inv(M))^T (HMC.py:133-136), and the
+//            reference's broadcast kinetic 1/2 sum(p^2 * M) =
+//            1/2 sum_j w_j p_j^2 with w = colsum(M) (HMC.py:118-121), whose
+//            p-gradient w p is the drift; io.chol holds the factor, io.metric w.
+enum { MK_UNIT = 0, MK_DIAG = 1, MK_DENSE = 2 };
+
+// p = L xi for the chain held by this lane group (L lower-triangular [d, d]
+// row-major): every lane needs all of its chain's xi, gathered chunk by
+// chunk from the group's lanes with shuffles; padding elements of p are 0.
+template <int NV>
+__device__ __forceinline__ void dense_momentum(const float* __restrict__ Lc, int d, int r, int lpc, int gbase,
+                                               const float4 (&xi)[NV], float4 (&p)[NV]) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) p[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int src = 0; src < lpc; ++src) {
+#pragma unroll
+    for (int vs = 0; vs < NV; ++vs) {
+      const int k0 = (vs * lpc + src) * 4;  // the same for every lane of the wave
+      float xs[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xs[e] = __shfl(f4c(xi[vs], e), gbase + src, 64);
+      if (k0 >= d) continue;
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int j = dim_of(v, e, r, lpc);
+          if (j < d && k0 <= j) {
+            const float* row = Lc + (int64_t)j * d + k0;
+            float acc = f4c(p[v], e);
+#pragma unroll
+            for (int ee = 0; ee < 4; ++ee)
+              if (k0 + ee <= j) acc += row[ee] * xs[ee];
+            f4(p[v], e) = acc;
+          }
+        }
+    }
+  }
+}
+
+template <int NV, int MK, int TK>
 __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io, const StepCfg& cfg, int64_t c,
                           int r, int lpc, int gbase) {
+  constexpr bool UNIT_M = MK == MK_UNIT;
   const int d = t.d;
   float4 x[NV], g[NV], m[NV];
   load_row<NV>(io.x + c * d, d, r, lpc, x);
@@ -247,6 +294,12 @@ __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io,
     const int tg = cfg.step0 + s;
     float4 p[NV], xl[NV], gl[NV];
     draw_normals<NV>(io, d, r, lpc, c, s, tg, RNG_HMC_MOMENTUM, p);
+    if (MK == MK_DENSE) {
+      float4 xi[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) xi[v] = p[v];
+      dense_momentum<NV>(io.chol, d, r, lpc, gbase, xi, p);
+    }
     double k0 = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
@@ -258,7 +311,7 @@ __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io,
         } else {
           int j = dim_of(v, e, r, lpc);
           float mj = f4c(m[v], e);
-          float pj = (j < d) ? f4c(p[v], e) / sqrtf(mj) : 0.f;
+          float pj = (j < d) ? (MK == MK_DENSE ? f4c(p[v], e) : f4c(p[v], e) / sqrtf(mj)) : 0.f;
           f4(p[v], e) = pj;
           k0 += 0.5 * (double)mj * (double)pj * (double)pj;
         }

@@ -7,36 +7,9 @@
 // NV float4 chunks of position / gradient in registers for the whole launch,
 // so the K fused steps touch HBM only for the thinned buffer rows, the noise
 // (injected-noise mode) and the state at entry/exit.
-#include "fmc_chain_steps.h"
+#include "fmc_chain_kernel.h"
 
 namespace fmc {
-
-enum KernelKind { KK_MALA = 0, KK_HMC = 1, KK_GRW = 2 };
-
-// One kernel per (lane layout NV, step kind, target kind TK): the target's
-// value and gradient are inlined without a run-time dispatch. HMC keeps
-// position, momentum, gradient, metric and the leapfrog copies in registers:
-// its workgroups are bounded at 512 threads, which gives it 256 VGPRs
-// instead of the 128 of a 1024-thread bound (spills at NV >= 2).
-template <int NV, int KIND, int TK>
-__global__ void __launch_bounds__(KIND == KK_HMC ? 512 : 1024) chain_kernel(fmc_target tg, ChainIO io, StepCfg cfg,
-                                                     int lpc, int n_lds) {
-  extern __shared__ float smem[];
-  TargetDev t = stage_target(tg, smem, n_lds);
-  const int lane = threadIdx.x & 63;
-  const int r = lane & (lpc - 1);
-  const int gbase = lane - r;
-  const int64_t c = (int64_t)blockIdx.x * (blockDim.x / lpc) + threadIdx.x / lpc;
-  if (c >= io.C) return;  // whole group leaves together
-  if (KIND == KK_MALA)
-    mala_group<NV, TK>(t, io, cfg, c, r, lpc, gbase);
-  else if (KIND == KK_GRW)
-    grw_group<NV, TK>(t, io, cfg, c, r, lpc, gbase);
-  else if (io.metric)
-    hmc_group<NV, false, TK>(t, io, cfg, c, r, lpc, gbase);
-  else
-    hmc_group<NV, true, TK>(t, io, cfg, c, r, lpc, gbase);
-}
 
 // ------------------------------------------------------- target eval ------
 template <int NV>
@@ -163,17 +136,36 @@ __global__ void grw_accept_kernel(float* x, float* logp, const float* __restrict
   }
 }
 
+// One wavefront per chain. Dense condition matrix (chol != NULL): the draw xi
+// is staged in LDS (d floats per wavefront) and p = L xi with L = chol(inv(M))
+// lower-triangular; metric then holds the kinetic weights colsum(M).
 __global__ void hmc_momentum_kernel(float* p, double* kinetic, int64_t C, int d, const float* metric,
-                                    uint32_t k0, uint32_t k1, int step, const float* noise) {
+                                    const float* chol, uint32_t k0, uint32_t k1, int step, const float* noise) {
+  extern __shared__ float xi_lds[];
   const int lane = threadIdx.x & 63;
   const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (c >= C) return;
   double k = 0.0;
-  for (int j = lane; j < d; j += 64) {
-    float mj = metric ? metric[j] : 1.f;
-    float pj = noise_at(noise, k0, k1, c, C, d, j, step, RNG_HMC_MOMENTUM) / sqrtf(mj);
-    p[c * d + j] = pj;
-    k += 0.5 * (double)mj * (double)pj * (double)pj;
+  if (chol) {
+    float* xs = xi_lds + (threadIdx.x >> 6) * d;
+    for (int j = lane; j < d; j += 64) xs[j] = noise_at(noise, k0, k1, c, C, d, j, step, RNG_HMC_MOMENTUM);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int j = lane; j < d; j += 64) {
+      const float* row = chol + (int64_t)j * d;
+      float pj = 0.f;
+      for (int kk = 0; kk <= j; ++kk) pj += row[kk] * xs[kk];
+      const float wj = metric[j];
+      p[c * d + j] = pj;
+      k += 0.5 * (double)wj * (double)pj * (double)pj;
+    }
+  } else {
+    for (int j = lane; j < d; j += 64) {
+      float mj = metric ? metric[j] : 1.f;
+      float pj = noise_at(noise, k0, k1, c, C, d, j, step, RNG_HMC_MOMENTUM) / sqrtf(mj);
+      p[c * d + j] = pj;
+      k += 0.5 * (double)mj * (double)pj * (double)pj;
+    }
   }
   k = wave_sum(k);
   if (lane == 0) kinetic[c] = k;
@@ -379,105 +371,6 @@ __global__ void rng_log_uniform_kernel(uint32_t k0, uint32_t k1, int64_t C, int 
   out[i] = philox_log_uniform(k0, k1, (uint32_t)c, (uint32_t)(step0 + s), sid);
 }
 
-// ----------------------------------------------------------- host side ----
-static int check_target(const fmc_target* t) {
-  FMC_ARG(t != nullptr, "target is NULL");
-  FMC_ARG(t->n_dim >= 1, "target n_dim must be >= 1 (got %d)", t->n_dim);
-  FMC_ARG(t->kind >= FMC_TARGET_GAUSS_DIAG && t->kind <= FMC_TARGET_DUAL_MOON, "unknown target kind %d",
-          t->kind);
-  if (t->kind == FMC_TARGET_GMM_DIAG) FMC_ARG(t->n_modes >= 1, "GMM target needs n_modes >= 1");
-  if (t->kind == FMC_TARGET_DUAL_MOON) FMC_ARG(t->n_dim >= 2, "dual-moon target needs n_dim >= 2");
-  int64_t need = target_param_count(t->kind, t->n_dim, t->n_modes);
-  FMC_ARG(t->n_params >= need, "target kind %d with d=%d needs %lld params, got %lld", t->kind, t->n_dim,
-          (long long)need, (long long)t->n_params);
-  FMC_ARG(need == 0 || t->params != nullptr, "target params pointer is NULL");
-  return FMC_OK;
-}
-
-struct Layout {
-  int lpc, nv, block;
-};
-
-static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
-
-// Heuristic lane layout: aim for ~4 dims per lane for long vectors (one float4
-// chunk, so the fused step's register footprint stays small), but keep whole
-// chains in one lane when d is tiny. Rosenbrock takes two chunks per lane: its
-// three neighbour reads per chunk and the group sum amortise over twice the
-// elements (d = 128: 16 x 2 beats 32 x 1 by 6 % for MALA and 7 % for GRW),
-// while the mixture's per-mode loop is 8 % slower at two chunks per lane.
-static Layout choose_layout(int d, int kind, const fmc_launch_cfg* cfg) {
-  Layout L{0, 0, 256};
-  if (cfg && cfg->lanes_per_chain > 0 && cfg->vec_per_lane > 0) {
-    L.lpc = cfg->lanes_per_chain;
-    L.nv = cfg->vec_per_lane;
-  } else {
-    int chunks = (d + 3) / 4;
-    if (kind == FMC_TARGET_GAUSS_DENSE) {
-      // O(d^2) gather: keep the chain in few lanes.
-      L.lpc = 1;
-      while (L.lpc * 8 < chunks) L.lpc *= 2;
-      L.nv = (chunks + L.lpc - 1) / L.lpc;
-    } else {
-      const int per_lane = (kind == FMC_TARGET_ROSENBROCK) ? 2 : 1;
-      L.lpc = 1;
-      while (L.lpc * per_lane < chunks && L.lpc < 64) L.lpc *= 2;
-      L.nv = (chunks + L.lpc - 1) / L.lpc;
-    }
-    if (L.nv > 2) L.nv = (L.nv <= 4) ? 4 : 8;
-  }
-  if (cfg && cfg->block_size > 0) L.block = cfg->block_size;
-  return L;
-}
-
-static int check_layout(const Layout& L, int d) {
-  FMC_ARG(is_pow2(L.lpc) && L.lpc <= 64, "lanes_per_chain must be a power of two <= 64 (got %d)", L.lpc);
-  FMC_ARG(L.nv == 1 || L.nv == 2 || L.nv == 4 || L.nv == 8, "vec_per_lane must be 1, 2, 4 or 8 (got %d)",
-          L.nv);
-  FMC_ARG(4LL * L.nv * L.lpc >= d, "layout lanes_per_chain=%d vec_per_lane=%d holds %d dims < d=%d", L.lpc,
-          L.nv, 4 * L.nv * L.lpc, d);
-  FMC_ARG(L.block >= 64 && L.block <= 1024 && L.block % 64 == 0, "block_size must be a multiple of 64 in [64,1024]");
-  return FMC_OK;
-}
-
-static int lds_floats(const fmc_target* t) {
-  int64_t n = target_param_count(t->kind, t->n_dim, t->n_modes);
-  return (n > 0 && n * 4 <= 48 * 1024) ? (int)n : 0;
-}
-
-template <int KIND, int TK>
-static void launch_chain_nv(hipStream_t st, const fmc_target* t, const ChainIO& io, const StepCfg& cfg,
-                            const Layout& L, dim3 g, dim3 b, size_t shm, int n_lds) {
-  switch (L.nv) {
-    case 1: hipLaunchKernelGGL((chain_kernel<1, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
-    case 2: hipLaunchKernelGGL((chain_kernel<2, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
-    case 4: hipLaunchKernelGGL((chain_kernel<4, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
-    default: hipLaunchKernelGGL((chain_kernel<8, KIND, TK>), g, b, shm, st, *t, io, cfg, L.lpc, n_lds); break;
-  }
-}
-
-template <int KIND>
-static int launch_chain(hipStream_t st, const fmc_target* t, const ChainIO& io, const StepCfg& cfg,
-                        const Layout& L) {
-  FMC_ARG(KIND != KK_HMC || L.block <= 512, "fused HMC launches take at most 512 threads per workgroup (got %d)",
-          L.block);
-  const int per_block = L.block / L.lpc;
-  const int64_t grid = (io.C + per_block - 1) / per_block;
-  FMC_ARG(grid <= 0x7fffffffLL, "too many chains for one launch");
-  const int n_lds = lds_floats(t);
-  const size_t shm = (size_t)n_lds * sizeof(float);
-  dim3 g((unsigned)grid), b((unsigned)L.block);
-  switch (t->kind) {  // check_target has accepted the kind
-    case FMC_TARGET_GAUSS_DIAG: launch_chain_nv<KIND, FMC_TARGET_GAUSS_DIAG>(st, t, io, cfg, L, g, b, shm, n_lds); break;
-    case FMC_TARGET_GAUSS_DENSE: launch_chain_nv<KIND, FMC_TARGET_GAUSS_DENSE>(st, t, io, cfg, L, g, b, shm, n_lds); break;
-    case FMC_TARGET_ROSENBROCK: launch_chain_nv<KIND, FMC_TARGET_ROSENBROCK>(st, t, io, cfg, L, g, b, shm, n_lds); break;
-    case FMC_TARGET_FUNNEL: launch_chain_nv<KIND, FMC_TARGET_FUNNEL>(st, t, io, cfg, L, g, b, shm, n_lds); break;
-    case FMC_TARGET_GMM_DIAG: launch_chain_nv<KIND, FMC_TARGET_GMM_DIAG>(st, t, io, cfg, L, g, b, shm, n_lds); break;
-    default: launch_chain_nv<KIND, FMC_TARGET_DUAL_MOON>(st, t, io, cfg, L, g, b, shm, n_lds); break;
-  }
-  return check_launch(KIND == KK_MALA ? "mala chain kernel" : KIND == KK_GRW ? "grw chain kernel" : "hmc chain kernel");
-}
-
 }  // namespace fmc
 
 using namespace fmc;
@@ -552,8 +445,8 @@ extern "C" int fmc_grw_run(void* stream, const fmc_target* target, float* x, flo
 extern "C" int fmc_tempered_run(void* stream, int32_t kernel, const fmc_target* target, float* x, float* logp,
                                 int64_t n_chains, int32_t n_temps, const float* temperatures, int32_t n_steps,
                                 int32_t step0, float step_size, int32_t n_leapfrog, const float* metric,
-                                uint32_t key0, uint32_t key1, const float* noise, const float* log_u,
-                                float* out_acc, int64_t acc_len, const fmc_launch_cfg* cfg) {
+                                const float* momentum_chol, uint32_t key0, uint32_t key1, const float* noise,
+                                const float* log_u, float* out_acc, int64_t acc_len, const fmc_launch_cfg* cfg) {
   int rc = check_target(target);
   if (rc) return rc;
   FMC_ARG(kernel == FMC_KERNEL_MALA || kernel == FMC_KERNEL_HMC || kernel == FMC_KERNEL_GRW,
@@ -568,8 +461,11 @@ extern "C" int fmc_tempered_run(void* stream, int32_t kernel, const fmc_target* 
   if (rows == 0) return FMC_OK;
   Layout L = choose_layout(target->n_dim, target->kind, cfg);
   if ((rc = check_layout(L, target->n_dim))) return rc;
+  FMC_ARG(momentum_chol == nullptr || (kernel == FMC_KERNEL_HMC && metric != nullptr),
+          "a momentum Cholesky factor needs the HMC kernel and the kinetic weights in metric");
   ChainIO io{x, logp, kernel == FMC_KERNEL_HMC ? metric : nullptr, noise, log_u, nullptr, nullptr, out_acc,
              rows, acc_len, 0, key0, key1, temperatures, n_temps};
+  io.chol = kernel == FMC_KERNEL_HMC ? momentum_chol : nullptr;
   StepCfg sc{n_steps, step0, step_size, n_leapfrog, 1};
   hipStream_t st = (hipStream_t)stream;
   if (kernel == FMC_KERNEL_MALA) return launch_chain<KK_MALA>(st, target, io, sc, L);
@@ -648,7 +544,8 @@ extern "C" int fmc_grw_accept(void* stream, float* x, float* logp, const float* 
 
 extern "C" int fmc_hmc_run(void* stream, const fmc_target* target, float* x, float* logp, int64_t n_chains,
                            int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
-                           const float* metric, uint32_t key0, uint32_t key1, const float* noise,
+                           const float* metric, const float* momentum_chol, uint32_t key0, uint32_t key1,
+                           const float* noise,
                            const float* log_u, float* out_pos, float* out_logp, float* out_acc,
                            int64_t buf_len, int64_t cursor, int32_t thinning, const fmc_launch_cfg* cfg) {
   int rc = check_target(target);
@@ -661,7 +558,10 @@ extern "C" int fmc_hmc_run(void* stream, const fmc_target* target, float* x, flo
   if (n_chains == 0) return FMC_OK;
   Layout L = choose_layout(target->n_dim, target->kind, cfg);
   if ((rc = check_layout(L, target->n_dim))) return rc;
+  FMC_ARG(momentum_chol == nullptr || metric != nullptr,
+          "a momentum Cholesky factor needs the kinetic weights in metric");
   ChainIO io{x, logp, metric, noise, log_u, out_pos, out_logp, out_acc, n_chains, buf_len, cursor, key0, key1};
+  io.chol = momentum_chol;
   StepCfg sc{n_steps, step0, step_size, n_leapfrog, thinning};
   return launch_chain<KK_HMC>((hipStream_t)stream, target, io, sc, L);
 }
@@ -695,13 +595,16 @@ extern "C" int fmc_mala_accept(void* stream, float* x, float* logp, float* grad,
 }
 
 extern "C" int fmc_hmc_momentum(void* stream, float* p, double* kinetic, int64_t n_chains, int32_t n_dim,
-                                const float* metric, uint32_t key0, uint32_t key1, int32_t step,
-                                const float* noise) {
+                                const float* metric, const float* momentum_chol, uint32_t key0, uint32_t key1,
+                                int32_t step, const float* noise) {
   FMC_ARG(p && kinetic, "NULL pointer");
   FMC_ARG(n_chains >= 0 && n_dim >= 1 && step >= 0, "bad sizes");
+  FMC_ARG(momentum_chol == nullptr || metric != nullptr, "a momentum Cholesky factor needs the kinetic weights");
+  FMC_ARG(momentum_chol == nullptr || n_dim <= 4096, "dense condition matrices need n_dim <= 4096");
   if (n_chains == 0) return FMC_OK;
-  hipLaunchKernelGGL(hmc_momentum_kernel, dim3(grid1(n_chains, 4)), dim3(256), 0, (hipStream_t)stream, p,
-                     kinetic, n_chains, n_dim, metric, key0, key1, step, noise);
+  const size_t shm = momentum_chol ? (size_t)4 * n_dim * sizeof(float) : 0;
+  hipLaunchKernelGGL(hmc_momentum_kernel, dim3(grid1(n_chains, 4)), dim3(256), shm, (hipStream_t)stream, p,
+                     kinetic, n_chains, n_dim, metric, momentum_chol, key0, key1, step, noise);
   return check_launch("hmc momentum");
 }
 

@@ -34,10 +34,13 @@ __global__ void __launch_bounds__(512) hmc_batched_kernel(const fmc_hmc_slot* __
   TargetDev t{sl.target.kind, sl.target.n_dim, sl.target.n_modes, sl.target.params};
   ChainIO io{sl.x,        sl.logp,     sl.metric,  sl.noise,  sl.log_u,  sl.out_pos, sl.out_logp,
              sl.out_acc,  sl.n_chains, sl.buf_len, sl.cursor, sl.key0,   sl.key1};
-  if (io.metric)
-    hmc_group<NV, false, -1>(t, io, cfg, c, r, lpc, gbase);
+  io.chol = sl.momentum_chol;
+  if (io.chol)
+    hmc_group<NV, MK_DENSE, -1>(t, io, cfg, c, r, lpc, gbase);
+  else if (io.metric)
+    hmc_group<NV, MK_DIAG, -1>(t, io, cfg, c, r, lpc, gbase);
   else
-    hmc_group<NV, true, -1>(t, io, cfg, c, r, lpc, gbase);
+    hmc_group<NV, MK_UNIT, -1>(t, io, cfg, c, r, lpc, gbase);
 }
 
 }  // namespace fmc

@@ -13,14 +13,42 @@ from .base import KERNEL_HMC, ProposalBase, StepOutputs, steps_needed
 from .MALA import _prep_state
 
 
+def condition_spec(M, n_dim: int, device):
+    """Kernel form of a condition matrix M (HMC.py:118-136): (kinetic
+    weights [d] or None for the identity, momentum factor [d, d] or None).
+    A scalar c means c * I and a vector its diagonal; a diagonal matrix gives
+    its diagonal; a dense matrix gives (colsum(M), chol(inv(M))), both
+    formed in FP64 on the host and stored in FP32."""
+    if isinstance(M, (int, float)) or torch.as_tensor(M).ndim == 0:
+        c = float(M)
+        return (None, None) if c == 1.0 else (torch.full((n_dim,), c, device=device, dtype=torch.float32), None)
+    Mt = torch.as_tensor(M, dtype=torch.float64).cpu()
+    if Mt.ndim == 1:
+        if Mt.numel() != n_dim:
+            raise ValueError(f"condition_matrix must have {n_dim} diagonal entries, got {Mt.numel()}")
+        return Mt.to(device, torch.float32).contiguous(), None
+    if Mt.ndim != 2:
+        raise ValueError("condition_matrix must be a scalar, a vector or a matrix")
+    if tuple(Mt.shape) != (n_dim, n_dim):
+        raise ValueError(f"condition_matrix must be [{n_dim}, {n_dim}], got {list(Mt.shape)}")
+    off = Mt - torch.diag(torch.diagonal(Mt))
+    if torch.count_nonzero(off) == 0:
+        diag = torch.diagonal(Mt)
+        return (None, None) if bool(torch.all(diag == 1)) else (diag.to(device, torch.float32).contiguous(), None)
+    chol = torch.linalg.cholesky(torch.linalg.inv(Mt))
+    return Mt.sum(0).to(device, torch.float32).contiguous(), chol.to(device, torch.float32).contiguous()
+
+
 class HMC(ProposalBase):
     """Hamiltonian Monte Carlo with the reference's leapfrog schedule
     (n_leapfrog + 1 drifts, n_leapfrog + 2 gradient kicks, HMC.py:47-50,91-95).
 
     ``condition_matrix`` may be a scalar c (c * I), a vector (its diagonal)
-    or a [d, d] matrix; the fused kernels support diagonal matrices (the
-    reference's tests all use the identity). Kinetic energy is
-    1/2 sum_j m_j p_j^2 and the momentum is xi / sqrt(m) (HMC.py:118-136).
+    or a [d, d] matrix. The momentum is xi @ chol(inv(M)).T (HMC.py:133-136),
+    xi / sqrt(m) for a diagonal M, and the kinetic energy the reference's
+    broadcast 1/2 sum(p^2 * M) (HMC.py:118-121): 1/2 sum_j m_j p_j^2 for a
+    diagonal M, 1/2 sum_j colsum_j(M) p_j^2 for a dense one (whose
+    p-gradient colsum(M) * p is then the drift).
     """
 
     step_size: float
@@ -45,23 +73,19 @@ class HMC(ProposalBase):
         self.leapfrog_coefs = coefs  # HMC.py:46-49 table (documentation / tests)
 
     def metric(self, n_dim: int, device) -> torch.Tensor | None:
-        """Diagonal of the condition matrix as a device vector (None = identity)."""
-        M = self.condition_matrix
-        if isinstance(M, (int, float)):
-            return None if float(M) == 1.0 else torch.full((n_dim,), float(M), device=device, dtype=torch.float32)
-        M = torch.as_tensor(M, dtype=torch.float32)
-        if M.ndim == 0:
-            return None if float(M) == 1.0 else torch.full((n_dim,), float(M), device=device, dtype=torch.float32)
-        if M.ndim == 1:
-            return M.to(device).contiguous()
-        if M.ndim == 2:
-            off = M - torch.diag(torch.diagonal(M))
-            if torch.count_nonzero(off) != 0:
-                raise NotImplementedError("dense (non-diagonal) condition matrices are not supported by the "
-                                          "HMC kernels")
-            diag = torch.diagonal(M).contiguous()
-            return None if bool(torch.all(diag == 1)) else diag.to(device)
-        raise ValueError("condition_matrix must be a scalar, a vector or a matrix")
+        """Kinetic-energy weights as a device vector (None = identity): the
+        diagonal of a diagonal condition matrix, colsum(M) of a dense one."""
+        return self.metric_spec(n_dim, device)[0]
+
+    def metric_spec(self, n_dim: int, device):
+        """(kinetic weights [d] or None, momentum factor chol(inv(M)) [d, d]
+        or None) of the condition matrix (see ``condition_spec``)."""
+        key = (int(n_dim), str(device), id(self.condition_matrix))
+        cache = self.__dict__.setdefault("_metric_cache", {})
+        if key not in cache:
+            cache.clear()
+            cache[key] = condition_spec(self.condition_matrix, n_dim, device)
+        return cache[key]
 
     def kernel(self, rng_key, position, log_prob, logpdf, data):
         """One HMC step (HMC.py:98-151); H0 uses the passed-in ``log_prob``."""
@@ -95,7 +119,7 @@ class HMC(ProposalBase):
     def _run_fused(self, target, x, n_run, k0, k1, noise, log_u, out: StepOutputs, thinning):
         C_, d = x.shape
         desc = target.descriptor()
-        m = self.metric(d, x.device)
+        m, chol = self.metric_spec(d, x.device)
         lp = torch.empty(C_, device=x.device, dtype=torch.float32)
         st = _native.stream_ptr()
         cfg = C.byref(self.launch_cfg) if self.launch_cfg is not None else None
@@ -107,7 +131,8 @@ class HMC(ProposalBase):
             _native.call(
                 "fmc_hmc_run", st, C.byref(desc), _native.dptr(x), _native.dptr(lp), C_, ns, s0,
                 float(self.step_size), self.n_leapfrog, _native.dptr(m, name="metric", numel=d if m is not None else None),
-                k0, k1, _native.dptr(nz, name="noise"), _native.dptr(lu, name="log_u"),
+                _native.dptr(chol, name="momentum factor", numel=d * d if chol is not None else None), k0, k1,
+                _native.dptr(nz, name="noise"), _native.dptr(lu, name="log_u"),
                 _native.dptr(out.positions, name="position buffer"), _native.dptr(out.log_probs, name="log_prob buffer"),
                 _native.dptr(out.accepts, name="acceptance buffer"), out.buf_len, out.cursor, thinning, cfg)
         self.last_log_prob = lp
@@ -115,8 +140,9 @@ class HMC(ProposalBase):
     def _run_generic(self, logpdf, data, x, n_run, k0, k1, noise, log_u, out: StepOutputs, thinning, lp_in=None):
         C_, d = x.shape
         st = _native.stream_ptr()
-        m = self.metric(d, x.device)
+        m, chol = self.metric_spec(d, x.device)
         mp = _native.dptr(m, name="metric", numel=d if m is not None else None)
+        cp = _native.dptr(chol, name="momentum factor", numel=d * d if chol is not None else None)
         lp, g = logpdf.value_and_grad(x, data)
         lp, g = lp.contiguous().clone(), g.contiguous().clone()
         if lp_in is not None:
@@ -127,8 +153,8 @@ class HMC(ProposalBase):
         L = self.n_leapfrog
         for s in range(n_run):
             nz = noise[s].contiguous() if noise is not None else None
-            _native.call("fmc_hmc_momentum", st, _native.dptr(p), _native.dptr(k_0, torch.float64), C_, d, mp, k0,
-                         k1, s, _native.dptr(nz, name="noise"))
+            _native.call("fmc_hmc_momentum", st, _native.dptr(p), _native.dptr(k_0, torch.float64), C_, d, mp, cp,
+                         k0, k1, s, _native.dptr(nz, name="noise"))
             xl = x.clone()
             _native.call("fmc_hmc_kick", st, _native.dptr(p), _native.dptr(g), C_, d, eps * 0.5)
             gl = g
@@ -152,7 +178,7 @@ class HMC(ProposalBase):
         self.last_log_prob = lp
 
     def _tempered_launch(self, n_dim, device):
-        return KERNEL_HMC, self.step_size, self.n_leapfrog, self.metric(n_dim, device)
+        return (KERNEL_HMC, self.step_size, self.n_leapfrog, *self.metric_spec(n_dim, device))
 
     def print_parameters(self):
         print("HMC parameters:")

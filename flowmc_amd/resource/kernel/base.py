@@ -69,8 +69,9 @@ class ProposalBase(Resource):
         and return the position of the last thinned row."""
 
 
-    # (fmc_kernel_kind, step_size, n_leapfrog, metric [d] or None) of the fused
-    # tempered launch, or None when the proposal has no fused local step
+    # (fmc_kernel_kind, step_size, n_leapfrog, metric [d] or None[, momentum
+    # factor [d, d] or None]) of the fused tempered launch, or None when the
+    # proposal has no fused local step
     def _tempered_launch(self, n_dim: int, device):
         return None
 
@@ -112,7 +113,8 @@ class ProposalBase(Resource):
                                      noise=noise, log_u=log_u)
             lp = self.last_log_prob
             return rows.reshape(C_, T, d), lp.reshape(C_, T)
-        kind, step_size, n_leapfrog, metric = launch
+        kind, step_size, n_leapfrog, metric, *rest = launch
+        chol = rest[0] if rest else None
         desc = tempered_logpdf.log_pdf.descriptor()
         lp = torch.empty(C_ * T, device=x.device, dtype=torch.float32)
         k0, k1 = frandom.key_words(rng_key)
@@ -128,7 +130,8 @@ class ProposalBase(Resource):
             _native.call("fmc_tempered_run", st, kind, C.byref(desc), _native.dptr(rows, name="positions"),
                          _native.dptr(lp, name="logp"), C_, T, _native.dptr(temps, name="temperatures", numel=T), ns,
                          s0, float(step_size), int(n_leapfrog),
-                         _native.dptr(metric, name="metric", numel=d) if metric is not None else None, k0, k1,
+                         _native.dptr(metric, name="metric", numel=d) if metric is not None else None,
+                         _native.dptr(chol, name="momentum factor", numel=d * d) if chol is not None else None, k0, k1,
                          _native.dptr(nz, name="noise") if nz is not None else None,
                          _native.dptr(lu, name="log_u") if lu is not None else None,
                          _native.dptr(accepts, name="acceptance buffer") if accepts is not None else None, acc_len,

@@ -19,6 +19,7 @@ from ... import _native
 from ... import autotune
 from ... import random as frandom
 from ...targets import Target
+from .HMC import condition_spec
 
 
 def _pow2ceil(v: int) -> int:
@@ -39,6 +40,7 @@ class BatchedHMC:
         self.step_size = float(step_size)
         self.n_leapfrog = int(n_leapfrog)
         self.device = torch.device(device)
+        # per slot: None (identity), a diagonal [d] or a condition matrix [d, d]
         self.metrics = list(metrics) if metrics is not None else [None] * T
         self.x = [torch.zeros((c, d), dtype=torch.float32, device=self.device) for c, d in zip(self.n_chains, self.dims)]
         self.logp = [torch.zeros(c, dtype=torch.float32, device=self.device) for c in self.n_chains]
@@ -55,6 +57,19 @@ class BatchedHMC:
             if 4 * nv * l < d:
                 raise ValueError(f"vec_per_lane={nv} cannot hold d={d} in 64 lanes")
         return nv, block, lpcs
+
+    def _metric(self, s):
+        """Slot s's (kinetic weights, momentum factor): a [d] vector is taken
+        as the diagonal as given (already on the device); a [d, d] matrix goes
+        through HMC's condition_spec (dense: colsum(M), chol(inv(M)))."""
+        m = self.metrics[s]
+        if m is None or m.ndim == 1:
+            return m, None
+        cache = self.__dict__.setdefault("_specs", {})
+        key = (s, id(m))
+        if key not in cache:
+            cache[key] = condition_spec(m, self.dims[s], self.device)
+        return cache[key]
 
     def _check_slot(self, s, out, noise, log_u, n_steps, step0, thinning, n_run):
         """Host-side checks of slot ``s`` (the kernel reads the slot table from
@@ -78,7 +93,7 @@ class BatchedHMC:
 
         shape_is("state", self.x[s], (C_, d))
         shape_is("log_prob state", self.logp[s], (C_,))
-        m = self.metrics[s]
+        m, chol = self._metric(s)
         if m is not None:
             shape_is("metric", m, (d,))
         if (noise is None) != (log_u is None):
@@ -101,7 +116,8 @@ class BatchedHMC:
             if cursor < 0 or last_row >= buf_len:
                 raise ValueError(f"slot {s}: rows {cursor}..{last_row} do not fit a buffer of length {buf_len}")
         tensors = [(f"slot {s} {name}", ten) for name, ten in
-                   [("state", self.x[s]), ("log_prob state", self.logp[s]), ("metric", m), ("noise", noise),
+                   [("state", self.x[s]), ("log_prob state", self.logp[s]), ("metric", m), ("momentum factor", chol),
+                    ("noise", noise),
                     ("log_u", log_u)] + bufs]
         return buf_len, cursor, tensors
 
@@ -131,16 +147,17 @@ class BatchedHMC:
         for s, t in enumerate(self.targets):
             k0, k1 = frandom.key_words(frandom.fold_in(key, s))
             out = outputs[s] if outputs is not None else None
-            m = self.metrics[s]
+            m, chol = self._metric(s)
             nz = noise[s] if noise is not None else None
             lu = log_u[s] if log_u is not None else None
             buf_len, cursor, _ = checked[s]
             out = out if out is not None else (None, None, None, 0)
-            self._keep += [m, nz, lu]
+            self._keep += [m, chol, nz, lu]
             slots[s].target = t.descriptor()
             slots[s].x = self.x[s].data_ptr()
             slots[s].logp = self.logp[s].data_ptr()
             slots[s].metric = _native.dptr(m, name="metric")
+            slots[s].momentum_chol = _native.dptr(chol, name="momentum factor")
             slots[s].noise = _native.dptr(nz, name="noise")
             slots[s].log_u = _native.dptr(lu, name="log_u")
             slots[s].out_pos = _native.dptr(out[0], name="position buffer")

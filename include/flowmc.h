@@ -133,14 +133,20 @@ int fmc_grw_run(void* stream, const fmc_target* target, float* x, float* logp,
 /* Replaces HMC.kernel (src/flowMC/resource/kernel/HMC.py:98-151) with its  */
 /* leapfrog scan (HMC.py:71-96) and coefficient table (HMC.py:47-50:        */
 /* n_leapfrog+1 drifts, n_leapfrog+2 kicks), scanned by TakeSerialSteps.    */
-/* metric: NULL = identity, else diag(metric[d]) (the condition matrix);    */
+/* metric: NULL = identity, else diag(metric[d]) (the condition matrix M);  */
 /* momentum p = xi / sqrt(metric), kinetic = 1/2 sum metric p^2.            */
+/* Dense M: momentum_chol = chol(inv(M)) [d, d] row-major lower triangle and */
+/* metric = colsum(M): p = xi @ momentum_chol^T (HMC.py:133-136) and the    */
+/* reference's broadcast kinetic 1/2 sum(p^2 * M) = 1/2 sum_j colsum_j p_j^2 */
+/* (HMC.py:118-121), whose p-gradient colsum * p is the drift. NULL when M  */
+/* is diagonal.                                                             */
 /* The Hamiltonian difference H0 - H1 is formed in FP64.                    */
 /* Noise: noise[(s*C + c)*d + j] is the momentum draw, log_u[s*C + c].      */
 /* ------------------------------------------------------------------------ */
 int fmc_hmc_run(void* stream, const fmc_target* target, float* x, float* logp,
                 int64_t n_chains, int32_t n_steps, int32_t step0, float step_size,
-                int32_t n_leapfrog, const float* metric, uint32_t key0, uint32_t key1,
+                int32_t n_leapfrog, const float* metric, const float* momentum_chol,
+                uint32_t key0, uint32_t key1,
                 const float* noise, const float* log_u, float* out_pos, float* out_logp,
                 float* out_acc, int64_t buf_len, int64_t cursor, int32_t thinning,
                 const fmc_launch_cfg* cfg);
@@ -163,8 +169,9 @@ int fmc_finite_rows(void* stream, const float* data, int64_t n_rows, int32_t n_d
 /* with a zero log prior: log p(x) / temperatures[r % n_temps]. logp [rows] */
 /* receives the tempered log-density of the final state (may be NULL);      */
 /* out_acc [rows, acc_len] (may be NULL) the decision of step step0 + s at  */
-/* column step0 + s. Noise layout, Philox streams, metric and step_size     */
-/* are those of fmc_mala_run / fmc_hmc_run / fmc_grw_run over the rows.     */
+/* column step0 + s. Noise layout, Philox streams, metric, momentum_chol   */
+/* and step_size are those of fmc_mala_run / fmc_hmc_run / fmc_grw_run over */
+/* the rows (momentum_chol: HMC only).                                      */
 /* ------------------------------------------------------------------------ */
 typedef enum fmc_kernel_kind {
   FMC_KERNEL_MALA = 0,
@@ -175,8 +182,8 @@ typedef enum fmc_kernel_kind {
 int fmc_tempered_run(void* stream, int32_t kernel, const fmc_target* target, float* x, float* logp,
                      int64_t n_chains, int32_t n_temps, const float* temperatures, int32_t n_steps,
                      int32_t step0, float step_size, int32_t n_leapfrog, const float* metric,
-                     uint32_t key0, uint32_t key1, const float* noise, const float* log_u,
-                     float* out_acc, int64_t acc_len, const fmc_launch_cfg* cfg);
+                     const float* momentum_chol, uint32_t key0, uint32_t key1, const float* noise,
+                     const float* log_u, float* out_acc, int64_t acc_len, const fmc_launch_cfg* cfg);
 
 /* Replaces ParallelTempering._exchange (parallel_tempering.py:291-372) for */
 /* all chains: per chain, adjacent pairs i = 0 .. n_temps-2 in order, swap  */
@@ -204,7 +211,8 @@ typedef struct fmc_hmc_slot {
   fmc_target target;
   float* x;        /* [C_t, d_t] */
   float* logp;     /* [C_t]      */
-  const float* metric; /* [d_t] or NULL */
+  const float* metric; /* [d_t] or NULL (kinetic weights colsum(M) with momentum_chol) */
+  const float* momentum_chol; /* [d_t, d_t] chol(inv(M)) of a dense M, or NULL */
   const float* noise;  /* [n_steps, C_t, d_t] or NULL */
   const float* log_u;  /* [n_steps, C_t] or NULL */
   float* out_pos;  /* [C_t, buf_len, d_t] or NULL */
@@ -257,10 +265,11 @@ int fmc_grw_accept(void* stream, float* x, float* logp, const float* prop, const
                    const float* log_u, float* out_pos, float* out_logp, float* out_acc,
                    int64_t buf_len, int64_t cursor, int32_t thinning);
 /* HMC pieces (HMC.py:128-149): momentum draw + kinetic energy (FP64),
- * drift x += a * metric * p, kick p -= b * grad, and the FP64 accept. */
+ * drift x += a * metric * p, kick p -= b * grad, and the FP64 accept.
+ * momentum_chol / metric as for fmc_hmc_run (dense M: n_dim <= 4096). */
 int fmc_hmc_momentum(void* stream, float* p, double* kinetic, int64_t n_chains,
-                     int32_t n_dim, const float* metric, uint32_t key0, uint32_t key1,
-                     int32_t step, const float* noise);
+                     int32_t n_dim, const float* metric, const float* momentum_chol,
+                     uint32_t key0, uint32_t key1, int32_t step, const float* noise);
 int fmc_hmc_drift(void* stream, float* x, const float* p, const float* metric,
                   int64_t n_elem_chains, int32_t n_dim, float coef);
 int fmc_hmc_kick(void* stream, float* p, const float* grad_logp, int64_t n_chains,

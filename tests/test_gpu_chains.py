@@ -122,13 +122,25 @@ def test_target_value_and_grad(gpu, name, d, layout):
     assert ((gr.cpu().double() - g_ref).abs() <= tol_g * scale_g).all()
 
 
-def _compare_traj(kind, name, d, C_, S, dt, thinning=1, n_leapfrog=5, seed=0):
-    tgt, fn = make_target(name, d, seed=seed)
+def dense_condition_matrix(d, g):
+    """An SPD condition matrix with off-diagonal couplings and positive
+    column sums (the kinetic weights of the reference's broadcast kinetic)."""
+    A = torch.randn(d, d, generator=g, dtype=torch.float64)
+    S = (A @ A.T) / d
+    M = torch.diag(0.5 + torch.rand(d, generator=g, dtype=torch.float64)) + 0.2 * S + 0.05
+    return M
+
+
+def _compare_traj(kind, name, d, C_, S, dt, thinning=1, n_leapfrog=5, seed=0, dense=False, target=None):
+    tgt, fn = make_target(name, d, seed=seed) if target is None else target
     g = torch.Generator().manual_seed(seed + 11)
     x0 = torch.randn(C_, d, generator=g, dtype=torch.float64) * 0.5
     noise = torch.randn(S, C_, d, generator=g, dtype=torch.float64)
     log_u = torch.log(torch.rand(S, C_, generator=g, dtype=torch.float64))
     metric = 0.5 + torch.rand(d, generator=g, dtype=torch.float64) if kind == "hmc" else None
+    if dense:
+        metric = dense_condition_matrix(d, g)
+        assert bool((metric.sum(0) > 0).all())
     n_rows = (S + thinning - 1) // thinning
     pos = torch.full((C_, n_rows + 3, d), float("nan"), device="cuda", dtype=torch.float32)
     lps = torch.full((C_, n_rows + 3), float("nan"), device="cuda", dtype=torch.float32)
@@ -605,3 +617,80 @@ def test_batched_hmc_keeps_the_funnel_law_at_full_size(gpu):
     assert abs(float(u2.mean()) - 1) < 5 * float(u2.std()) / n ** 0.5
     lp = torch.cat([lp.double().cpu() for lp in bh.logp])
     assert bool(torch.isfinite(lp).all())
+
+
+@pytest.mark.parametrize("name,d", [("gauss_dense", 5), ("gmm", 16), ("funnel", 40), ("rosenbrock", 9)])
+def test_hmc_dense_condition_matrix_matches_oracle(gpu, name, d):
+    """Dense condition matrix M (HMC.py:118-136): momentum xi @ chol(inv(M)).T,
+    broadcast kinetic 1/2 sum_j colsum_j(M) p_j^2, fused kernel against the
+    FP64 oracle on injected noise."""
+    dt = {"gauss_dense": 0.2, "gmm": 0.1, "funnel": 0.03, "rosenbrock": 0.02}[name]
+    acc = _compare_traj("hmc", name, d, 96, 16, dt, n_leapfrog=6, dense=True)
+    assert float(acc.mean()) > 0.05
+
+
+def test_hmc_dense_condition_matrix_generic_target_matches_oracle(gpu):
+    """The per-step path (a user target differentiated by torch) with a dense
+    condition matrix: fmc_hmc_momentum's staged-LDS product against the oracle."""
+    d = 7
+    g = torch.Generator().manual_seed(5)
+    mean = torch.randn(d, generator=g, dtype=torch.float64)
+    prec = 0.5 + torch.rand(d, generator=g, dtype=torch.float64)
+
+    def user_logpdf(x, data):
+        return -0.5 * torch.sum(data["prec"] * (x - data["mean"]) ** 2)
+
+    fn = lambda z: OT.gauss_diag(z, mean, prec)
+    data = {"mean": mean.float().cuda(), "prec": prec.float().cuda()}
+    C_, S, dt, L = 64, 10, 0.25, 5
+    x0 = torch.randn(C_, d, generator=g, dtype=torch.float64) * 0.5
+    noise = torch.randn(S, C_, d, generator=g, dtype=torch.float64)
+    log_u = torch.log(torch.rand(S, C_, generator=g, dtype=torch.float64))
+    M = dense_condition_matrix(d, g).float().double()
+    pos = torch.zeros(C_, S, d, device=gpu)
+    lps = torch.zeros(C_, S, device=gpu)
+    accs = torch.zeros(C_, S, device=gpu)
+    HMC(M.float(), dt, L).sample_steps(fr.PRNGKey(0), x0.float().cuda(), LogPDF(user_logpdf, n_dims=d), data, S, 1,
+                                      StepOutputs(pos, lps, accs, 0), noise=noise.float().cuda(),
+                                      log_u=log_u.float().cuda())
+    r_pos, r_lp, r_acc, _, ratios = OC.take_serial_steps("hmc", x0.float().double(), fn, noise.float().double(),
+                                                         log_u.float().double(), 1, step_size=dt, n_leapfrog=L,
+                                                         metric=M)
+    ok = ~((ratios - log_u.float().double().T).abs() < MARGIN).any(1)
+    assert int((~ok).sum()) <= 4
+    assert torch.equal(accs.cpu().double()[ok], r_acc[ok])
+    assert ((pos.cpu().double()[ok] - r_pos[ok]).abs() <= 2e-4 * (1 + r_pos[ok].abs())).all()
+    assert 0.05 < float(r_acc.mean())
+
+
+def test_batched_hmc_dense_condition_matrix_matches_oracle(gpu):
+    """A batch mixing unit, diagonal and dense condition matrices."""
+    from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
+
+    dims = [9, 24, 40]
+    g = torch.Generator().manual_seed(12)
+    metrics = [None, 0.5 + torch.rand(24, generator=g, dtype=torch.float64), dense_condition_matrix(40, g)]
+    C_, S, eps, L = 48, 8, 0.04, 10
+    x0 = [torch.randn(C_, d, generator=g, dtype=torch.float64) * 0.5 for d in dims]
+    noise = [torch.randn(S, C_, d, generator=g, dtype=torch.float64) for d in dims]
+    log_u = [torch.log(torch.rand(S, C_, generator=g, dtype=torch.float64)) for _ in dims]
+    bh = BatchedHMC([TG.Funnel(d) for d in dims], C_, step_size=eps, n_leapfrog=L,
+                    metrics=[None, metrics[1].float().cuda(), metrics[2].float()])
+    for x, xi in zip(bh.x, x0):
+        x.copy_(xi.float())
+    outs = [(torch.zeros(C_, S, d, device=gpu), torch.zeros(C_, S, device=gpu), torch.zeros(C_, S, device=gpu), 0)
+            for d in dims]
+    bh.run(fr.PRNGKey(0), S, outputs=outs, noise=[n.float().cuda() for n in noise],
+           log_u=[u.float().cuda() for u in log_u], fused_steps=4)
+    torch.cuda.synchronize()
+    for i, d in enumerate(dims):
+        m = torch.ones(d, dtype=torch.float64) if metrics[i] is None else metrics[i].float().double()
+        r_pos, _, r_acc, _, ratios = OC.take_serial_steps("hmc", x0[i].float().double(), OT.funnel,
+                                                          noise[i].float().double(), log_u[i].float().double(), 1,
+                                                          step_size=eps, n_leapfrog=L, metric=m)
+        ok = ~((ratios - log_u[i].float().double().T).abs() < MARGIN).any(1)
+        assert int((~ok).sum()) <= 4
+        ga = outs[i][2].cpu().double()
+        assert torch.equal(ga[ok], r_acc[ok]), f"d={d}"
+        gp = outs[i][0].cpu().double()
+        assert ((gp[ok] - r_pos[ok]).abs() <= 2e-4 * (1 + r_pos[ok].abs())).all(), f"d={d}"
