@@ -92,6 +92,7 @@ class fmc_flow(C.Structure):
         ("n_hidden", C.c_int32),
         ("hidden", C.c_int32 * 4),  # FMC_FLOW_MAX_HIDDEN
         ("precision", C.c_int32),
+        ("flags", C.c_int32),
         ("range_min", C.c_float),
         ("range_max", C.c_float),
         ("min_bin_size", C.c_float),
@@ -102,14 +103,17 @@ class fmc_flow(C.Structure):
     ]
 
 
+FLOW_DENSE_BASE = 1  # FMC_FLOW_DENSE_BASE
+
+
 def make_flow(n_dim, n_layers, n_bins, hidden, precision, range_min=-10.0, range_max=10.0, min_bin_size=1e-4,
-              min_knot_slope=1e-4, theta=None, packed=None, packed_bias=None) -> "fmc_flow":
+              min_knot_slope=1e-4, theta=None, packed=None, packed_bias=None, flags=0) -> "fmc_flow":
     """fmc_flow descriptor with the conditioner's hidden widths given as a list."""
     hidden = [int(h) for h in hidden]
     if not 1 <= len(hidden) <= FLOW_MAX_HIDDEN:
         raise ValueError(f"the flow kernels take 1 to {FLOW_MAX_HIDDEN} hidden layers, got {len(hidden)}")
     hw = (C.c_int32 * FLOW_MAX_HIDDEN)(*(hidden + [0] * (FLOW_MAX_HIDDEN - len(hidden))))
-    return fmc_flow(n_dim, n_layers, n_bins, len(hidden), hw, precision, range_min, range_max, min_bin_size,
+    return fmc_flow(n_dim, n_layers, n_bins, len(hidden), hw, precision, flags, range_min, range_max, min_bin_size,
                     min_knot_slope, theta, packed, packed_bias)
 
 

@@ -344,6 +344,38 @@ __global__ void __launch_bounds__(256) flow_pack_kernel(FlowDev f, PackTable tab
   }
 }
 
+// ------------------------------------------------ base Gaussian factor ---
+// chol(base_cov) for a full-covariance base Gaussian (Gaussian.log_prob /
+// sample, common.py:285-293), one workgroup, column by column in global
+// memory (left-looking, FP64 accumulation): L [d x d] row-major with zeros
+// above the diagonal, then sum_j log L_jj. A covariance that is not
+// positive definite gives NaN entries, as the reference's factorisation does.
+__global__ void __launch_bounds__(256) base_chol_kernel(const float* __restrict__ cov, int d, float* L) {
+  __shared__ double diag_s;
+  for (int j = 0; j < d; ++j) {
+    if (threadIdx.x == 0) {
+      double s = cov[(int64_t)j * d + j];
+      for (int k = 0; k < j; ++k) s -= (double)L[(int64_t)j * d + k] * L[(int64_t)j * d + k];
+      diag_s = sqrt(s);
+      L[(int64_t)j * d + j] = (float)diag_s;
+    }
+    __syncthreads();
+    const double ljj = diag_s;
+    for (int i = j + 1 + threadIdx.x; i < d; i += 256) {
+      double s = cov[(int64_t)i * d + j];
+      for (int k = 0; k < j; ++k) s -= (double)L[(int64_t)i * d + k] * L[(int64_t)j * d + k];
+      L[(int64_t)i * d + j] = (float)(s / ljj);
+    }
+    for (int i = threadIdx.x; i < j; i += 256) L[(int64_t)i * d + j] = 0.f;  // upper triangle of column j
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double lg = 0.0;
+    for (int j = 0; j < d; ++j) lg += log((double)L[(int64_t)j * d + j]);
+    L[(int64_t)d * d] = (float)lg;
+  }
+}
+
 // --------------------------------------------------------------- AdamW ---
 __global__ void __launch_bounds__(256) sumsq_kernel(const float* __restrict__ g, int64_t n, double* part) {
   double s = 0.0;
@@ -433,7 +465,7 @@ extern "C" int fmc_flow_sizes(const fmc_flow* fl, int64_t* theta_size, int64_t* 
   const FlowDims& dm = fd.dm;
   if (theta_size) *theta_size = dm.th_total();
   if (packed_size) *packed_size = dm.pk_total();
-  if (packed_bias_size) *packed_bias_size = dm.pb_total();
+  if (packed_bias_size) *packed_bias_size = dm.pb_alloc();
   if (store_rows) *store_rows = dm.st_rows(0) > dm.st_rows(1) ? dm.st_rows(0) : dm.st_rows(1);
   size_t s1 = 0, s2 = 0;
   int mbf = pass_mb(dm, fl->precision == FMC_PREC_BF16, s1);
@@ -468,7 +500,11 @@ extern "C" int fmc_flow_pack(void* stream, const fmc_flow* fl) {
                        fl->packed_bias);
   else
     hipLaunchKernelGGL(flow_pack_kernel<PF32>, grid, dim3(256), 0, st, fd, tab, (float*)fl->packed, fl->packed_bias);
-  return check_launch("flow pack");
+  int rc2 = check_launch("flow pack");
+  if (rc2 || !dm.base_dense) return rc2;
+  hipLaunchKernelGGL(base_chol_kernel, dim3(1), dim3(256), 0, st, fl->theta + dm.th_base_cov(), dm.d,
+                     fl->packed_bias + dm.pb_base());
+  return check_launch("base covariance factor");
 }
 
 extern "C" int fmc_flow_weight_grad(void* stream, const fmc_flow* fl, int32_t layer, const void* store,

@@ -710,7 +710,7 @@ enum { FL_STD = 1, FL_BASE = 2, FL_SAMPLE = 4 };
 template <class PT>
 struct PassSmem {
   // byte offsets inside dynamic LDS
-  int xs, ldr, ldp, r1, r2, total;
+  int xs, ldr, ldp, r1, r2, ub, total;
   int ldx, lh, lo_, la0;
   __host__ __device__ PassSmem(const FlowDims& dm, int MT) {
     ldx = dm.d + 1;
@@ -734,6 +734,8 @@ struct PassSmem {
     o += sr;
     r2 = o;
     o += sr;
+    ub = o;  // dense base Gaussian: base noise / whitened residuals of the tile [MT][ldx] (FP32)
+    o += dm.base_dense ? rup(MT * ldx * 4, 16) : 0;
     total = o;
   }
 };
@@ -812,6 +814,8 @@ static int check_flow(const fmc_flow* fl, FlowDev& fd) {
   dm.nh = fl->n_hidden;
   for (int i = 0; i < FLOW_MAX_HIDDEN; ++i) dm.h[i] = i < fl->n_hidden ? fl->hidden[i] : 0;
   dm.kstep = fl->precision == FMC_PREC_BF16 ? 16 : 2;
+  FMC_ARG((fl->flags & ~FMC_FLOW_DENSE_BASE) == 0, "unknown flow flags 0x%x", fl->flags);
+  dm.base_dense = (fl->flags & FMC_FLOW_DENSE_BASE) ? 1 : 0;
   dm.finalize();
   fd.dm = dm;
   fd.theta = fl->theta;

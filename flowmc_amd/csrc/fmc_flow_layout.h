@@ -74,6 +74,8 @@ struct FlowDims {
   int nh;                    // hidden layers of the conditioner, 1..FLOW_MAX_HIDDEN
   int h[FLOW_MAX_HIDDEN];    // their widths (true sizes)
   int kstep;                 // 16 (bf16) or 2 (fp32): K granularity of one MFMA
+  int base_dense;            // the base Gaussian's full covariance is used (its Cholesky factor
+                             // follows the layers' packed biases: pb_base())
   // derived by finalize() (so device code reads offsets instead of looping
   // over the hidden layers): hsp[i] = sum_{j<i} hp(j); hmm[i] = sum over
   // 0 < j < i of hp(j) hp(j-1); the last and the widest padded hidden width
@@ -195,6 +197,9 @@ struct FlowDims {
     return (int64_t)(l / 2) * (pb_size(0) + pb_size(1)) + ((l & 1) ? pb_size(0) : 0);
   }
   FMC_HD int64_t pb_total() const { return pb_layer(L); }
+  // chol(base_cov) [d x d] row-major, then sum_j log L_jj (dense base only)
+  FMC_HD int64_t pb_base() const { return pb_total(); }
+  FMC_HD int64_t pb_alloc() const { return pb_total() + (base_dense ? (int64_t)d * d + 1 : 0); }
 
   // ---- theta offsets (FP32 master parameters)
   FMC_HD int lin_in(int i) const { return i == 0 ? d : h[i - 1]; }

@@ -33,6 +33,10 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   const int64_t m0 = (int64_t)blockIdx.x * MT;
   const float* th = f.theta;
   const T* pk = reinterpret_cast<const T*>(f.packed);
+  // full-covariance base Gaussian: chol(base_cov) row-major, then sum log L_jj
+  float* ub = reinterpret_cast<float*>(smem + sm.ub);
+  const float* Lb = f.pbias + dm.pb_base();
+  const bool dense_sample = dm.base_dense && a.dir != 0 && (a.flags & FL_SAMPLE);
 
   // ---- load tile (+ standardisation / base sampling)
   for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
@@ -49,19 +53,46 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       }
       if (a.dir == 0 && (a.flags & FL_STD)) {
         v = (v - th[dm.th_data_mean() + j]) / sqrtf(th[dm.th_data_cov() + (int64_t)j * d + j]);
-      } else if (a.dir != 0 && (a.flags & FL_SAMPLE)) {
+      } else if (a.dir != 0 && (a.flags & FL_SAMPLE) && !dense_sample) {
         v = th[dm.th_base_mean() + j] + sqrtf(th[dm.th_base_cov() + (int64_t)j * d + j]) * v;
       }
     }
+    if (dense_sample) ub[m * ldx + j] = v;
     xs[m * ldx + j] = v;
   }
   __syncthreads();
+  if (dense_sample) {  // base draw mean + L xi (jax.random.multivariate_normal, Cholesky method)
+    for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+      const int m = tw.m, j = tw.j;
+      const float* Lr = Lb + (int64_t)j * d;
+      float acc = th[dm.th_base_mean() + j];
+      for (int k = 0; k <= j; ++k) acc += Lr[k] * ub[m * ldx + k];
+      xs[m * ldx + j] = acc;
+    }
+    __syncthreads();
+  }
   // exact inverse with BASE_LOGP: the pass returns the flow log-density of
   // its output, log N(z) - sum of the inverse log-dets; start from -log N(z)
   const bool inv_logq = (a.dir == 2) && (a.flags & FL_BASE);
   for (int m = tid; m < MT; m += 256) {
     float v = 0.f;
-    if (inv_logq) {
+    if (inv_logq && dm.base_dense) {
+      // -log N(z; mean, L L^T) = |u|^2 / 2 + sum log L_jj + d/2 log(2 pi), L u = z - mean
+      float q = 0.f;
+      if (dense_sample) {
+        for (int j = 0; j < d; ++j) q += ub[m * ldx + j] * ub[m * ldx + j];
+      } else {
+        for (int j = 0; j < d; ++j) {
+          const float* Lr = Lb + (int64_t)j * d;
+          float r = xs[m * ldx + j] - th[dm.th_base_mean() + j];
+          for (int k = 0; k < j; ++k) r -= Lr[k] * ub[m * ldx + k];
+          const float u = r / Lr[j];
+          ub[m * ldx + j] = u;
+          q += u * u;
+        }
+      }
+      v = 0.5f * q + Lb[(int64_t)d * d] + 0.91893853320467274f * (float)d;
+    } else if (inv_logq) {
       float q = 0.f, lg = 0.f;
       for (int j = 0; j < d; ++j) {
         float var = th[dm.th_base_cov() + (int64_t)j * d + j];
@@ -242,7 +273,30 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     int64_t row = m0 + m;
     if (row >= a.n) continue;
     float ld = ldr[m];
-    if (a.dir == 0 && (a.flags & FL_BASE)) {
+    if (a.dir == 0 && (a.flags & FL_BASE) && dm.base_dense) {
+      // log N(y; mean, L L^T) (common.py:285-286): forward substitution
+      // L u = y - mean, then -|u|^2 / 2 - sum log L_jj - d/2 log(2 pi);
+      // its y-gradient -Sigma^-1 (y - mean) = -L^-T u by back substitution
+      float q = 0.f;
+      for (int j = 0; j < d; ++j) {
+        const float* Lr = Lb + (int64_t)j * d;
+        float r = xs[m * ldx + j] - th[dm.th_base_mean() + j];
+        for (int k = 0; k < j; ++k) r -= Lr[k] * ub[m * ldx + k];
+        const float u = r / Lr[j];
+        ub[m * ldx + j] = u;
+        q += u * u;
+      }
+      if (a.gy_out) {
+        for (int j = d - 1; j >= 0; --j) {
+          float r = ub[m * ldx + j];
+          for (int k = j + 1; k < d; ++k) r -= Lb[(int64_t)k * d + j] * ub[m * ldx + k];
+          const float w = r / Lb[(int64_t)j * d + j];
+          ub[m * ldx + j] = w;
+          a.gy_out[row * d + j] = a.g_logp[row] * (-w);
+        }
+      }
+      ld += -0.5f * q - Lb[(int64_t)d * d] - 0.91893853320467274f * (float)d;
+    } else if (a.dir == 0 && (a.flags & FL_BASE)) {
       // log N(y; base_mean, diag(base_cov)) (common.py:285-286 with a diagonal covariance)
       float q = 0.f, lg = 0.f;
       for (int j = 0; j < d; ++j) {

@@ -10,6 +10,8 @@ from __future__ import annotations
 
 from abc import ABC, abstractmethod
 
+import math
+
 import torch
 
 
@@ -124,8 +126,10 @@ class MaskedCouplingLayer(Bijection):
 
 
 class Gaussian(Distribution):
-    """Multivariate Gaussian base distribution (common.py:243-293); the flow
-    kernels require a diagonal covariance."""
+    """Multivariate Gaussian base distribution (common.py:243-293). The flow
+    kernels evaluate it inside their passes (diagonal fast path, or the full
+    covariance through its Cholesky factor); these methods are the same
+    densities for direct use."""
 
     def __init__(self, mean: torch.Tensor, cov: torch.Tensor, learnable: bool = False):
         self._mean = torch.as_tensor(mean, dtype=torch.float32)
@@ -141,14 +145,18 @@ class Gaussian(Distribution):
         return self._cov
 
     def log_prob(self, x):
+        """multivariate_normal.logpdf(x, mean, cov) (common.py:285-286)."""
         d = self._mean.shape[-1]
-        var = torch.diagonal(self._cov)
-        u = torch.as_tensor(x, dtype=torch.float32, device=var.device) - self._mean.to(var.device)
-        return -0.5 * (u * u / var).sum(-1) - 0.5 * torch.log(var).sum() - 0.5 * d * torch.log(
-            torch.tensor(2 * torch.pi, device=var.device))
+        L = torch.linalg.cholesky(self._cov.double())
+        u = torch.as_tensor(x, dtype=torch.float64, device=L.device) - self._mean.to(L.device, torch.float64)
+        w = torch.linalg.solve_triangular(L, u.reshape(-1, d).T, upper=False).T.reshape(u.shape)
+        lp = -0.5 * (w * w).sum(-1) - torch.log(torch.diagonal(L)).sum() - 0.5 * d * math.log(2 * math.pi)
+        return lp.to(torch.float32)
 
     def sample(self, rng_key, n_samples: int):
+        """mean + xi @ chol(cov).T (jax.random.multivariate_normal, common.py:288-293)."""
         from ... import random as fr
 
         z = fr.normal(rng_key, (n_samples, self._mean.shape[-1]), device=self._cov.device)
-        return self._mean.to(z.device) + z * torch.sqrt(torch.diagonal(self._cov)).to(z.device)
+        L = torch.linalg.cholesky(self._cov.double()).to(torch.float32)
+        return self._mean.to(z.device) + z @ L.T.to(z.device)

@@ -87,11 +87,18 @@ class MaskedCouplingRQSpline(NFModel):
         self.min_knot_slope = 1e-4
         self.precision = "bfloat16" if _PREC[precision] == 1 else "float32"
         self.exact_inverse = bool(kwargs.get("exact_inverse", False))
-        self._configure()
+        bd = kwargs.get("base_dist")
+        if bd is not None and getattr(bd, "learnable", False):
+            raise NotImplementedError("learnable base distributions are not supported by the flow kernels")
         theta = kwargs.get("_theta")
         if theta is None:
             theta = self._init_theta(key, kwargs)
         self.theta = theta.to(self.device, torch.float32).contiguous()
+        # a base covariance with off-diagonal entries takes the full-covariance
+        # base Gaussian (its Cholesky factor is packed with the biases)
+        bc = self.theta[-self._n_features ** 2:].view(self._n_features, self._n_features)
+        self._base_dense = bool(torch.count_nonzero(bc - torch.diag(torch.diagonal(bc))) > 0)
+        self._configure()
         self._dirty = True
         self._ws: dict[int, _Workspace] = {}
 
@@ -124,7 +131,8 @@ class MaskedCouplingRQSpline(NFModel):
                                  _PREC[self.precision], self.spline_range[0], self.spline_range[1], self.min_bin_size,
                                  self.min_knot_slope, tp,
                                  pk.data_ptr() if (pk is not None and theta_ptr is None) else None,
-                                 pb.data_ptr() if (pb is not None and theta_ptr is None) else None)
+                                 pb.data_ptr() if (pb is not None and theta_ptr is None) else None,
+                                 flags=_native.FLOW_DENSE_BASE if getattr(self, "_base_dense", False) else 0)
 
     def _leaf_shapes(self):
         """(name, shape) of one layer's leaves in theta order: scale, shift,
@@ -214,8 +222,6 @@ class MaskedCouplingRQSpline(NFModel):
         else:
             bm = torch.as_tensor(bd.mean, dtype=torch.float32).reshape(d).cpu()
             bc = torch.as_tensor(bd.cov, dtype=torch.float32).reshape(d, d).cpu()
-            if torch.count_nonzero(bc - torch.diag(torch.diagonal(bc))) != 0:
-                raise NotImplementedError("the flow kernels need a diagonal base covariance")
         parts += [dm.reshape(-1).cpu(), dc.reshape(-1).cpu(), bm.reshape(-1), bc.reshape(-1)]
         return torch.cat([p.reshape(-1).to(torch.float32) for p in parts])
 

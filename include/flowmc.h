@@ -394,6 +394,14 @@ int fmc_rnvp_weight_grad(void* stream, const fmc_rnvp* flow, int32_t layer, floa
 /* ------------------------------------------------------------------------ */
 typedef enum fmc_precision { FMC_PREC_F32 = 0, FMC_PREC_BF16 = 1 } fmc_precision;
 
+/* fmc_flow.flags. With FMC_FLOW_DENSE_BASE the base Gaussian uses its full
+ * covariance (Gaussian.log_prob / sample, common.py:285-293:
+ * multivariate_normal.logpdf and mean + chol(cov) xi); fmc_flow_pack then also
+ * writes chol(base_cov) and its log-determinant after the layers' packed
+ * biases (packed_bias holds d*d + 1 more floats). Without it only the
+ * diagonal of base_cov is read (exact for a diagonal covariance). */
+enum { FMC_FLOW_DENSE_BASE = 1 };
+
 #define FMC_FLOW_MAX_HIDDEN 4
 
 typedef struct fmc_flow {
@@ -402,6 +410,7 @@ typedef struct fmc_flow {
                               /* (hidden_size of rqSpline.py:395, common.py:86-107) */
   int32_t hidden[FMC_FLOW_MAX_HIDDEN]; /* their widths; entries >= n_hidden unused */
   int32_t precision;          /* fmc_precision of the conditioner GEMMs */
+  int32_t flags;              /* FMC_FLOW_DENSE_BASE: base_cov has off-diagonal entries */
   float range_min, range_max; /* spline range (rqSpline.py:399)        */
   float min_bin_size, min_knot_slope; /* rqSpline.py:299-300           */
   const float* theta;
@@ -415,7 +424,8 @@ typedef struct fmc_flow {
 int fmc_flow_sizes(const fmc_flow* flow, int64_t* theta_size, int64_t* packed_size,
                    int64_t* packed_bias_size, int64_t* store_rows, int32_t* tile_fwd,
                    int32_t* tile_bwd);
-/* theta -> packed MFMA operands + biases (after every parameter update). */
+/* theta -> packed MFMA operands + biases, and chol(base_cov) with
+ * FMC_FLOW_DENSE_BASE (after every parameter update). */
 int fmc_flow_pack(void* stream, const fmc_flow* flow);
 
 enum { FMC_FLOW_FORWARD = 0, FMC_FLOW_INVERSE = 1, FMC_FLOW_INVERSE_EXACT = 2 };

@@ -100,3 +100,17 @@ def test_hidden_layer_count_is_checked():
         N.call("fmc_flow_sizes", C.byref(flow(4, 2, 8, [32, 32, 600])), None, None, None, None, None, None)
     with pytest.raises(ValueError):
         N.make_flow(4, 2, 8, [], 1)
+
+
+def test_gaussian_view_log_prob_uses_the_full_covariance():
+    # Gaussian.log_prob = multivariate_normal.logpdf (common.py:285-286), dense covariance
+    from flowmc_amd.resource.model.common import Gaussian
+
+    g = torch.Generator().manual_seed(3)
+    A = torch.randn(5, 5, generator=g, dtype=torch.float64)
+    cov = A @ A.T / 5 + 0.3 * torch.eye(5, dtype=torch.float64)
+    mean = torch.randn(5, generator=g, dtype=torch.float64)
+    x = torch.randn(20, 5, generator=g, dtype=torch.float64)
+    want = torch.distributions.MultivariateNormal(mean, cov).log_prob(x)
+    got = Gaussian(mean, cov).log_prob(x)
+    assert torch.allclose(got.double(), want, atol=1e-4, rtol=1e-5)
