@@ -58,15 +58,78 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def flow_flops_per_sample(d, L, K, h1, h2):
-    """Minimal conditioner FLOPs of one pass: sum over layers of
-    2 * (d_c*h1 + h1*h2 + h2*d_t*(3K+1))."""
-    tot = 0
-    for l in range(L):
-        dc = (d + 1) // 2 if l % 2 else d // 2
-        dt = d - dc
-        tot += 2 * (dc * h1 + h1 * h2 + h2 * dt * (3 * K + 1))
-    return tot
+def flow_layer_flops(d, layer, K, hidden):
+    """Minimal conditioner FLOPs of one coupling layer per sample:
+    2 * (d_c*h_0 + sum_i h_{i-1}*h_i + h_last*d_t*(3K+1))."""
+    dc = (d + 1) // 2 if layer % 2 else d // 2
+    dt = d - dc
+    tot = dc * hidden[0] + sum(a * b for a, b in zip(hidden[:-1], hidden[1:])) + hidden[-1] * dt * (3 * K + 1)
+    return 2 * tot
+
+
+def flow_flops_per_sample(d, L, K, h1, h2=None):
+    """Minimal conditioner FLOPs of one pass (hidden widths (h1, h2) or the
+    list h1): the sum of flow_layer_flops over the layers."""
+    hidden = [h1, h2] if h2 is not None else list(h1)
+    return sum(flow_layer_flops(d, l, K, hidden) for l in range(L))
+
+
+def cpu_rate(make_work, n_all, n_one=None, reps=5, warmup=1):
+    """Throughput of a CPU leg. ``make_work(n)`` returns a callable that does n
+    units of work (chain-steps, samples, ...). With all host threads on n_all
+    units and with one thread on n_one units (n_all // 8 by default): one
+    warm-up call, then the median time of ``reps`` calls. Returns
+    (units/s all threads, units/s one thread, threads used)."""
+    import statistics
+
+    import torch
+
+    threads = os.cpu_count() or 1
+    n_one = n_one or max(1, n_all // 8)
+    rates = []
+    for nt, n in ((threads, n_all), (1, n_one)):
+        torch.set_num_threads(nt)
+        fn = make_work(n)
+        for _ in range(warmup):
+            fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        rates.append(n / statistics.median(ts))
+    torch.set_num_threads(threads)
+    return rates[0], rates[1], threads
+
+
+def trace_marker():
+    """One tiny fmc_philox_blocks launch: in a kernel trace, the launches
+    between two of them are the timed region (nothing else launches that
+    kernel in these workloads)."""
+    import torch
+
+    import flowmc_amd._native as N
+
+    buf = torch.empty(4, dtype=torch.int32, device="cuda")
+    N.call("fmc_philox_blocks", N.stream_ptr(), 0, 0, 0, 0, 0, 0, 1, buf.data_ptr())
+
+
+def mode_occupancy(pos, target):
+    """Fraction of the samples [..., d] whose most likely mixture component
+    (largest weighted component density) is each mode."""
+    import torch
+
+    x = pos.reshape(-1, pos.shape[-1])
+    mu = target.means.to(x.device, torch.float32)
+    sg = target.sigmas.to(x.device, torch.float32)
+    lw = torch.log(target.weights.to(x.device, torch.float32))
+    counts = torch.zeros(mu.shape[0], dtype=torch.float64, device=x.device)
+    for i in range(0, x.shape[0], 65536):
+        xb = x[i:i + 65536]
+        z = (xb[:, None, :] - mu) / sg
+        q = lw - torch.log(sg).sum(-1) - 0.5 * (z * z).sum(-1)
+        counts += torch.bincount(q.argmax(-1), minlength=mu.shape[0]).double()
+    return (counts / counts.sum()).tolist()
 
 
 def time_kernel(fn, reps=5):
@@ -182,11 +245,14 @@ def bench_nfmcmc(args):
         torch.cuda.synchronize()
     bundle, sampler = make(C_, args.steps, fr.PRNGKey(args.seed))
     theta0 = bundle.resources["model"].theta.detach().clone()
+    x_run = x0.clone()
+    trace_marker()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sampler.sample(x0.clone(), {})
+    sampler.sample(x_run, {})
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    trace_marker()
     res = sampler.resources
     an = nfmcmc_analysis(res["positions_production"].data, target, cfg["n_local"], cfg["n_global"])
     la = res["local_accs_production"].data
@@ -194,11 +260,44 @@ def bench_nfmcmc(args):
     local_acc = float(la[torch.isfinite(la)].mean())
     global_acc = float(ga[torch.isfinite(ga)].mean())
     value = an["min_ess_bulk"] / wall
-    # roofline of the dominant kernel: one flow inverse pass over the C*S proposals
+    occupancy = mode_occupancy(res["positions_production"].data, target)
+    # Roofline of the dominant kernel of the timed region: the flow adjoint
+    # (per layer, 10k-sample training batches; the largest share of the run's
+    # GPU time), timed alone with HIP events on its stream. Its algorithmic
+    # FLOPs per launch are twice the layer's forward (recompute + transposed
+    # products) over the batch.
+    import ctypes as _C
+
+    import flowmc_amd._native as N
+
     model = res["model"]
+    B = cfg["batch_size"]
+    ws = model._workspace(B)
+    model._ensure_packed()
+    fs = model._struct()
+    xb = res["positions_production"].data[:, -1].reshape(-1, d)
+    xb = xb[torch.randint(0, xb.shape[0], (B,), device=dev)].contiguous()
+    N.call("fmc_flow_pass", N.stream_ptr(), _C.byref(fs), 0, 1 | 2, xb.data_ptr(), B, 0, 0, 0, None,
+           ws.logp.data_ptr(), ws.saved.data_ptr(), ws.g_logp.data_ptr(), ws.gy.data_ptr())
+    lay = L - 1
+
+    def adjoint():
+        N.call("fmc_flow_backward_layer", N.stream_ptr(), _C.byref(fs), lay, ws.saved[lay].data_ptr(), B,
+               ws.g_logp.data_ptr(), ws.gy.data_ptr(), ws.gx.data_ptr(), ws.store[0].data_ptr(), ws.ld,
+               ws.aff[0].data_ptr(), ws.n_part)
+
+    ms_adj = time_kernel(adjoint, reps=20)
+    fl_adj = 2 * flow_layer_flops(d, lay, K, H) * B
+    ach_adj = fl_adj / (ms_adj * 1e-3) / 1e12
+    traffic_adj = None  # HBM bytes per launch from the counter runs of the flow_adjoint workload
+    tfa = os.path.join(ROOT, "profiles", "flow_bwd_kernel_traffic.json")
+    if os.path.exists(tfa) and B == 10000:
+        with open(tfa) as f:
+            traffic_adj = json.load(f).get("bytes_per_launch")
+    # the next kernel: one flow inverse pass over the C*S proposals of a global step
     S = cfg["n_global"]
     ms = time_kernel(lambda: model.sample(fr.PRNGKey(3), C_ * S), reps=3)
-    fl = flow_flops_per_sample(d, L, K, H[0], H[1]) * C_ * S
+    fl = flow_flops_per_sample(d, L, K, H) * C_ * S
     achieved = fl / (ms * 1e-3) / 1e12
     traffic = None  # HBM bytes per launch from the counter runs of the same pass (flow_pass workload)
     tf = os.path.join(ROOT, "profiles", "flow_pass_kernel_traffic.json")
@@ -208,11 +307,17 @@ def bench_nfmcmc(args):
     out = {
         "value": value,
         "ms_per_step": wall * 1e3 / (2 * args.steps),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "flow_pass_kernel (exact inverse, BF16)", "kernel_ms": ms,
-                     "flops_per_launch": fl},
-        "extra": {"wall_s": wall, **an, "local_acceptance": local_acc, "global_acceptance": global_acc},
+        "roofline": {"bound": "mfma", "achieved": ach_adj, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach_adj / BF16_PEAK_TFLOPS, "traffic": traffic_adj,
+                     "kernel": f"flow_bwd_kernel_h256 (adjoint of one coupling layer, {B}-sample batch, BF16)",
+                     "kernel_ms": ms_adj, "flops_per_launch": fl_adj},
+        "extra": {"wall_s": wall, **an, "local_acceptance": local_acc, "global_acceptance": global_acc,
+                  "mode_occupancy": occupancy,
+                  "roofline_flow_pass": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
+                                         "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic,
+                                         "kernel": "flow_pass_kernel (exact inverse over the 409,600 proposals "
+                                                   "of a global step, BF16)",
+                                         "kernel_ms": ms, "flops_per_launch": fl}},
         "config": {"workload": "nfmcmc_gmm64", "chains": C_, "dim": d, "modes": 8, "mala_step": cfg["step_size"],
                    "flow": {"layers": L, "bins": K, "hidden": H, "precision": args.precision,
                             "exact_inverse": cfg["exact_inverse"]},
@@ -224,6 +329,26 @@ def bench_nfmcmc(args):
     }
     del res, sampler, bundle
     torch.cuda.empty_cache()
+    if not args.no_cold:
+        # The same run started from x0 ~ N(0, I): MALA at 0.05 reaches only the
+        # basins near the start and the flow trains on what the chains hold,
+        # so modes stay unvisited; reported for comparison (usually invalid).
+        _, cs = make(C_, args.steps, fr.PRNGKey(args.seed))
+        xc = torch.randn(C_, d, device=dev, generator=torch.Generator(device=dev).manual_seed(11 + args.seed))
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        cs.sample(xc, {})
+        torch.cuda.synchronize()
+        cwall = time.perf_counter() - tc
+        cres = cs.resources
+        can = nfmcmc_analysis(cres["positions_production"].data, target, cfg["n_local"], cfg["n_global"])
+        cga = cres["global_accs_production"].data
+        out["extra"]["cold_start"] = {
+            "initial_positions": "x0 ~ N(0, I)", "value": can["min_ess_bulk"] / cwall, "wall_s": cwall, **can,
+            "mode_occupancy": mode_occupancy(cres["positions_production"].data, target),
+            "global_acceptance": float(cga[torch.isfinite(cga)].mean())}
+        del cres, cs
+        torch.cuda.empty_cache()
     if not args.no_plain:
         import bench_plain as BP
 
@@ -248,12 +373,13 @@ def bench_nfmcmc(args):
 
 
 def cpu_baseline_nfmcmc(args, cfg, gpu_wall, min_ess, C_):
-    """Oracle (FP32 torch-CPU restatement of the reference, full-width
-    conditioner, 2 value_and_grad per MALA step) timed on a bounded sample:
-    100 MALA steps and 4 of the 100 NF proposals for a subset of chains, and
-    one 10k-sample training step; the full run's CPU time is extrapolated
-    (chain-proportional parts scaled to all chains, training steps counted)
-    and the fused run's min-ESS is divided by it."""
+    """The oracle (FP32 torch-CPU restatement of the reference: full-width
+    conditioner, one-hot bins, 2 value_and_grad per MALA step) timed per
+    component on bounded samples (one warm-up, median of 5, all host threads
+    and one thread): MALA chain-steps, NF proposals (exact-inverse sample,
+    its log_prob, the target) and training samples (forward + backward). The
+    full run's CPU time is these rates times the run's counts; the fused
+    run's min-ESS is divided by it (same algorithm, so the same ESS)."""
     import torch
 
     from oracle import chains as OC
@@ -261,38 +387,55 @@ def cpu_baseline_nfmcmc(args, cfg, gpu_wall, min_ess, C_):
     from oracle import targets as OT
     from flowmc_amd import targets as TG
 
-    torch.set_num_threads(os.cpu_count() or 1)
     d, L, K, H = 64, cfg["n_layers"], cfg["n_bins"], cfg["hidden"]
     tgt = TG.gaussian_mixture_64d(seed=0, device="cpu")
     fn = lambda z: OT.gmm_diag(z, tgt.means.float(), tgt.sigmas.float())
-    p = F.init_params(d, L, K, H[0], H[1], dtype=torch.float32, hidden_var=1e-2)
-    cs = args.cpu_chains
-    x = torch.randn(cs, d)
-    t0 = time.perf_counter()
-    for _ in range(cfg["n_local"]):
-        x, _, _, _ = OC.mala_step(x, fn, cfg["step_size"], torch.randn(cs, d), torch.log(torch.rand(cs)))
-    t_local = time.perf_counter() - t0
+    p = F.init_params(d, L, K, H, dtype=torch.float32, hidden_var=1e-2)
     theta = F.to_flat(p).clone().requires_grad_(True)
-    data = torch.randn(cfg["batch_size"], d)
-    t0 = time.perf_counter()
-    loss = F.loss(F.from_flat(theta, d, L, K, H[0], H[1]), data, K)
-    loss.backward()
-    t_train_step = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    with torch.no_grad():
-        z = torch.randn(cs * 4, d)
-        prop = F.sample_from_noise(p, z, K, exact=True)
-        F.log_prob(p, prop, K)
-        fn(prop)
-    t_global = (time.perf_counter() - t0) * cfg["n_global"] / 4
-    per_chain = (t_local + t_global) / cs
+
+    def mala(n):  # n chain-steps: up to 512 chains, 10 steps
+        c = max(1, n // 10)
+        x = torch.randn(c, d)
+
+        def run():
+            nonlocal x
+            for _ in range(10):
+                x, _, _, _ = OC.mala_step(x, fn, cfg["step_size"], torch.randn(c, d), torch.log(torch.rand(c)))
+        return run
+
+    def proposals(n):  # n global proposals: exact-inverse samples, their flow density and the target
+        def run():
+            with torch.no_grad():
+                prop = F.sample_from_noise(p, torch.randn(n, d), K, exact=True)
+                F.log_prob(p, prop, K)
+                fn(prop)
+        return run
+
+    def train(n):  # n training samples: loss, forward + backward
+        data = torch.randn(n, d)
+
+        def run():
+            loss = F.loss(F.from_flat(theta, d, L, K, H), data, K)
+            torch.autograd.grad(loss, theta)
+        return run
+
+    r_mala = cpu_rate(mala, 5120, 640)
+    r_prop = cpu_rate(proposals, 2048, 256)
+    r_train = cpu_rate(train, 2000, 250)
+    n_rounds = 2 * cfg["n_loops"]
+    n_local = n_rounds * C_ * cfg["n_local"]
+    n_global = n_rounds * C_ * cfg["n_global"]
     steps_per_round = cfg["n_epochs"] * max(cfg["n_max_examples"] // cfg["batch_size"], 1)
-    total = cfg["n_loops"] * 2 * per_chain * C_ + cfg["n_loops"] * steps_per_round * t_train_step
-    return {"value": min_ess / total, "unit": "min-ESS/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"FP32 torch-CPU oracle restatement ({cpu_model()}): {cs} chains x (100 MALA steps + "
-                      f"4 of 100 NF proposals per chain) and one 10k-sample training step timed; full run of "
-                      f"{C_} chains x {2 * cfg['n_loops']} rounds and {cfg['n_loops'] * steps_per_round} training "
-                      f"steps extrapolated to {total:.0f} s; ESS taken from the fused run (same algorithm)"}
+    n_train = cfg["n_loops"] * steps_per_round * cfg["batch_size"]
+    tot = [n_local / r_mala[i] + n_global / r_prop[i] + n_train / r_train[i] for i in (0, 1)]
+    return {"value": min_ess / tot[0], "unit": "min-ESS/s", "cores": r_mala[2], "kind": "port",
+            "value_1_thread": min_ess / tot[1], "statistic": "median of 5 timed calls after one warm-up",
+            "sample": f"FP32 torch-CPU oracle restatement ({cpu_model()}), {r_mala[2]} threads (1 thread on 1/8 of "
+                      f"each sample): {r_mala[0]:.3g} MALA chain-steps/s (512 chains x 10 steps per call), "
+                      f"{r_prop[0]:.3g} NF proposals/s (2048 per call), {r_train[0]:.3g} training samples/s "
+                      f"(2000 per call); the run's {n_local:.3g} chain-steps, {n_global:.3g} proposals and "
+                      f"{n_train:.3g} training samples at those rates = {tot[0]:.0f} s ({tot[1]:.0f} s on 1 "
+                      f"thread); min-ESS of the fused run (same algorithm)"}
 
 
 # --------------------------------------------------------------- C3 -------
@@ -394,34 +537,36 @@ def _bench_local_rosenbrock(args, kind):
 
 def cpu_baseline_mala_rosenbrock(args, kind="mala"):
     """Oracle MALA (2 value_and_grad per step as MALA.py:59,71-73) or GRW (one
-    target evaluation per step), FP32, on a bounded sample of chains and steps,
-    timed on the host cores."""
+    target evaluation per step), FP32, timed with cpu_rate on bounded samples
+    (1024 chains x 8 steps per call; one thread: 128 chains)."""
     import torch
 
     from oracle import chains as OC
     from oracle import targets as OT
 
-    torch.set_num_threads(os.cpu_count() or 1)
-    C_, d, dt = args.cpu_chains, 128, 0.01
-    g = torch.Generator().manual_seed(0)
-    x = torch.randn(C_, d, generator=g)
+    d, dt = 128, 0.01
     fn = lambda z: OT.rosenbrock(z)
-    steps = 0
-    lp = fn(x)
-    OC.mala_step(x, fn, dt, torch.randn(C_, d, generator=g), torch.log(torch.rand(C_, generator=g)))
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        xi = torch.randn(C_, d, generator=g)
-        lu = torch.log(torch.rand(C_, generator=g))
-        if kind == "mala":
-            x, _, _, _ = OC.mala_step(x, fn, dt, xi, lu)
-        else:
-            x, lp, _, _ = OC.grw_step(x, lp, fn, dt, xi, lu)
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": C_ * steps / el, "unit": "chain-steps/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{C_} chains x {steps} {kind.upper()} steps, d={d}, FP32 torch-CPU oracle restatement "
-                      f"({cpu_model()})"}
+
+    def work(n):
+        c = max(1, n // 8)
+        x = torch.randn(c, d)
+        lp = fn(x)
+
+        def run():
+            nonlocal x, lp
+            for _ in range(8):
+                xi, lu = torch.randn(c, d), torch.log(torch.rand(c))
+                if kind == "mala":
+                    x, _, _, _ = OC.mala_step(x, fn, dt, xi, lu)
+                else:
+                    x, lp, _, _ = OC.grw_step(x, lp, fn, dt, xi, lu)
+        return run
+
+    r = cpu_rate(work, 8192, 1024)
+    return {"value": r[0], "unit": "chain-steps/s", "cores": r[2], "kind": "port", "value_1_thread": r[1],
+            "statistic": "median of 5 timed calls after one warm-up",
+            "sample": f"1024 chains x 8 {kind.upper()} steps per call (1 thread: 128 chains), d={d}, FP32 "
+                      f"torch-CPU oracle restatement ({cpu_model()})"}
 
 
 # --------------------------------------------------------------- C2 -------
@@ -455,7 +600,7 @@ def bench_flow_train(args):
     wall = time.perf_counter() - t0
     l1 = float(-m.log_prob(data[:20000]).mean())
     # dominant kernel: the per-layer adjoint + weight-gradient; report the whole step on MFMA
-    fl = 3 * flow_flops_per_sample(d, L, K, H[0], H[1]) * B
+    fl = 3 * flow_flops_per_sample(d, L, K, H) * B
     achieved = fl * args.steps / wall / 1e12
     ms_fwd = time_kernel(lambda: m.log_prob(batches[0]), reps=5)
     out = {
@@ -465,7 +610,7 @@ def bench_flow_train(args):
                      "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
                      "kernel": "train step (forward + adjoint + weight grad + AdamW)",
                      "forward_pass_ms": ms_fwd,
-                     "forward_pass_tflops": flow_flops_per_sample(d, L, K, H[0], H[1]) * B / (ms_fwd * 1e-3) / 1e12},
+                     "forward_pass_tflops": flow_flops_per_sample(d, L, K, H) * B / (ms_fwd * 1e-3) / 1e12},
         "extra": {"loss_before": l0, "loss_after": l1},
         "config": {"workload": "flow_train_d32", "dim": d, "layers": L, "bins": K, "hidden": H, "batch": B,
                    "examples": N, "optimizer": "adam(1e-3)", "precision": args.precision},
@@ -476,24 +621,28 @@ def bench_flow_train(args):
 
 
 def cpu_baseline_flow_train(args, d, L, K, H, B):
+    """Oracle loss + autograd gradient (FP32 torch-CPU) per training sample,
+    timed with cpu_rate (2000-sample batches; one thread: 250)."""
     import torch
 
     from oracle import flow as F
 
-    torch.set_num_threads(os.cpu_count() or 1)
-    p = F.init_params(d, L, K, H[0], H[1], dtype=torch.float32, hidden_var=1e-2)
+    p = F.init_params(d, L, K, H, dtype=torch.float32, hidden_var=1e-2)
     theta = F.to_flat(p).clone().requires_grad_(True)
-    nb = min(B, args.cpu_chains)
-    x = torch.randn(nb, d)
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        loss = F.loss(F.from_flat(theta, d, L, K, H[0], H[1]), x, K)
-        (gt,) = torch.autograd.grad(loss, theta)
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": steps * nb / el, "unit": "samples/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{steps} forward+backward steps on {nb}-sample batches, FP32 torch-CPU oracle ({cpu_model()})"}
+
+    def work(n):
+        x = torch.randn(n, d)
+
+        def run():
+            loss = F.loss(F.from_flat(theta, d, L, K, H), x, K)
+            torch.autograd.grad(loss, theta)
+        return run
+
+    r = cpu_rate(work, 2000, 250)
+    return {"value": r[0], "unit": "samples/s", "cores": r[2], "kind": "port", "value_1_thread": r[1],
+            "statistic": "median of 5 timed calls after one warm-up",
+            "sample": f"forward + backward of 2000-sample batches (1 thread: 250), FP32 torch-CPU oracle "
+                      f"({cpu_model()})"}
 
 
 # ------------------------------------------------- flow pass (profiling) --
@@ -518,7 +667,7 @@ def bench_flow_pass(args):
     for i in range(max(1, args.warmup)):
         m.sample(fr.PRNGKey(i), n)
     ms = time_kernel(lambda: m.sample(fr.PRNGKey(1), n), reps=args.steps)
-    fl = flow_flops_per_sample(d, L, K, H[0], H[1]) * n
+    fl = flow_flops_per_sample(d, L, K, H) * n
     achieved = fl / (ms * 1e-3) / 1e12
     traffic = None
     tf = os.path.join(ROOT, "profiles", "flow_pass_kernel_traffic.json")
@@ -579,7 +728,7 @@ def bench_realnvp_train(args):
     out = {
         "value": args.steps * B / wall,
         "ms_per_step": wall * 1e3 / args.steps,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "train step (fused forward + per-layer adjoint + partial reduction + AdamW), FP32",
                      "flops_per_step": fl_step},
@@ -594,24 +743,28 @@ def bench_realnvp_train(args):
 
 
 def cpu_baseline_realnvp(args, d, L, H, B):
+    """Oracle RealNVP loss + autograd gradient (FP32 torch-CPU) per training
+    sample, timed with cpu_rate (2000-sample batches; one thread: 250)."""
     import torch
 
     from oracle import realnvp as R
 
-    torch.set_num_threads(os.cpu_count() or 1)
     p = R.init_params(d, L, H, dtype=torch.float32)
     theta = R.to_flat(p).clone().requires_grad_(True)
-    nb = min(B, args.cpu_chains)
-    x = torch.randn(nb, d)
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        loss = R.loss(R.from_flat(theta, d, L, H), x)
-        (gt,) = torch.autograd.grad(loss, theta)
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": steps * nb / el, "unit": "samples/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{steps} forward+backward steps on {nb}-sample batches, FP32 torch-CPU oracle ({cpu_model()})"}
+
+    def work(n):
+        x = torch.randn(n, d)
+
+        def run():
+            loss = R.loss(R.from_flat(theta, d, L, H), x)
+            torch.autograd.grad(loss, theta)
+        return run
+
+    r = cpu_rate(work, 2000, 250)
+    return {"value": r[0], "unit": "samples/s", "cores": r[2], "kind": "port", "value_1_thread": r[1],
+            "statistic": "median of 5 timed calls after one warm-up",
+            "sample": f"forward + backward of 2000-sample batches (1 thread: 250), FP32 torch-CPU oracle "
+                      f"({cpu_model()})"}
 
 
 # ----------------------------------------------------- parallel tempering --
@@ -678,7 +831,7 @@ def bench_pt_gmm64(args):
         "value": C_ * T * S * args.steps / wall,
         "ms_per_step": wall * 1e3 / args.steps,
         # compute-bound: the 8-mode value + gradient per step; the HBM side is in extra
-        "roofline": {"bound": "mfma", "achieved": flops / (k_ms * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": flops / (k_ms * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / (k_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "chain_kernel (fused tempered MALA, FP32 vector ALU, one launch of 100 steps per call)",
                      "kernel_ms": k_ms, "flops_per_launch": flops},
@@ -691,41 +844,43 @@ def bench_pt_gmm64(args):
 
 
 def cpu_baseline_pt(args, tgt, T, S, dt):
-    """Oracle parallel tempering (tempered MALA steps, swap sweep, adaptation),
-    FP32 torch on the host cores, on a bounded sample of chains."""
+    """Oracle parallel tempering (tempered MALA steps, then the swap sweep and
+    the adaptation), FP32 torch on the host cores, timed with cpu_rate: one
+    call = 10 tempered MALA steps of 128 chains x T temperatures and one
+    exchange + adaptation (one thread: 16 chains)."""
     import torch
 
     from oracle import chains as OC
     from oracle import targets as OT
     from oracle import tempering as OPT
 
-    torch.set_num_threads(os.cpu_count() or 1)
-    C_, d = max(8, args.cpu_chains // 16), tgt.n_dim
+    d = tgt.n_dim
     means, sigmas = tgt.means.float().cpu(), tgt.sigmas.float().cpu()
     fn = lambda z: OT.gmm_diag(z, means, sigmas)
-    temps = torch.linspace(1.0, 5.0, T)
-    g = torch.Generator().manual_seed(0)
-    x = means[torch.randint(0, means.shape[0], (C_ * T,), generator=g)] + torch.randn(C_ * T, d, generator=g)
-    fr_ = OPT.tempered_rows(fn, temps, C_)
-    steps = calls = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        for _ in range(S):
-            x, _, _, _ = OC.mala_step(x, lambda z: fr_(z).to(z.dtype), dt, torch.randn(C_ * T, d, generator=g),
-                                      torch.log(torch.rand(C_ * T, generator=g)))
-            steps += 1
-            if time.perf_counter() - t0 > args.cpu_seconds:
-                break
-        xs = x.reshape(C_, T, d)
-        xs, _, acc = OPT.exchange(xs, fn(x).reshape(C_, T), temps, torch.log(torch.rand(C_, T - 1, generator=g)))
-        temps = OPT.adapt_temperature(temps, acc).float()
-        fr_ = OPT.tempered_rows(fn, temps, C_)
-        x = xs.reshape(C_ * T, d)
-        calls += 1
-    el = time.perf_counter() - t0
-    return {"value": C_ * T * steps / el, "unit": "tempered chain-steps/s", "cores": torch.get_num_threads(),
-            "kind": "port", "sample": f"{C_} chains x {T} temperatures, {steps} tempered MALA steps in {calls} "
-                                      f"strategy calls, FP32 torch-CPU oracle restatement ({cpu_model()})"}
+    n_steps = 10
+
+    def work(n):  # n tempered chain-steps
+        c = max(1, n // (T * n_steps))
+        temps = torch.linspace(1.0, 5.0, T)
+        x = means[torch.randint(0, means.shape[0], (c * T,))] + torch.randn(c * T, d)
+
+        def run():
+            nonlocal x, temps
+            fr_ = OPT.tempered_rows(fn, temps, c)
+            for _ in range(n_steps):
+                x, _, _, _ = OC.mala_step(x, lambda z: fr_(z).to(z.dtype), dt, torch.randn(c * T, d),
+                                          torch.log(torch.rand(c * T)))
+            xs, _, acc = OPT.exchange(x.reshape(c, T, d), fn(x).reshape(c, T), temps,
+                                      torch.log(torch.rand(c, T - 1)))
+            temps = OPT.adapt_temperature(temps, acc).float()
+            x = xs.reshape(c * T, d)
+        return run
+
+    r = cpu_rate(work, 128 * T * n_steps, 16 * T * n_steps)
+    return {"value": r[0], "unit": "tempered chain-steps/s", "cores": r[2], "kind": "port", "value_1_thread": r[1],
+            "statistic": "median of 5 timed calls after one warm-up",
+            "sample": f"128 chains x {T} temperatures x {n_steps} tempered MALA steps + one swap sweep and "
+                      f"adaptation per call (1 thread: 16 chains), FP32 torch-CPU oracle restatement ({cpu_model()})"}
 
 
 # --------------------------------------------------------------- C4 -------
@@ -796,7 +951,7 @@ def bench_hmc_funnels(args):
     out = {
         "value": C_ * len(dims) * T / wall,
         "ms_per_step": wall * 1e3 / T,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "hmc_batched_kernel (FP32 vector ALU)", "kernel_ms": gpu_s * 1e3 / n_launch,
                      "flops_per_launch": flops_total / n_launch},
@@ -814,32 +969,241 @@ def bench_hmc_funnels(args):
 
 
 def cpu_baseline_hmc_funnels(args, dims, C_, L):
-    """Oracle HMC steps (autograd gradients, FP32 torch on the host cores) on
-    every funnel of the batch in turn, all chains of a target at once, for
-    about --cpu-seconds; the rate counts chain-steps over all targets."""
+    """Oracle HMC steps (autograd gradients, FP32 torch on the host cores):
+    one call = one HMC step of every funnel of the batch with 64 chains each
+    (one thread: 8), timed with cpu_rate; the rate counts chain-steps."""
     import torch
 
     from oracle import chains as OC
     from oracle import targets as OT
 
-    torch.set_num_threads(os.cpu_count() or 1)
-    g = torch.Generator().manual_seed(0)
-    xs = [torch.randn(C_, d, generator=g) for d in dims]
-    lps = [OT.funnel(x) for x in xs]
-    ones = [torch.ones(d) for d in dims]
-    steps = sweeps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        for i, d in enumerate(dims):
-            xs[i], lps[i], _, _ = OC.hmc_step(xs[i], lps[i], OT.funnel, 0.05, L, ones[i],
-                                              torch.randn(C_, d, generator=g),
-                                              torch.log(torch.rand(C_, generator=g)))
-            steps += C_
-        sweeps += 1
-    el = time.perf_counter() - t0
-    return {"value": steps / el, "unit": "chain-steps/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{sweeps} sweeps of one HMC step over all {len(dims)} funnels x {C_} chains (L={L}), "
+    def work(n):  # n chain-steps over all funnels
+        c = max(1, n // len(dims))
+        xs = [torch.randn(c, d) for d in dims]
+        lps = [OT.funnel(x) for x in xs]
+        ones = [torch.ones(d) for d in dims]
+
+        def run():
+            for i, d in enumerate(dims):
+                xs[i], lps[i], _, _ = OC.hmc_step(xs[i], lps[i], OT.funnel, 0.05, L, ones[i], torch.randn(c, d),
+                                                  torch.log(torch.rand(c)))
+        return run
+
+    r = cpu_rate(work, 64 * len(dims), 8 * len(dims))
+    return {"value": r[0], "unit": "chain-steps/s", "cores": r[2], "kind": "port", "value_1_thread": r[1],
+            "statistic": "median of 5 timed calls after one warm-up",
+            "sample": f"one HMC step (L={L}) of all {len(dims)} funnels x 64 chains per call (1 thread: 8 chains), "
                       f"FP32 torch-CPU oracle restatement with autograd gradients ({cpu_model()})"}
+
+
+# --------------------------------------------------------------- C1 -------
+def bench_mala_gauss5d(args):
+    """MALA on the 5-D correlated Gaussian (Sigma = Q diag(lam) Q^T, Q seeded
+    orthogonal, lam log-uniform in [0.1, 10]): 1000 chains from x0 ~ N(0, I),
+    step 0.1, ``--steps`` (1000) steps through TakeSerialSteps, all steps in
+    one fused launch, then one TrainModel round on the chain buffer
+    (MaskedCouplingRQSpline 4 layers / 8 bins / [32, 32], clip + AdamW lr
+    1e-3, batch 1000, 5 epochs over n_max_examples 10000 from the last 100
+    rows of every chain). value = chain-steps/s of the MALA phase; the
+    training round is reported beside it. At 1000 chains the fused kernel
+    occupies a few dozen wavefronts: it is latency-bound, far below the HBM
+    roofline its bytes would allow."""
+    import torch
+
+    from flowmc_amd import random as fr
+    from flowmc_amd import targets as TG
+    from flowmc_amd.resource.buffers import Buffer
+    from flowmc_amd.resource.kernel.MALA import MALA
+    from flowmc_amd.resource.logPDF import LogPDF
+    from flowmc_amd.resource.model.nf_model.rqSpline import MaskedCouplingRQSpline
+    from flowmc_amd.resource.optimizer import Optimizer
+    from flowmc_amd.resource.states import State
+    from flowmc_amd.strategy.take_steps import TakeSerialSteps
+    from flowmc_amd.strategy.train_model import TrainModel
+
+    dev = torch.device("cuda")
+    C_, d, dt, S = args.chains or 1000, 5, 0.1, args.steps
+    tgt = TG.correlated_gaussian_5d(seed=0, device=dev)
+
+    def resources(n_steps):
+        return {"positions": Buffer("positions", (C_, n_steps, d), 1, device=dev),
+                "log_prob": Buffer("log_prob", (C_, n_steps), 1, device=dev),
+                "acceptance": Buffer("acceptance", (C_, n_steps), 1, device=dev),
+                "mala": MALA(dt, fused_steps=max(n_steps, 1)), "logpdf": LogPDF(tgt, n_dims=d),
+                "sampler_state": State({"positions": "positions", "log_prob": "log_prob",
+                                        "acceptance": "acceptance"}, name="sampler_state")}
+
+    def strategy(n_steps):
+        return TakeSerialSteps("logpdf", kernel_name="mala", state_name="sampler_state",
+                               buffer_names=["positions", "log_prob", "acceptance"], n_steps=n_steps)
+
+    x0 = torch.randn(C_, d, device=dev, generator=torch.Generator(device=dev).manual_seed(args.seed))
+    if args.warmup > 0:
+        strategy(args.warmup)(fr.PRNGKey(5), resources(args.warmup), x0.clone(), {})
+    res = resources(S)
+    strat = strategy(S)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    trace_marker()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    _, res, last = strat(fr.PRNGKey(args.seed), res, x0.clone(), {})
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    trace_marker()
+    gpu_ms = ev0.elapsed_time(ev1)
+    pos = res["positions"].data
+    acc = res["acceptance"].data
+    # the TrainModel round on the chain buffer
+    model = MaskedCouplingRQSpline(d, 4, [32, 32], 8, fr.PRNGKey(1), precision=args.precision)
+    res["model"] = model
+    res["optimizer"] = Optimizer(model, learning_rate=1e-3)
+    res["loss"] = Buffer("loss", (5,), 0, device=dev)
+    train = TrainModel("model", "positions", "optimizer", loss_buffer_name="loss", n_epochs=5, batch_size=1000,
+                       n_max_examples=10000, history_window=100)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, res, _ = train(fr.PRNGKey(2), res, last, {})
+    torch.cuda.synchronize()
+    train_wall = time.perf_counter() - t1
+    losses = res["loss"].data.tolist()
+    cov_emp = torch.cov(pos[:, S // 2:].reshape(-1, d).T.double())
+    cov_t = tgt.covariance.to(dev)
+    bytes_total = C_ * (S * (4 * d + 8) + (8 * d + 8))
+    achieved = bytes_total / (gpu_ms * 1e-3) / 1e9
+    out = {
+        "value": C_ * S / wall,
+        "ms_per_step": wall * 1e3 / S,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"chain_kernel<MALA, GAUSS_DENSE> ({S} steps in one launch; latency-bound at "
+                               f"{C_} chains)", "kernel_ms": gpu_ms, "bytes_per_launch": bytes_total,
+                     "ns_per_step": gpu_ms * 1e6 / S},
+        "extra": {"acceptance": float(acc.mean()), "train_wall_s": train_wall,
+                  "train_samples_per_s": 5 * 10000 / train_wall, "train_losses": losses,
+                  "c1_wall_s": wall + train_wall,
+                  "max_rel_cov_err_second_half": float(((cov_emp - cov_t).abs() / cov_t.diagonal().sqrt()[:, None]
+                                                        / cov_t.diagonal().sqrt()[None, :]).max())},
+        "config": {"workload": "mala_gauss5d", "chains": C_, "dim": d, "step_size": dt, "steps": S,
+                   "initial_positions": "x0 ~ N(0, I)",
+                   "train": {"layers": 4, "bins": 8, "hidden": [32, 32], "optimizer": "clip(1) + adamw(1e-3)",
+                             "batch": 1000, "epochs": 5, "n_max_examples": 10000, "history_window": 100,
+                             "precision": args.precision}},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_gauss5d(tgt, C_, d, dt, S)
+    return out
+
+
+def cpu_baseline_gauss5d(tgt, C_, d, dt, S):
+    """Oracle MALA on the same 5-D Gaussian (2 value_and_grad per step) and the
+    oracle flow's training step, FP32 torch-CPU, timed with cpu_rate: 1000
+    chains x 10 steps per call (one thread: 125 chains); 1000-sample batches
+    (one thread: 125). The whole workload's CPU time is extrapolated from the
+    two rates in extra."""
+    import torch
+
+    from oracle import chains as OC
+    from oracle import flow as F
+    from oracle import targets as OT
+
+    P = tgt.precision.double().cpu().float()
+    fn = lambda z: OT.gauss_dense(z, torch.zeros(d), P)
+
+    def mala(n):
+        c = max(1, n // 10)
+        x = torch.randn(c, d)
+
+        def run():
+            nonlocal x
+            for _ in range(10):
+                x, _, _, _ = OC.mala_step(x, fn, dt, torch.randn(c, d), torch.log(torch.rand(c)))
+        return run
+
+    p = F.init_params(d, 4, 8, [32, 32], dtype=torch.float32, hidden_var=1e-2)
+    theta = F.to_flat(p).clone().requires_grad_(True)
+
+    def train(n):
+        x = torch.randn(n, d)
+
+        def run():
+            torch.autograd.grad(F.loss(F.from_flat(theta, d, 4, 8, [32, 32]), x, 8), theta)
+        return run
+
+    r = cpu_rate(mala, C_ * 10, C_ * 10 // 8)
+    rt = cpu_rate(train, 1000, 125)
+    total = C_ * S / r[0] + 50000 / rt[0]
+    return {"value": r[0], "unit": "chain-steps/s", "cores": r[2], "kind": "port", "value_1_thread": r[1],
+            "statistic": "median of 5 timed calls after one warm-up",
+            "train_samples_per_s": rt[0], "c1_cpu_s": total,
+            "sample": f"{C_} chains x 10 MALA steps per call (1 thread: {C_ // 8} chains), d={d}; training: "
+                      f"1000-sample forward + backward per call; FP32 torch-CPU oracle restatement ({cpu_model()}); "
+                      f"the workload's {C_ * S} chain-steps and 50,000 training samples take {total:.1f} s at "
+                      f"those rates"}
+
+
+# --------------------------------------------------- flow adjoint (profiling) --
+def bench_flow_adjoint(args):
+    """The dominant kernel of the NF-MCMC workload's timed region alone: the
+    adjoint of one coupling layer (d=64, [256, 256], 8 bins, BF16) over a
+    10,000-sample training batch, ``--steps`` launches. Also runs one 256 MiB
+    device-to-device copy (a kernel of known traffic) so that counter runs
+    (FETCH_SIZE / WRITE_SIZE) can be scaled to bytes; traffic measured that
+    way is read from profiles/flow_bwd_kernel_traffic.json when present."""
+    import ctypes as _C
+
+    import torch
+
+    import flowmc_amd._native as N
+    from flowmc_amd import random as fr
+    from flowmc_amd.resource.model.nf_model.rqSpline import MaskedCouplingRQSpline
+
+    dev = torch.device("cuda")
+    d, L, K, H, B = 64, 10, 8, [256, 256], 10000
+    m = MaskedCouplingRQSpline(d, L, H, K, fr.PRNGKey(0), precision=args.precision, device=dev)
+    x = torch.randn(B, d, device=dev) * 2
+    m.grad_log_prob_sum(x)  # builds the workspace and the saved layer inputs
+    ws = m._workspace(B)
+    fs = m._struct()
+    N.call("fmc_flow_pass", N.stream_ptr(), _C.byref(fs), 0, 1 | 2, x.data_ptr(), B, 0, 0, 0, None,
+           ws.logp.data_ptr(), ws.saved.data_ptr(), ws.g_logp.data_ptr(), ws.gy.data_ptr())
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)  # calibration copy: 256 MiB read + 256 MiB written
+    lay = L - 1
+
+    def adjoint():
+        N.call("fmc_flow_backward_layer", N.stream_ptr(), _C.byref(fs), lay, ws.saved[lay].data_ptr(), B,
+               ws.g_logp.data_ptr(), ws.gy.data_ptr(), ws.gx.data_ptr(), ws.store[0].data_ptr(), ws.ld,
+               ws.aff[0].data_ptr(), ws.n_part)
+
+    for _ in range(max(1, args.warmup)):
+        adjoint()
+    trace_marker()
+    ms = time_kernel(adjoint, reps=args.steps)
+    trace_marker()
+    fl = 2 * flow_layer_flops(d, lay, K, H) * B
+    achieved = fl / (ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "flow_bwd_kernel_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    store_bytes = m._store_rows * ws.ld * (2 if args.precision == "bfloat16" else 4)
+    return {
+        "value": B / (ms * 1e-3),
+        "ms_per_step": ms,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS if args.precision == "bfloat16"
+                     else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / (BF16_PEAK_TFLOPS if args.precision == "bfloat16" else FP32_PEAK_TFLOPS),
+                     "traffic": traffic, "kernel": "flow_bwd_kernel_h256 (adjoint of one coupling layer)",
+                     "kernel_ms": ms, "flops_per_launch": fl,
+                     "algorithmic_bytes_per_launch": B * (3 * d * 4 + 4) + store_bytes},
+        "config": {"workload": "flow_adjoint", "batch": B, "dim": d, "bins": K, "hidden": H,
+                   "precision": args.precision},
+    }
 
 
 WORKLOADS = {
@@ -854,6 +1218,10 @@ WORKLOADS = {
                     "f32/f64"),
     "flow_pass": (bench_flow_pass, "flow samples/s (inverse pass, d=64, 10 layers, [256,256])", "samples/s",
                   "bf16/f32"),
+    "flow_adjoint": (bench_flow_adjoint, "flow adjoint samples/s (one coupling layer, d=64, [256,256], batch 10k)",
+                     "samples/s", "bf16/f32"),
+    "mala_gauss5d": (bench_mala_gauss5d, "chain-steps/s (fused MALA, 5-D correlated Gaussian, 1000 chains)",
+                     "chain-steps/s", "f32"),
     "realnvp_train_d32": (bench_realnvp_train, "RealNVP train samples/s (d=32, 8 layers, hidden 256, batch 10k)",
                           "samples/s", "f32"),
     "pt_mala_gmm64": (bench_pt_gmm64, "tempered chain-steps/s (parallel tempering, 64-D 8-mode mixture, 5 temperatures)",
@@ -861,10 +1229,10 @@ WORKLOADS = {
 }
 DEFAULT_STEPS = {"nfmcmc_gmm64": 20, "mala_rosenbrock_128d": 10048, "flow_train_d32": 50, "hmc_funnels": 1000,
                  "flow_pass": 10, "grw_rosenbrock_128d": 10048, "pt_mala_gmm64": 50,
-                 "realnvp_train_d32": 50}
+                 "realnvp_train_d32": 50, "flow_adjoint": 50, "mala_gauss5d": 1000}
 DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d32": 3, "hmc_funnels": 16,
                   "flow_pass": 1, "grw_rosenbrock_128d": 640, "pt_mala_gmm64": 3,
-                  "realnvp_train_d32": 3}
+                  "realnvp_train_d32": 3, "flow_adjoint": 3, "mala_gauss5d": 100}
 
 
 def main():
@@ -889,6 +1257,8 @@ def main():
     ap.add_argument("--plain-budget", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cold", action="store_true",
+                    help="nfmcmc_gmm64: skip the second run started from x0 ~ N(0, I)")
     ap.add_argument("--no-graph", action="store_true",
                     help="Rosenbrock MALA / GRW, batched HMC: launch the fused steps eagerly instead of replaying a HIP graph")
     args = ap.parse_args()
