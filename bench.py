@@ -1235,7 +1235,87 @@ DEFAULT_WARMUP = {"nfmcmc_gmm64": 1, "mala_rosenbrock_128d": 640, "flow_train_d3
                   "realnvp_train_d32": 3, "flow_adjoint": 3, "mala_gauss5d": 100}
 
 
+def summarize_trace(path, top=25):
+    """Markdown table of the kernels a rocprofv3 kernel_trace.csv records
+    between the first two trace_marker() launches (the timed region)."""
+    import collections
+    import csv
+
+    with open(path) as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
+    marks = [i for i, r in enumerate(rows) if "philox_blocks_kernel" in r["Kernel_Name"]]
+    if len(marks) < 2:
+        raise SystemExit(f"{path}: {len(marks)} trace markers, need 2")
+    a, b = marks[0], marks[1]
+    win = rows[a + 1:b]
+    span = (int(rows[b]["Start_Timestamp"]) - int(rows[a]["End_Timestamp"])) / 1e6
+    tot = collections.defaultdict(lambda: [0, 0.0])
+    busy = 0.0
+    for r in win:
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        n = r["Kernel_Name"]
+        n = n.split("(")[0] if n.startswith(("void fmc", "fmc::")) else n[:100]
+        tot[n][0] += 1
+        tot[n][1] += dur
+        busy += dur
+    lines = [f"window: {len(win)} launches between the markers, {span:.1f} ms under the profiler; kernel time "
+             f"{busy / 1e3:.1f} ms ({100 * busy / 1e3 / span:.1f} % of the window)", "",
+             "| kernel | calls | avg (us) | total (ms) | % of kernel time |", "|---|---:|---:|---:|---:|"]
+    for n, (c, t) in sorted(tot.items(), key=lambda kv: -kv[1][1])[:top]:
+        lines.append(f"| `{n}` | {c} | {t / c:.1f} | {t / 1e3:.2f} | {100 * t / busy:.2f} |")
+    print("\n".join(lines))
+
+
+def summarize_traffic(fetch_csv, write_csv, kernel, label, out_json):
+    """Per-launch HBM traffic of ``kernel`` from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv files), each scaled
+    by what the same counter reads for the workload's 256 MiB calibration
+    copy (__amd_rocclr_copyBuffer of 268,435,456 bytes each way)."""
+    import collections
+    import csv
+
+    def per_dispatch(path, counter):
+        vals = collections.defaultdict(lambda: [None, 0.0])
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                v = vals[r["Dispatch_Id"]]
+                v[0] = r["Kernel_Name"]
+                v[1] += float(r["Counter_Value"])
+        return list(vals.values())
+
+    def kib(rows, pred):
+        xs = [v for n, v in rows if pred(n)]
+        if not xs:
+            raise SystemExit(f"no dispatch matches in the counter file")
+        return xs
+
+    cal = 268435456
+    out = {"kernel": label}
+    for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
+        rows = per_dispatch(path, counter)
+        k = kib(rows, lambda n: kernel in n)
+        c = [v for v in kib(rows, lambda n: "copyBuffer" in n) if v * 1024 > cal / 4]
+        out[f"{key}_size_kib_per_dispatch"] = sum(k) / len(k)
+        out.setdefault("calibration", {"kernel": "__amd_rocclr_copyBuffer", "bytes_read": cal, "bytes_written": cal})
+        out["calibration"][f"{key}_size_kib"] = max(c)
+        out["calibration"][f"{key}_scale"] = cal / (max(c) * 1024)
+    out["bytes_read_per_launch"] = out["fetch_size_kib_per_dispatch"] * 1024 * out["calibration"]["fetch_scale"]
+    out["bytes_written_per_launch"] = out["write_size_kib_per_dispatch"] * 1024 * out["calibration"]["write_scale"]
+    out["bytes_per_launch"] = out["bytes_read_per_launch"] + out["bytes_written_per_launch"]
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
 def main():
+    if len(sys.argv) >= 7 and sys.argv[1] == "--traffic-summary":
+        summarize_traffic(*sys.argv[2:7])
+        return
+    if len(sys.argv) >= 3 and sys.argv[1] == "--trace-summary":
+        summarize_trace(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 25)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="nfmcmc_gmm64", choices=list(WORKLOADS))
