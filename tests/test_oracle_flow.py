@@ -66,9 +66,42 @@ def test_spline_inverse_and_logdet_by_fd():
     assert bool(((y1 - y0) > 0).all())  # monotone
 
 
-def test_flow_change_of_variables_full_jacobian():
+@pytest.mark.parametrize("hidden", [[8], [8, 8], [8, 6, 8], [8, 5, 7, 6]])
+def test_mlp_any_depth_matches_torch_sequential(hidden):
+    """The conditioner of n hidden layers equals torch.nn's Linear / Tanh
+    stack built from the same weights (common.py:86-112): every hidden layer
+    applied once, tanh after each hidden layer, none after the output."""
+    d, K = 4, 3
+    lp = F.init_params(d, 1, K, hidden, seed=11, hidden_var=2.0)["layers"][0]
+    widths = [d] + list(hidden) + [d * (3 * K + 1)]
+    mods = []
+    for i in range(len(widths) - 1):
+        lin = torch.nn.Linear(widths[i], widths[i + 1], dtype=torch.float64)
+        with torch.no_grad():
+            lin.weight.copy_(lp[f"W{i + 1}"])
+            lin.bias.copy_(lp[f"b{i + 1}"])
+        mods.append(lin)
+        if i < len(widths) - 2:
+            mods.append(torch.nn.Tanh())
+    net = torch.nn.Sequential(*mods)
+    c = torch.randn(9, d) * 2
+    with torch.no_grad():
+        assert torch.allclose(F.mlp(lp, c), net(c), atol=1e-12)
+        # a conditioner that skips its last hidden layer (or puts tanh on the output) differs
+        bad = dict(lp)
+        assert F.n_linear(lp) == len(hidden) + 1
+        if len(hidden) > 1:
+            n = len(hidden)
+            bad.pop(f"W{n + 1}"), bad.pop(f"b{n + 1}")
+            bad[f"W{n}"] = torch.randn(widths[-1], widths[n - 1])
+            bad[f"b{n}"] = lp[f"b{n + 1}"]
+            assert not torch.allclose(F.mlp(bad, c), net(c), atol=1e-3)
+
+
+@pytest.mark.parametrize("hidden", [[8], [8, 8], [8, 6, 8, 5]])
+def test_flow_change_of_variables_full_jacobian(hidden):
     d, L, K = 3, 3, 4
-    p = F.init_params(d, L, K, 8, 8, seed=3, scale=0.3)
+    p = F.init_params(d, L, K, hidden, seed=3, scale=0.3)
     x = torch.randn(5, d) * 2
     y, ld = F.forward(p, x, K)
     for i in range(5):
@@ -80,15 +113,50 @@ def test_flow_change_of_variables_full_jacobian():
     assert torch.equal(yc[:, m], x[:, m])
 
 
-def test_flow_density_integrates_to_one_2d():
+def _dense_base(p, d):
+    """A correlated base Gaussian (full covariance, nonzero mean)."""
+    g = torch.Generator().manual_seed(12)
+    A = torch.randn(d, d, generator=g)
+    p["base_cov"] = A @ A.T / d + 0.5 * torch.eye(d)
+    p["base_mean"] = torch.randn(d, generator=g) * 0.5
+    return p
+
+
+@pytest.mark.parametrize("dense_base", [False, True])
+def test_flow_density_integrates_to_one_2d(dense_base):
     d, L, K = 2, 4, 6
     p = F.init_params(d, L, K, 8, 8, seed=4, scale=0.1)
+    if dense_base:
+        p = _dense_base(p, d)
     g = torch.linspace(-14, 14, 561)
     X, Y = torch.meshgrid(g, g, indexing="ij")
     pts = torch.stack([X.reshape(-1), Y.reshape(-1)], -1)
     lp = F.log_prob(p, pts, K).reshape(561, 561)
     integral = torch.trapezoid(torch.trapezoid(torch.exp(lp), g, dim=1), g)
     assert abs(float(integral) - 1.0) < 2e-3
+
+
+def test_identity_flow_with_dense_base_is_the_base_gaussian():
+    """With zero conditioner output weights and zero ScalarAffine parameters
+    every layer is the identity (rqSpline.py:20-39: zero logits give the
+    identity spline), so log_prob is the base MVN density (scipy) and the
+    samples have the base mean and covariance (jax.random.multivariate_normal
+    draws mean + chol(cov) xi, common.py:276-293)."""
+    from scipy.stats import multivariate_normal
+
+    d, L, K = 5, 2, 4
+    p = _dense_base(F.init_params(d, L, K, [6, 6], seed=13), d)
+    for lp in p["layers"]:
+        n = F.n_linear(lp)
+        lp[f"W{n}"].zero_()
+        lp[f"b{n}"].zero_()
+    x = torch.randn(40, d) * 1.5
+    ref = multivariate_normal(p["base_mean"].numpy(), p["base_cov"].numpy()).logpdf(x.numpy())
+    assert torch.allclose(F.log_prob(p, x, K), torch.as_tensor(ref), atol=1e-10)
+    xi = torch.randn(200000, d, generator=torch.Generator().manual_seed(14))
+    z = F.sample_from_noise(p, xi, K, exact=True)
+    assert torch.allclose(z.mean(0), p["base_mean"], atol=0.02)
+    assert torch.allclose(torch.cov(z.T), p["base_cov"], atol=0.03)
 
 
 def test_reference_inverse_order_quirk():
