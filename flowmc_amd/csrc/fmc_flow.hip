@@ -33,6 +33,7 @@ namespace fmc {
 constexpr int WG_SPLIT = 8;
 static_assert(16 % WG_SPLIT == 0, "each wave finishes a whole number of accumulator registers");
 constexpr int WG_XCDS = 8;
+constexpr int WG_PF = 8;  // weight-gradient prefetch depth (batch slices)
 
 // Problem table of one layer, built on the host: problem i covers tiles
 // start[i] .. start[i+1]-1, its G rows (gradient of the problem's output)
@@ -115,35 +116,54 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
   f32x16_t acc = {};
   float bsum = 0.f;
   // this wave takes batch slices wave, wave + WG_SPLIT, ... ; fragments
-  // (L2-resident store) are prefetched P slices ahead
-  constexpr int P = PF_DEPTH;
+  // (L2-resident store) are prefetched P slices ahead. Every full group of
+  // P slices but the last refills its ring slots unconditionally, so the
+  // wait before a slice's MFMA covers that slice's loads only.
+  constexpr int P = WG_PF;
   using GF = decltype(PT::load_g(st, 0, 0, 0, 0));
   GF ga[P], ab[P];
   const int nsteps = (int)(ld / PT::KSTEP);
   const int gRb = gRow / 32 + nb, aRb = aRow / 32 + kb;
   const int my = (nsteps - wave + WG_SPLIT - 1) / WG_SPLIT;  // slices of this wave
+  // the prologue loads every slot (slots past the wave's last slice repeat
+  // it, or read slice 0 when the wave has none), so the loop's waits stay exact
 #pragma unroll
-  for (int p = 0; p < P; ++p)
-    if (p < my) {
-      const int sl = wave + p * WG_SPLIT;
+  for (int p = 0; p < P; ++p) {
+    const int sl = my > 0 ? wave + (p < my ? p : my - 1) * WG_SPLIT : 0;
+    ga[p] = PT::load_g(st, nsteps, gRb, sl, lane);
+    ab[p] = PT::load_g(st, nsteps, aRb, sl, lane);
+  }
+  const int ng = my / P, tail = my - ng * P;
+  int i = 0;
+  for (int g = 0; g + 1 < ng; ++g) {
+#pragma unroll
+    for (int p = 0; p < P; ++p, ++i) {
+      acc = PT::mma(ga[p], ab[p], acc);
+      if (want_bias) bsum += PT::hsum(ga[p]);
+      const int sl = wave + (i + P) * WG_SPLIT;
       ga[p] = PT::load_g(st, nsteps, gRb, sl, lane);
       ab[p] = PT::load_g(st, nsteps, aRb, sl, lane);
     }
-  for (int s0 = 0; s0 < my; s0 += P) {
+  }
+  if (ng > 0) {  // the last full group refills the tail's slots only
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-      const int i = s0 + p;
-      if (i < my) {
-        acc = PT::mma(ga[p], ab[p], acc);
-        if (want_bias) bsum += PT::hsum(ga[p]);
-        if (i + P < my) {
-          const int sl = wave + (i + P) * WG_SPLIT;
-          ga[p] = PT::load_g(st, nsteps, gRb, sl, lane);
-          ab[p] = PT::load_g(st, nsteps, aRb, sl, lane);
-        }
+    for (int p = 0; p < P; ++p, ++i) {
+      acc = PT::mma(ga[p], ab[p], acc);
+      if (want_bias) bsum += PT::hsum(ga[p]);
+      if (p < tail) {
+        const int sl = wave + (i + P) * WG_SPLIT;
+        ga[p] = PT::load_g(st, nsteps, gRb, sl, lane);
+        ab[p] = PT::load_g(st, nsteps, aRb, sl, lane);
       }
     }
   }
+  const int nt = ng > 0 ? tail : my;  // slices left, in ring slots 0 .. nt-1
+#pragma unroll
+  for (int p = 0; p < P - 1; ++p)
+    if (p < nt) {
+      acc = PT::mma(ga[p], ab[p], acc);
+      if (want_bias) bsum += PT::hsum(ga[p]);
+    }
 #pragma unroll
   for (int v = 0; v < 16; ++v) red[wave][v][lane] = acc[v];
   rsum[wave][lane] = bsum;

@@ -11,16 +11,16 @@ namespace fmc {
 // multiples of 32) x samples m0 .. m0+MT, from src[m * lds + f] (rows f >=
 // nvalid are written as zeros). One thread per EPL-element lane chunk, so
 // consecutive threads write consecutive chunks of a 64-lane block.
-template <class PT>
+template <class PT, int MT>
 __device__ __forceinline__ void store_fm(typename PT::T* st, int64_t ld, int row0, int nf, int nvalid,
-                                         const typename PT::T* src, int lds, int64_t m0, int MT) {
+                                         const typename PT::T* src, int lds, int64_t m0) {
   using T = typename PT::T;
   constexpr int EPL = PT::EPL, KS = PT::KSTEP;
+  constexpr int SPT = MT / KS;  // batch slices in the tile
   const int64_t ns = ld / KS;
-  const int spt = MT / KS;                 // batch slices in the tile
-  const int nchunk = (nf / 32) * spt * 64;
+  const int nchunk = (nf / 32) * SPT * 64;
   for (int c = threadIdx.x; c < nchunk; c += 256) {
-    const int lane = c & 63, sl = (c >> 6) % spt, rbl = (c >> 6) / spt;
+    const int lane = c & 63, sl = (c >> 6) % SPT, rbl = (c >> 6) / SPT;
     const int f = rbl * 32 + (lane & 31);
     const int mb = sl * KS + (lane >> 5) * EPL;
     T* dst = st + (((int64_t)(row0 / 32 + rbl) * ns + m0 / KS + sl) * 64 + lane) * EPL;
@@ -29,8 +29,12 @@ __device__ __forceinline__ void store_fm(typename PT::T* st, int64_t ld, int row
         T t[EPL];
         uint4 u;
       } pk;
+      if (f < nvalid) {
 #pragma unroll
-      for (int e = 0; e < EPL; ++e) pk.t[e] = (f < nvalid) ? src[(mb + e) * lds + f] : PT::cvt(0.f);
+        for (int e = 0; e < EPL; ++e) pk.t[e] = src[(mb + e) * lds + f];
+      } else {
+        pk.u = uint4{0u, 0u, 0u, 0u};
+      }
       *reinterpret_cast<uint4*>(dst) = pk.u;
     } else {
 #pragma unroll
@@ -39,15 +43,38 @@ __device__ __forceinline__ void store_fm(typename PT::T* st, int64_t ld, int row
   }
 }
 
+// 1 - h^2 of one lane's 16 elements of a 32x32 hidden block, kept in
+// registers from the forward recompute to the adjoint (computed in FP32 from
+// the FP32 activation: the BF16 copy of h in LDS would wreck it near
+// saturation). BF16 conditioner: stored as BF16 pairs (relative rounding
+// 2^-9, below the BF16 rounding of the gradients it scales), which frees the
+// registers of a deeper weight-prefetch ring; FP32 conditioner: FP32.
+template <class PT>
+struct DhBlock {
+  float v[16];
+  __device__ __forceinline__ void put2(int i, float a, float b) {
+    v[2 * i] = a;
+    v[2 * i + 1] = b;
+  }
+  __device__ __forceinline__ float get(int i) const { return v[i]; }
+};
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+template <>
+struct DhBlock<PBF16> {
+  bf16x2_t p[8];
+  __device__ __forceinline__ void put2(int i, float a, float b) { p[i] = bf16x2_t{(__bf16)a, (__bf16)b}; }
+  __device__ __forceinline__ float get(int i) const { return (float)p[i >> 1][i & 1]; }
+};
+
 template <class PT, int MB, int HMAX, int KT, int NH>
 __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a) {
   using T = typename PT::T;
   constexpr int MT = 32 * MB;
+  constexpr int P = BWD_PF;     // ring depth of the [MT x hidden] products
+  constexpr int PR = BWD_PF_R;  // ring depth of the transposed output product (two fragments a step)
   // 32x32 tiles per wave of an [MT x hidden] block (hidden <= HMAX); the same
   // wave owns the same tiles in the forward recompute and in the adjoint, so
-  // the FP32 tanh derivatives 1 - h^2 of all NH hidden layers stay in its
-  // registers (the LDS copy of h may be BF16, whose rounding would wreck
-  // 1 - h^2 near saturation).
+  // the tanh derivatives of all NH hidden layers stay in its registers.
   constexpr int NCW = (HMAX / 32) / 4;  // column blocks per wave (even)
   extern __shared__ __align__(16) unsigned char smem[];
   const FlowDims& dm = f.dm;
@@ -68,21 +95,40 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
   const T* W = reinterpret_cast<const T*>(f.packed) + dm.pk_layer(l);
   const float* bl = f.pbias + dm.pb_layer(l);
   T* st = reinterpret_cast<T*>(a.store);
-  const int hlp = dm.hlp(), P = dm.P();
+  const int hlp = dm.hlp(), P3 = dm.P();
   const float sc = th[dm.th_scale(l)], sh = th[dm.th_shift(l)];
   const float es = expf(sc);
 
-  // ---- load x (layer input), gy, g_logp; apply the affine part
-  for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
-    const int m = tw.m, j = tw.j;
-    int64_t row = m0 + m;
-    bool ok = row < a.n;
-    xs[m * ldx + j] = ok ? (a.x_in[row * d + j] + sh) * es : 0.f;
-    gb[m * ldx + j] = ok ? a.gy[row * d + j] : 0.f;
+  // ---- load x (layer input), gy, g_logp: a thread issues all its loads of
+  // a batch of 8 elements before it writes any of them, so the tile costs
+  // one memory round trip per 2048 elements, not one per 256
+  {
+    const int tot = MT * d;
+    const int64_t base = m0 * d, lim = (a.n - m0) * (int64_t)d;  // valid elements of the tile
+    const float gl = (tid < MT && m0 + tid < a.n) ? a.g_logp[m0 + tid] : 0.f;
+    for (int e0 = 0; e0 < tot; e0 += 256 * 8) {
+      float xv[8], gv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * 256 + tid;
+        const bool ok = e < tot && e < lim;
+        xv[u] = ok ? a.x_in[base + e] : 0.f;
+        gv[u] = ok ? a.gy[base + e] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * 256 + tid;
+        if (e < tot) {
+          const int m = e / d, j = e - m * d;
+          xs[m * ldx + j] = (e < lim) ? (xv[u] + sh) * es : 0.f;
+          gb[m * ldx + j] = gv[u];
+        }
+      }
+    }
+    if (tid < MT) glp[tid] = gl;
   }
-  for (int m = tid; m < MT; m += 256) glp[m] = (m0 + m < a.n) ? a.g_logp[m0 + m] : 0.f;
   __syncthreads();
-  // ---- recompute the conditioner forward
+  // ---- the conditioner input (kc columns, zero beyond dc)
   const int dc = dm.dc(l), kc = dm.kc(l);
   T* a0 = gpb;
   const int la0 = sm.lg;
@@ -90,12 +136,14 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
     const int m = tw.m, k = tw.j;
     a0[m * la0 + k] = PT::cvt(k < dc ? xs[m * ldx + dm.cond(l, k)] : 0.f);
   }
-  __syncthreads();
-  store_fm<PT>(st, a.ld, (int)dm.st_c(l), dm.ncp(l), kc, a0, la0, m0, MT);
-  // wave w owns column blocks w, w+4, w+8, ... (all MB row blocks) of every
-  // [MT x hidden] product below; index ci of the register arrays is the
-  // wave-local column block.
-  f32x16_t dh[NH][NCW][MB];
+  // Stores of the fragment-blocked activations / gradients are issued after
+  // the MFMAs of the next product (the LDS copy is still intact then), so
+  // their completion overlaps the epilogue instead of delaying the next
+  // weight fragments (loads and stores share one in-order counter).
+  // ---- recompute the conditioner forward; wave w owns column blocks w,
+  // w+4, w+8, ... (all MB row blocks) of every [MT x hidden] product; index
+  // ci of the register arrays is the wave-local column block.
+  DhBlock<PT> dh[NH][NCW][MB];
   {
     const T* in = a0;
     int ldin = la0, KB = kc / PT::KSTEP;
@@ -104,13 +152,26 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
       T* out = hbuf[i & 1];
       const int NBt = dm.hp(i) / 32;
       const float* bi = bl + dm.pb_b(i);
+      const T* Wi = W + dm.pk_wh(l, i);
+      BRing<PT, 2, P> ring;
+      if (wave < NBt) panel_prefetch(ring, Wi, KB, wave, 4, (wave + 4 < NBt) ? 2 : 1, 0, KB, lane);
+      __syncthreads();  // `in` complete
 #pragma unroll
       for (int ci = 0; ci < NCW; ci += 2) {
         const int c0 = wave + 4 * ci;
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
         if (c0 < NBt) {
-          const int ncb = (c0 + 4 < NBt) ? 2 : 1;
-          f32x16_t acc[2][MB] = {};
-          mfma_panel<PT, MB, 2, PF_DEPTH>(in, ldin, W + dm.pk_wh(l, i), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+          if (ci > 0) panel_prefetch(ring, Wi, KB, c0, 4, ncb, 0, KB, lane);
+          panel_run<PT, MB, 2, P>(ring, in, ldin, Wi, KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+        }
+        if (ci == 0) {  // the previous product's input
+          if (i == 0)
+            store_fm<PT, MT>(st, a.ld, (int)dm.st_c(l), dm.ncp(l), kc, a0, la0, m0);
+          else
+            store_fm<PT, MT>(st, a.ld, (int)dm.st_h(l, i - 1), dm.hp(i - 1), dm.hp(i - 1), in, sm.lh, m0);
+        }
+        if (c0 < NBt) {
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             if (j < ncb) {
@@ -119,17 +180,16 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
 #pragma unroll
               for (int r = 0; r < MB; ++r)
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                  float h = PT::act(acc[j][r][v] + bb);
-                  out[(r * 32 + out_row(v, lane)) * sm.lh + col] = PT::cvt(h);
-                  dh[i][ci + j][r][v] = 1.f - h * h;
+                for (int v = 0; v < 16; v += 2) {
+                  const float h0 = PT::act(acc[j][r][v] + bb), h1 = PT::act(acc[j][r][v + 1] + bb);
+                  out[(r * 32 + out_row(v, lane)) * sm.lh + col] = PT::cvt(h0);
+                  out[(r * 32 + out_row(v + 1, lane)) * sm.lh + col] = PT::cvt(h1);
+                  dh[i][ci + j][r].put2(v >> 1, 1.f - h0 * h0, 1.f - h1 * h1);
                 }
             }
           }
         }
       }
-      __syncthreads();
-      store_fm<PT>(st, a.ld, (int)dm.st_h(l, i), dm.hp(i), dm.hp(i), out, sm.lh, m0, MT);
       in = out;
       ldin = sm.lh;
       KB = dm.hp(i) / PT::KSTEP;
@@ -147,19 +207,31 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
   if (dm.reg_spline()) {
     // rounds of 4 dim pairs (one per wave): transposed GEMM3 -> parameters in
     // registers -> spline VJP in registers -> gp (gradient order) in LDS ->
-    // gradient store + g_h2 += gp W3
+    // g_hl += gp W3. Rounds alternate between two gp buffers (gpb and the
+    // hidden buffer not holding the last hidden layer, free by now), so a
+    // round needs one barrier; a round's gp is stored at its end, after the
+    // next round's first W3 fragments are requested.
+    // (selected as LDS byte offsets: a pointer picked from an array lost its
+    // address space and became FLAT accesses, which the loads' waits count)
+    const int gp_off[2] = {sm.gpb, (NH & 1) ? sm.hb2 : sm.hb1};
     const int npair = dm.n_pairs(l), dtl = dm.dt(l), K = dm.K;
     const T* W3r = W + dm.pk_w3r(l);
     const T* W3rt = W + dm.pk_w3rt(l);
     const float* b3r = bl + dm.pb_b3r(l);
     const int KB3 = hlp / PT::KSTEP, KB3t = dm.n3r(l) / PT::KSTEP;
     const int hf = lane >> 5;
-    for (int q0 = 0; q0 < npair; q0 += 4) {
+    BRing<PT, 2, PR> ring;
+    if (wave < npair) panel_prefetch(ring, W3r, KB3, 2 * wave, 1, 2, 0, KB3, lane);
+    __syncthreads();  // the last hidden layer complete
+    store_fm<PT, MT>(st, a.ld, (int)dm.st_h(l, NH - 1), hlp, hlp, hlast, sm.lh, m0);
+    int rnd = 0;
+    for (int q0 = 0; q0 < npair; q0 += 4, ++rnd) {
       const int nq = (npair - q0) < 4 ? (npair - q0) : 4;
+      T* const gcur = reinterpret_cast<T*>(smem + ((rnd & 1) ? gp_off[1] : gp_off[0]));
+      const int q = q0 + wave;
       if (wave < nq) {
-        const int q = q0 + wave;
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH, true>(hlast, sm.lh, W3r, KB3, 2 * q, 1, 2, 0, KB3, 0, acc, lane);
+        panel_run<PT, MB, 2, PR, true>(ring, hlast, sm.lh, W3r, KB3, 2 * q, 1, 2, 0, KB3, 0, acc, lane);
         const int k = 2 * q + hf;
         const bool valid = k < dtl;
         const int j = valid ? dm.trans(l, k) : 0;
@@ -190,7 +262,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
             for (int i = 0; i < 32; ++i) go[i] = 0.f;
           }
           // slot 8t+e of this lane's dim -> gradient-order column 64 wave + 16t + 8hf + e
-          T* dst = gpb + m * sm.lg + 64 * wave + 8 * hf;
+          T* dst = gcur + m * sm.lg + 64 * wave + 8 * hf;
 #pragma unroll
           for (int t4 = 0; t4 < 4; ++t4) {
             if constexpr (sizeof(T) == 2) {  // 8 BF16 slots: one 16-byte LDS store
@@ -208,9 +280,86 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
           }
         }
       }
-      __syncthreads();
-      store_fm<PT>(st, a.ld, (int)dm.st_gp(l) + 64 * q0, 64 * nq, 64 * nq, gpb, sm.lg, m0, MT);
+      // g_hl += gp W3 over this round's k-range
       const int kb0 = 64 * q0 / PT::KSTEP, kb1 = 64 * (q0 + nq) / PT::KSTEP;
+      const int c0 = wave;  // column blocks wave, wave + 4 (NCW == 2) or wave + 4*ci
+      BRing<PT, 2, P> ring2;
+      if (c0 < NB2) panel_prefetch(ring2, W3rt, KB3t, c0, 4, (c0 + 4 < NB2) ? 2 : 1, kb0, kb1, lane);
+      __syncthreads();  // gcur complete
+#pragma unroll
+      for (int ci = 0; ci < NCW; ci += 2) {
+        const int cb = wave + 4 * ci;
+        if (cb < NB2) {
+          const int ncb = (cb + 4 < NB2) ? 2 : 1;
+          if (ci > 0) panel_prefetch(ring2, W3rt, KB3t, cb, 4, ncb, kb0, kb1, lane);
+          f32x16_t acc2[2][MB];
+#pragma unroll
+          for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int r = 0; r < MB; ++r) acc2[jj][r] = gh2[ci + jj][r];
+          panel_run<PT, MB, 2, P>(ring2, gcur, sm.lg, W3rt, KB3t, cb, 4, ncb, kb0, kb1, 0, acc2, lane);
+#pragma unroll
+          for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int r = 0; r < MB; ++r) gh2[ci + jj][r] = acc2[jj][r];
+        }
+      }
+      // this wave's pair in the next round (the ring is always rewritten, so
+      // its old contents are dead during the VJP)
+      const int qn = q0 + 4 + wave;
+      if (qn < npair) {
+        panel_prefetch(ring, W3r, KB3, 2 * qn, 1, 2, 0, KB3, lane);
+      } else {
+        ring = BRing<PT, 2, PR>{};
+      }
+      store_fm<PT, MT>(st, a.ld, (int)dm.st_gp(l) + 64 * q0, 64 * nq, 64 * nq, gcur, sm.lg, m0);
+    }
+    // every read of the last hidden layer ended before the last round's
+    // barrier; the gp buffers are next written after a barrier
+  } else {
+    __syncthreads();  // the last hidden layer complete
+    store_fm<PT, MT>(st, a.ld, (int)dm.st_h(l, NH - 1), hlp, hlp, hlast, sm.lh, m0);
+    const int nch = dm.n_chunks(l);
+    const int KB3 = hlp / PT::KSTEP;
+    const int KB3t = dm.n3p(l) / PT::KSTEP;
+    for (int c = 0; c < nch; ++c) {
+      const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
+      const int NBt = nc / 32;
+      for (int c0 = wave; c0 < NBt; c0 += 8) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, PF_DEPTH>(hlast, sm.lh, W + dm.pk_w3(l), KB3, coff / 32 + c0, 4, ncb, 0, KB3, 0, acc,
+                                        lane);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+            const float bb = bl[dm.pb_b3(l) + coff + col];
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+#pragma unroll
+              for (int v = 0; v < 16; ++v) obuf[(r * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[j][r][v] + bb;
+          }
+        }
+      }
+      __syncthreads();
+      for (int i = tid; i < MT * cnt; i += 256) {
+        int m = i % MT, jj = i / MT;
+        int j = dm.trans(l, c * dm.G() + jj);
+        const float* pp = obuf + m * sm.lo_ + jj * P3;
+        float gyv = gb[m * ldx + j];
+        float gxv = spline_vjp<T>(pp, f, xs[m * ldx + j], gyv, glp[m], gpb + m * sm.lg + jj * P3,
+                                  [](float v) { return PT::cvt(v); });
+        gb[m * ldx + j] = gxv;
+      }
+      for (int i = tid; i < MT * (nc - cnt * P3); i += 256) {
+        int w = nc - cnt * P3;
+        int m = i / w, cc = cnt * P3 + (i - m * w);
+        gpb[m * sm.lg + cc] = PT::cvt(0.f);
+      }
+      __syncthreads();
+      store_fm<PT, MT>(st, a.ld, (int)dm.st_gp(l) + coff, nc, nc, gpb, sm.lg, m0);
+      const int kb0 = coff / PT::KSTEP, kb1 = (coff + nc) / PT::KSTEP;
 #pragma unroll
       for (int ci = 0; ci < NCW; ci += 2) {
         const int c0 = wave + 4 * ci;
@@ -221,7 +370,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
           for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < MB; ++r) acc[j][r] = gh2[ci + j][r];
-          mfma_panel<PT, MB, 2, PF_DEPTH>(gpb, sm.lg, W3rt, KB3t, c0, 4, ncb, kb0, kb1, 0, acc, lane);
+          mfma_panel<PT, MB, 2, PF_DEPTH>(gpb, sm.lg, W + dm.pk_w3t(l), KB3t, c0, 4, ncb, kb0, kb1, 0, acc, lane);
 #pragma unroll
           for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -231,67 +380,8 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
       __syncthreads();
     }
   }
-  const int nch = dm.reg_spline() ? 0 : dm.n_chunks(l);
-  const int KB3 = hlp / PT::KSTEP;
-  const int KB3t = dm.n3p(l) / PT::KSTEP;
-  for (int c = 0; c < nch; ++c) {
-    const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
-    const int NBt = nc / 32;
-    for (int c0 = wave; c0 < NBt; c0 += 8) {
-      const int ncb = (c0 + 4 < NBt) ? 2 : 1;
-      f32x16_t acc[2][MB] = {};
-      mfma_panel<PT, MB, 2, PF_DEPTH>(hlast, sm.lh, W + dm.pk_w3(l), KB3, coff / 32 + c0, 4, ncb, 0, KB3, 0, acc,
-                                      lane);
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (j < ncb) {
-          const int col = (c0 + 4 * j) * 32 + (lane & 31);
-          const float bb = bl[dm.pb_b3(l) + coff + col];
-#pragma unroll
-          for (int r = 0; r < MB; ++r)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) obuf[(r * 32 + out_row(v, lane)) * sm.lo_ + col] = acc[j][r][v] + bb;
-        }
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < MT * cnt; i += 256) {
-      int m = i % MT, jj = i / MT;
-      int j = dm.trans(l, c * dm.G() + jj);
-      const float* pp = obuf + m * sm.lo_ + jj * P;
-      float gyv = gb[m * ldx + j];
-      float gxv = spline_vjp<T>(pp, f, xs[m * ldx + j], gyv, glp[m], gpb + m * sm.lg + jj * P,
-                                [](float v) { return PT::cvt(v); });
-      gb[m * ldx + j] = gxv;
-    }
-    for (int i = tid; i < MT * (nc - cnt * P); i += 256) {
-      int w = nc - cnt * P;
-      int m = i / w, cc = cnt * P + (i - m * w);
-      gpb[m * sm.lg + cc] = PT::cvt(0.f);
-    }
-    __syncthreads();
-    store_fm<PT>(st, a.ld, (int)dm.st_gp(l) + coff, nc, nc, gpb, sm.lg, m0, MT);
-    const int kb0 = coff / PT::KSTEP, kb1 = (coff + nc) / PT::KSTEP;
-#pragma unroll
-    for (int ci = 0; ci < NCW; ci += 2) {
-      const int c0 = wave + 4 * ci;
-      if (c0 < NB2) {
-        const int ncb = (c0 + 4 < NB2) ? 2 : 1;
-        f32x16_t acc[2][MB];
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < MB; ++r) acc[j][r] = gh2[ci + j][r];
-        mfma_panel<PT, MB, 2, PF_DEPTH>(gpb, sm.lg, W + dm.pk_w3t(l), KB3t, c0, 4, ncb, kb0, kb1, 0, acc, lane);
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < MB; ++r) gh2[ci + j][r] = acc[j][r];
-      }
-    }
-    __syncthreads();
-  }
-  // ---- g_{NH-1} = g_hl * (1 - h_{NH-1}^2)   (in place of h_{NH-1})
+  // ---- g_{NH-1} = g_hl * (1 - h_{NH-1}^2)   (in place of h_{NH-1}: every
+  // read of it ended before the last barrier)
 #pragma unroll
   for (int ci = 0; ci < NCW; ++ci) {
     const int nb = wave + 4 * ci;
@@ -301,24 +391,30 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
       for (int r = 0; r < MB; ++r)
 #pragma unroll
         for (int v = 0; v < 16; ++v)
-          hlast[(r * 32 + out_row(v, lane)) * sm.lh + col] = PT::cvt(gh2[ci][r][v] * dh[NH - 1][ci][r][v]);
+          hlast[(r * 32 + out_row(v, lane)) * sm.lh + col] = PT::cvt(gh2[ci][r][v] * dh[NH - 1][ci][r].get(v));
     }
   }
-  __syncthreads();
-  store_fm<PT>(st, a.ld, (int)dm.st_gh(l, NH - 1), hlp, hlp, hlast, sm.lh, m0, MT);
   // ---- g_{i-1} = (g_i W_i) * (1 - h_{i-1}^2), i = NH-1 .. 1   (in place of h_{i-1})
 #pragma unroll
   for (int i = NH - 1; i >= 1; --i) {
     const T* gin = hbuf[i & 1];
     T* gout = hbuf[(i - 1) & 1];
     const int KB = dm.hp(i) / PT::KSTEP, NBt = dm.hp(i - 1) / 32;
+    const T* Wt = W + dm.pk_wht(l, i);
+    BRing<PT, 2, P> ring;
+    if (wave < NBt) panel_prefetch(ring, Wt, KB, wave, 4, (wave + 4 < NBt) ? 2 : 1, 0, KB, lane);
+    __syncthreads();  // g_i complete
 #pragma unroll
     for (int ci = 0; ci < NCW; ci += 2) {
       const int c0 = wave + 4 * ci;
+      const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+      f32x16_t acc[2][MB] = {};
       if (c0 < NBt) {
-        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
-        f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PF_DEPTH>(gin, sm.lh, W + dm.pk_wht(l, i), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+        if (ci > 0) panel_prefetch(ring, Wt, KB, c0, 4, ncb, 0, KB, lane);
+        panel_run<PT, MB, 2, P>(ring, gin, sm.lh, Wt, KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+      }
+      if (ci == 0) store_fm<PT, MT>(st, a.ld, (int)dm.st_gh(l, i), dm.hp(i), dm.hp(i), gin, sm.lh, m0);
+      if (c0 < NBt) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           if (j < ncb) {
@@ -327,31 +423,60 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
             for (int r = 0; r < MB; ++r)
 #pragma unroll
               for (int v = 0; v < 16; ++v)
-                gout[(r * 32 + out_row(v, lane)) * sm.lh + col] = PT::cvt(acc[j][r][v] * dh[i - 1][ci + j][r][v]);
+                gout[(r * 32 + out_row(v, lane)) * sm.lh + col] =
+                    PT::cvt(acc[j][r][v] * dh[i - 1][ci + j][r].get(v));
           }
         }
       }
     }
-    __syncthreads();
-    store_fm<PT>(st, a.ld, (int)dm.st_gh(l, i - 1), dm.hp(i - 1), dm.hp(i - 1), gout, sm.lh, m0, MT);
   }
-  // ---- g_c = g_0 W_0c, scattered onto the conditioning dims
+  // ---- g_c = g_0 W_0c, scattered onto the conditioning dims. With one
+  // column block (dc <= 32) the k-range is split over the four waves and
+  // their partial blocks summed in a fixed order (in the gp buffer, free now).
   {
     const int KB = dm.hp(0) / PT::KSTEP, NBt = dm.ncp(l) / 32;
-    for (int c0 = wave; c0 < NBt; c0 += 8) {
-      const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+    const T* W0t = W + dm.pk_wht(l, 0);
+    const bool split = NBt == 1 && 4 * MT * 33 * 4 <= MT * sm.lg * (int)sizeof(T);
+    if (split) {
+      const int k0 = wave * KB / 4, k1 = (wave + 1) * KB / 4;
+      BRing<PT, 2, P> ring;
+      panel_prefetch(ring, W0t, KB, 0, 4, 1, k0, k1, lane);
+      __syncthreads();  // g_0 complete
       f32x16_t acc[2][MB] = {};
-      mfma_panel<PT, MB, 2, PF_DEPTH>(hbuf[0], sm.lh, W + dm.pk_wht(l, 0), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+      panel_run<PT, MB, 2, P>(ring, hbuf[0], sm.lh, W0t, KB, 0, 4, 1, k0, k1, k0, acc, lane);
+      store_fm<PT, MT>(st, a.ld, (int)dm.st_gh(l, 0), dm.hp(0), dm.hp(0), hbuf[0], sm.lh, m0);
+      float* part = reinterpret_cast<float*>(gpb);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (j < ncb) {
-          const int k = (c0 + 4 * j) * 32 + (lane & 31);
-          if (k < dc) {
-            const int jd = dm.cond(l, k);
+      for (int r = 0; r < MB; ++r)
 #pragma unroll
-            for (int r = 0; r < MB; ++r)
+        for (int v = 0; v < 16; ++v) part[(wave * MT + r * 32 + out_row(v, lane)) * 33 + (lane & 31)] = acc[0][r][v];
+      __syncthreads();
+      for (int e = tid; e < MT * 32; e += 256) {
+        const int m = e >> 5, k = e & 31;
+        if (k < dc) {
+          const float s4 = ((part[m * 33 + k] + part[(MT + m) * 33 + k]) + part[(2 * MT + m) * 33 + k]) +
+                           part[(3 * MT + m) * 33 + k];
+          gb[m * ldx + dm.cond(l, k)] += s4;
+        }
+      }
+    } else {
+      __syncthreads();  // g_0 complete
+      store_fm<PT, MT>(st, a.ld, (int)dm.st_gh(l, 0), dm.hp(0), dm.hp(0), hbuf[0], sm.lh, m0);
+      for (int c0 = wave; c0 < NBt; c0 += 8) {
+        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
+        f32x16_t acc[2][MB] = {};
+        mfma_panel<PT, MB, 2, P>(hbuf[0], sm.lh, W0t, KB, c0, 4, ncb, 0, KB, 0, acc, lane);
 #pragma unroll
-              for (int v = 0; v < 16; ++v) gb[(r * 32 + out_row(v, lane)) * ldx + jd] += acc[j][r][v];
+        for (int j = 0; j < 2; ++j) {
+          if (j < ncb) {
+            const int k = (c0 + 4 * j) * 32 + (lane & 31);
+            if (k < dc) {
+              const int jd = dm.cond(l, k);
+#pragma unroll
+              for (int r = 0; r < MB; ++r)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) gb[(r * 32 + out_row(v, lane)) * ldx + jd] += acc[j][r][v];
+            }
           }
         }
       }

@@ -27,6 +27,7 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 // Output element v of lane l is row 8*(v/4) + 4*(l/32) + v%4, column l%32.
 struct PF32 {
   using T = float;
+  using BF = float;  // one B fragment (per lane)
   static constexpr int EPL = 1, KSTEP = 2, PAD = 1;
   static __device__ __forceinline__ float load_a(const T* As, int lda, int row0, int kb, int lane) {
     return As[(row0 + (lane & 31)) * lda + kb * 2 + (lane >> 5)];
@@ -56,6 +57,7 @@ struct PF32 {
 
 struct PBF16 {
   using T = __bf16;
+  using BF = bf16x8_t;
   static constexpr int EPL = 8, KSTEP = 16, PAD = 8;
   static __device__ __forceinline__ bf16x8_t load_a(const T* As, int lda, int row0, int kb, int lane) {
     return *reinterpret_cast<const bf16x8_t*>(As + (row0 + (lane & 31)) * lda + kb * 16 + (lane >> 5) * 8);
@@ -165,41 +167,78 @@ __device__ __forceinline__ void mfma_block(const typename PT::T* As, int lda, in
 // so their latency hides behind the MFMAs of the previous steps.
 // TR = true swaps the operands: the packed image is the A operand (output
 // rows = its rows), the LDS rows are the B operand (output columns).
-template <class PT, int MB, int CB, int P, bool TR = false>
-__device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, const typename PT::T* Bp, int KBtot,
-                                           int nb0, int nbstep, int ncb, int kb0, int kb1, int a_kb0,
-                                           f32x16_t (&acc)[CB][MB], int lane) {
-  using BF = decltype(PT::load_b(Bp, 0, 0, 0, 0));
-  using AF = decltype(PT::load_a(As, 0, 0, 0, 0));
-  BF ring[P][CB];
+// panel_prefetch issues the ring's first P k-steps and panel_run consumes
+// them, so that a caller can put the first loads before a barrier.
+template <class PT, int CB, int P>
+struct BRing {
+  typename PT::BF r[P][CB];
+};
+
+// Every ring slot is loaded unconditionally (k and column indices clamped
+// to valid blocks; the MFMAs of steps past kb1 and of a missing second
+// column block are skipped instead), so each group issues the same loads and
+// the compiler's wait before a step covers that step's fragments only. A
+// conditional load made it wait for every outstanding load (and store) at the
+// top of each group.
+template <class PT, int CB, int P>
+__device__ __forceinline__ void panel_prefetch(BRing<PT, CB, P>& ring, const typename PT::T* Bp, int KBtot, int nb0,
+                                               int nbstep, int ncb, int kb0, int kb1, int lane) {
+  // (an empty range loads block kb0, or the last block: any valid block)
+  const int klast = kb1 > kb0 ? kb1 - 1 : (kb0 < KBtot ? kb0 : KBtot - 1);
 #pragma unroll
   for (int p = 0; p < P; ++p)
 #pragma unroll
-    for (int j = 0; j < CB; ++j)
-      if (kb0 + p < kb1 && j < ncb) ring[p][j] = PT::load_b(Bp, KBtot, nb0 + j * nbstep, kb0 + p, lane);
+    for (int j = 0; j < CB; ++j) {
+      const int nb = j < ncb ? nb0 + j * nbstep : nb0;
+      ring.r[p][j] = PT::load_b(Bp, KBtot, nb, kb0 + p < klast ? kb0 + p : klast, lane);
+    }
+}
+
+template <class PT, int MB, int CB, int P, bool TR = false>
+__device__ __forceinline__ void panel_run(BRing<PT, CB, P>& ring, const typename PT::T* As, int lda,
+                                          const typename PT::T* Bp, int KBtot, int nb0, int nbstep, int ncb, int kb0,
+                                          int kb1, int a_kb0, f32x16_t (&acc)[CB][MB], int lane) {
+  using AF = decltype(PT::load_a(As, 0, 0, 0, 0));
+  const int klast = kb1 - 1;
   for (int kb = kb0; kb < kb1; kb += P) {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int k = kb + p;
+      AF a[MB];
+#pragma unroll
+      for (int r = 0; r < MB; ++r) a[r] = PT::load_a(As, lda, r * 32, (k < klast ? k : klast) - kb0 + a_kb0, lane);
       if (k < kb1) {
-        AF a[MB];
 #pragma unroll
-        for (int r = 0; r < MB; ++r) a[r] = PT::load_a(As, lda, r * 32, k - kb0 + a_kb0, lane);
-#pragma unroll
-        for (int j = 0; j < CB; ++j) {
+        for (int j = 0; j < CB; ++j)
           if (j < ncb) {
 #pragma unroll
             for (int r = 0; r < MB; ++r)
-              acc[j][r] = TR ? PT::mma(ring[p][j], a[r], acc[j][r]) : PT::mma(a[r], ring[p][j], acc[j][r]);
-            if (k + P < kb1) ring[p][j] = PT::load_b(Bp, KBtot, nb0 + j * nbstep, k + P, lane);
+              acc[j][r] = TR ? PT::mma(ring.r[p][j], a[r], acc[j][r]) : PT::mma(a[r], ring.r[p][j], acc[j][r]);
           }
-        }
+      }
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        const int nb = j < ncb ? nb0 + j * nbstep : nb0;
+        ring.r[p][j] = PT::load_b(Bp, KBtot, nb, k + P < klast ? k + P : klast, lane);
       }
     }
   }
 }
 
+template <class PT, int MB, int CB, int P, bool TR = false>
+__device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, const typename PT::T* Bp, int KBtot,
+                                           int nb0, int nbstep, int ncb, int kb0, int kb1, int a_kb0,
+                                           f32x16_t (&acc)[CB][MB], int lane) {
+  BRing<PT, CB, P> ring;
+  panel_prefetch(ring, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, lane);
+  panel_run<PT, MB, CB, P, TR>(ring, As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, a_kb0, acc, lane);
+}
+
 constexpr int PF_DEPTH = 4;  // B-fragment prefetch depth (k-steps); the kernels run 2-3 waves per SIMD
+// the adjoint runs about one wave per SIMD at training batch sizes: a deeper
+// ring in the registers its packed tanh derivatives free
+constexpr int BWD_PF = 4;
+constexpr int BWD_PF_R = 4;
 // the pass kernel fits an 8-deep ring in its three-waves-per-SIMD budget (167 VGPRs)
 constexpr int PASS_PF = 8;
 
@@ -773,10 +812,13 @@ struct BwdSmem {
     o += rup(MT * ldx * 4, 16);
     glp = o;
     o += rup(MT * 4, 16);
+    // with the register spline, the hidden buffer that does not hold the
+    // last hidden layer also stages every other round's parameter gradients
+    const int hbb = rup(MT * (dm.reg_spline() && lg > lh ? lg : lh) * tb, 16);
     hb1 = o;
-    o += rup(MT * lh * tb, 16);
+    o += hbb;
     hb2 = o;
-    o += rup(MT * lh * tb, 16);
+    o += hbb;
     ob = o;  // spline parameters of a chunk (chunked path only)
     o += dm.reg_spline() ? 0 : rup(MT * lo_ * 4, 16);
     gpb = o;
