@@ -225,13 +225,41 @@ __device__ __forceinline__ void panel_run(BRing<PT, CB, P>& ring, const typename
   }
 }
 
+// The guarded form (loads only inside the range): for kernels that run
+// several waves per SIMD, where the other waves hide the group-top wait and
+// the clamped form's extra loads of short k-ranges would cost L1 bandwidth
+// (the flow pass kernel, three waves per SIMD).
 template <class PT, int MB, int CB, int P, bool TR = false>
 __device__ __forceinline__ void mfma_panel(const typename PT::T* As, int lda, const typename PT::T* Bp, int KBtot,
                                            int nb0, int nbstep, int ncb, int kb0, int kb1, int a_kb0,
                                            f32x16_t (&acc)[CB][MB], int lane) {
-  BRing<PT, CB, P> ring;
-  panel_prefetch(ring, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, lane);
-  panel_run<PT, MB, CB, P, TR>(ring, As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, a_kb0, acc, lane);
+  using AF = decltype(PT::load_a(As, 0, 0, 0, 0));
+  typename PT::BF ring[P][CB];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int j = 0; j < CB; ++j)
+      if (kb0 + p < kb1 && j < ncb) ring[p][j] = PT::load_b(Bp, KBtot, nb0 + j * nbstep, kb0 + p, lane);
+  for (int kb = kb0; kb < kb1; kb += P) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int k = kb + p;
+      if (k < kb1) {
+        AF a[MB];
+#pragma unroll
+        for (int r = 0; r < MB; ++r) a[r] = PT::load_a(As, lda, r * 32, k - kb0 + a_kb0, lane);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+          if (j < ncb) {
+#pragma unroll
+            for (int r = 0; r < MB; ++r)
+              acc[j][r] = TR ? PT::mma(ring[p][j], a[r], acc[j][r]) : PT::mma(a[r], ring[p][j], acc[j][r]);
+            if (k + P < kb1) ring[p][j] = PT::load_b(Bp, KBtot, nb0 + j * nbstep, k + P, lane);
+          }
+        }
+      }
+    }
+  }
 }
 
 constexpr int PF_DEPTH = 4;  // B-fragment prefetch depth (k-steps); the kernels run 2-3 waves per SIMD
