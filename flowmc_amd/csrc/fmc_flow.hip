@@ -16,6 +16,8 @@
 // src/flowMC/resource/optimizer.py:19-22.
 #include "fmc_flow_common.h"
 
+#include <type_traits>
+
 namespace fmc {
 
 // --------------------------------------------------- weight gradients ----
@@ -85,7 +87,8 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
   __shared__ float red[WG_SPLIT][16][64];
   __shared__ float rsum[WG_SPLIT][64];
   const FlowDims& dm = f.dm;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index in a scalar register: branches on it are uniform
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int total = tab.total;
   const int per = (total + WG_XCDS - 1) / WG_XCDS;
   int tt = (int)(blockIdx.x % WG_XCDS) * per + (int)(blockIdx.x / WG_XCDS);
@@ -208,6 +211,241 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
       if (idx >= 0) grad[idx] = b;
     }
   }
+}
+
+// ----------------------------------------- split weight gradients ----
+// gW = G^T Act of one layer as a split-K product: the batch (the reduction
+// axis, 10^4 samples) is cut into S ranges and every (block, range) pair is
+// one workgroup of 8 waves. A block is WS_N G row blocks x up to WS_K A row
+// blocks (128 x 256 elements of gW); per batch slice the block's 12 store
+// fragments are staged in LDS once (double-buffered, WS_KS slices a stage)
+// and wave w multiplies G block w/2 with A blocks 4(w%2) .. +3, so a 1 KB
+// fragment read from L2 feeds 2.7 MFMAs instead of 0.5. The FP32 partial
+// blocks go to a workspace and flow_wgrad_reduce_kernel sums the S partials
+// of each element in a fixed order (deterministic, like the one-workgroup-
+// per-tile kernel above, whose sums it matches to rounding).
+constexpr int WS_N = 4, WS_K = 8, WS_KS = 4, WS_FR = WS_N + WS_K, WS_WAVES = 8;
+constexpr int WS_PART = WS_N * 32 * WS_K * 32 + WS_N * 32;  // floats of one partial (block + bias rows)
+
+struct WsTable {
+  WgTable wt;                 // the layer's problems
+  int blk_start[WG_MAXP + 1];  // first block of problem i
+  int nbk[WG_MAXP];           // k-chunks of problem i (blocks = n-chunks x k-chunks)
+  int n_blocks, S;
+};
+
+static WsTable wsplit_table(const FlowDims& dm, int l, int64_t ld, int max_wg) {
+  WsTable t{};
+  t.wt = wgrad_table(dm, l);
+  int b = 0;
+  for (int i = 0; i <= dm.nh; ++i) {
+    const int n_g = (t.wt.start[i + 1] - t.wt.start[i]) / (t.wt.n_a[i] / 32) * 32;  // G rows of problem i
+    const int nbn = (n_g / 32 + WS_N - 1) / WS_N, nbk = (t.wt.n_a[i] / 32 + WS_K - 1) / WS_K;
+    t.blk_start[i] = b;
+    t.nbk[i] = nbk;
+    b += nbn * nbk;
+  }
+  for (int i = dm.nh + 1; i <= WG_MAXP; ++i) t.blk_start[i] = b;
+  t.n_blocks = b;
+  const int ns = (int)(ld / 16);  // batch slices of a BF16 store (FP32: 8x as many; the split stays)
+  // S a multiple of the XCD count where the grid allows: workgroup b runs on
+  // XCD b % 8 and the workgroups of batch range s (all blocks) then share
+  // XCD s % 8, so the A-row slices every block of that range reads come
+  // from one L2 (a range spread over eight XCDs was read from memory by each)
+  int S = max_wg / b;
+  if (S >= WG_XCDS) S -= S % WG_XCDS;
+  S = S < 1 ? 1 : S;
+  const int smax = ns / (2 * WS_KS);  // at least two stages per range
+  S = S > smax ? (smax < 1 ? 1 : smax) : S;
+  t.S = S;
+  return t;
+}
+
+template <class PT>
+__global__ void __launch_bounds__(64 * WS_WAVES) flow_wgrad_split_kernel(FlowDev f, WsTable tab, const void* store,
+                                                                         int64_t ld, float* part) {
+  using T = typename PT::T;
+  using GF = decltype(PT::load_g(static_cast<const T*>(nullptr), 0, 0, 0, 0));
+  constexpr int FB = 64 * (int)sizeof(GF);  // bytes of one fragment
+  extern __shared__ __align__(16) unsigned char wsm[];
+  const FlowDims& dm = f.dm;
+  // the wave index in a scalar register: branches on it are uniform
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int S = tab.S;
+  const int blk = blockIdx.x / S, sidx = blockIdx.x - blk * S;
+  int prob = 0;
+#pragma unroll
+  for (int i = 1; i < WG_MAXP; ++i) prob += (blk >= tab.blk_start[i]) ? 1 : 0;
+  const int nbk = pick(tab.nbk, prob);
+  const int lb = blk - pick(tab.blk_start, prob);
+  const int nch = lb / nbk, kch = lb - nch * nbk;
+  const int n_a = pick(tab.wt.n_a, prob);
+  const int n_g = (pick(tab.wt.start, prob + 1) - pick(tab.wt.start, prob)) / (n_a / 32) * 32;
+  const int nbs = (n_g / 32 - nch * WS_N) < WS_N ? (n_g / 32 - nch * WS_N) : WS_N;  // G row blocks of the block
+  const int kbs = (n_a / 32 - kch * WS_K) < WS_K ? (n_a / 32 - kch * WS_K) : WS_K;  // A row blocks
+  const int gRb0 = pick(tab.wt.g_row, prob) / 32 + nch * WS_N, aRb0 = pick(tab.wt.a_row, prob) / 32 + kch * WS_K;
+  const T* st = reinterpret_cast<const T*>(store);
+  const int ns = (int)(ld / PT::KSTEP);
+  const int s0 = (int)((int64_t)sidx * ns / S), s1 = (int)((int64_t)(sidx + 1) * ns / S);
+  const int nstage = (s1 - s0 + WS_KS - 1) / WS_KS;
+  // wave roles: G block wn, A blocks wk0 .. wk0+3
+  const int wn = wave >> 1, wk0 = (wave & 1) * 4;
+  const bool want_bias = (wave & 1) == 0 && kch == 0;
+  // this wave's staged fragments: f = wave + 8 u (u < 6) of the stage's 48
+  constexpr int NU = WS_KS * WS_FR / WS_WAVES;
+  static_assert(WS_KS * WS_FR % WS_WAVES == 0, "stage fragments split evenly over the waves");
+  // two register sets of staged fragments: the loads of stage s + 2 are
+  // issued before stage s is multiplied and written to LDS after stage s + 1
+  // is, so each stage's loads have two stages' MFMAs to land. Every load is
+  // unconditional (stages past the range repeat its last slice), so the
+  // wait before a stage's LDS write covers that stage's loads only.
+  GF rg[2][NU];
+  auto load_stage = [&](GF (&dst)[NU], int stg) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int fr = wave + WS_WAVES * u, j = fr / WS_FR, which = fr - j * WS_FR;
+      int sl = s0 + stg * WS_KS + j;
+      sl = sl < s1 ? sl : s1 - 1;
+      const int rb = which < WS_N ? gRb0 + (which < nbs ? which : nbs - 1)
+                                  : aRb0 + (which - WS_N < kbs ? which - WS_N : kbs - 1);
+      dst[u] = PT::load_g(st, ns, rb, sl, lane);
+    }
+  };
+  auto put_stage = [&](const GF (&src)[NU], int buf) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int fr = wave + WS_WAVES * u;
+      *reinterpret_cast<GF*>(wsm + ((size_t)(buf * WS_KS * WS_FR + fr) * 64 + lane) * sizeof(GF)) = src[u];
+    }
+  };
+  auto frag = [&](int buf, int j, int which) -> GF {
+    return *reinterpret_cast<const GF*>(wsm + ((size_t)((buf * WS_KS + j) * WS_FR + which) * 64 + lane) * sizeof(GF));
+  };
+  f32x16_t acc[4] = {};
+  float bsum = 0.f;
+  load_stage(rg[0], 0);
+  load_stage(rg[1], 1);
+  put_stage(rg[0], 0);
+  __syncthreads();
+  // one stage on LDS buffer / register set CUR (a template argument, so the
+  // register sets are indexed statically)
+  auto stage = [&](auto cur_c, int stg) {
+    constexpr int cur = decltype(cur_c)::value;
+    load_stage(rg[cur], stg + 2);  // (rg[cur] was written to LDS at the previous stage)
+#pragma unroll
+    for (int j = 0; j < WS_KS; ++j) {
+      if (s0 + stg * WS_KS + j < s1 && wn < nbs) {
+        const GF g = frag(cur, j, wn);
+        if (want_bias) bsum += PT::hsum(g);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+          if (wk0 + kk < kbs) acc[kk] = PT::mma(g, frag(cur, j, WS_N + wk0 + kk), acc[kk]);
+      }
+    }
+    put_stage(rg[cur ^ 1], cur ^ 1);
+    __syncthreads();
+  };
+  // an even number of stages (a padding stage multiplies nothing)
+  for (int stg = 0; stg < nstage; stg += 2) {
+    stage(std::integral_constant<int, 0>{}, stg);
+    stage(std::integral_constant<int, 1>{}, stg + 1);
+  }
+  // partial block [n_local < 128][k_local < 256] and bias rows [128]
+  float* pp = part + (size_t)blockIdx.x * WS_PART;
+  if (wn < nbs) {
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = (wk0 + kk) * 32 + (lane & 31);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) pp[(wn * 32 + out_row(v, lane)) * (WS_K * 32) + k] = acc[kk][v];
+    }
+    if (!(wave & 1)) {
+      // lane l and l+32 hold the two halves of G row wn*32 + l%32
+      const float b = bsum + __shfl_xor(bsum, 32, 64);
+      if (lane < 32) pp[WS_N * 32 * WS_K * 32 + wn * 32 + lane] = b;
+    }
+  }
+}
+
+// Fixed-order sum of the S partials of every element of every block, mapped
+// to theta (as the epilogue of flow_wgrad_kernel), plus the ScalarAffine sums.
+__global__ void __launch_bounds__(256) flow_wgrad_reduce_kernel(FlowDev f, int l, WsTable tab, const float* part,
+                                                                const float* aff_part, int n_part, float* grad) {
+  const FlowDims& dm = f.dm;
+  if (blockIdx.x == gridDim.x - 1) {
+    // ScalarAffine gradients: thread t sums a contiguous run of the
+    // partials, then a fixed pairwise tree over the 256 runs
+    __shared__ float r0[256], r1[256];
+    const int t = threadIdx.x, chunk = (n_part + 255) / 256;
+    float s0 = 0.f, s1 = 0.f;
+    for (int i = t * chunk; i < (t + 1) * chunk && i < n_part; ++i) {
+      s0 += aff_part[2 * i];
+      s1 += aff_part[2 * i + 1];
+    }
+    r0[t] = s0;
+    r1[t] = s1;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) {
+        r0[t] += r0[t + h];
+        r1[t] += r1[t + h];
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      grad[dm.th_scale(l)] = r0[0];
+      grad[dm.th_shift(l)] = r1[0];
+    }
+    return;
+  }
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int blk = (int)(e / WS_PART), q = (int)(e - (int64_t)blk * WS_PART);
+  if (blk >= tab.n_blocks) return;
+  int prob = 0;
+#pragma unroll
+  for (int i = 1; i < WG_MAXP; ++i) prob += (blk >= tab.blk_start[i]) ? 1 : 0;
+  const int nbk = pick(tab.nbk, prob);
+  const int lb = blk - pick(tab.blk_start, prob);
+  const int nch = lb / nbk, kch = lb - nch * nbk;
+  const int n_a = pick(tab.wt.n_a, prob);
+  const int n_g = (pick(tab.wt.start, prob + 1) - pick(tab.wt.start, prob)) / (n_a / 32) * 32;
+  const bool bias = q >= WS_N * 32 * WS_K * 32;
+  const int nl = bias ? q - WS_N * 32 * WS_K * 32 : q / (WS_K * 32), kl = bias ? 0 : q - nl * (WS_K * 32);
+  const int n = nch * WS_N * 32 + nl, k = kch * WS_K * 32 + kl;
+  if (n >= n_g || k >= n_a || (bias && kch != 0)) return;
+  const int S = tab.S;
+  const float* pp = part + (size_t)blk * S * WS_PART + q;
+  // loads of 8 partials issued together, summed in the fixed order 0 .. S-1
+  float t = 0.f;
+  int si = 0;
+  for (; si + 8 <= S; si += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = pp[(size_t)(si + u) * WS_PART];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += v[u];
+  }
+  for (; si < S; ++si) t += pp[(size_t)si * WS_PART];
+  const int64_t thw = pick(tab.wt.th_w, prob), thb = pick(tab.wt.th_b, prob);
+  const int n_out = pick(tab.wt.n_out, prob), in_w = pick(tab.wt.in_w, prob);
+  const bool out_layer = prob == dm.nh;
+  int64_t idx = -1;
+  if (bias) {
+    if (out_layer) {
+      const int row = dm.gp_row(l, n);
+      if (row >= 0) idx = thb + row;
+    } else if (n < n_out) {
+      idx = thb + n;
+    }
+  } else if (out_layer) {
+    const int row = dm.gp_row(l, n);
+    if (row >= 0 && k < in_w) idx = thw + (int64_t)row * in_w + k;
+  } else if (prob == 0) {
+    if (n < n_out && k < dm.dc(l)) idx = thw + (int64_t)n * in_w + dm.cond(l, k);
+  } else {
+    if (n < n_out && k < in_w) idx = thw + (int64_t)n * in_w + k;
+  }
+  if (idx >= 0) grad[idx] = t;
 }
 
 // ------------------------------------------------------------- packing ---
@@ -549,6 +787,71 @@ extern "C" int fmc_flow_weight_grad(void* stream, const fmc_flow* fl, int32_t la
     hipLaunchKernelGGL(flow_wgrad_kernel<PF32>, g, b, 0, st, fd, layer, tab, store, store_ld, affine_partials,
                        (int)(store_ld / 32), grad);
   return check_launch("flow weight grad");
+}
+
+// workgroups of the split weight gradient: one per CU (96 KB of LDS each)
+static int wsplit_max_wg() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0, cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
+      n = cu;
+    else
+      n = 256;
+  }
+  return n;
+}
+
+extern "C" int fmc_flow_weight_grad_ws_size(const fmc_flow* fl, int64_t store_ld, int64_t* bytes) {
+  FlowDev fd;
+  int rc = check_flow(fl, fd);
+  if (rc) return rc;
+  FMC_ARG(bytes != nullptr, "bytes is NULL");
+  FMC_ARG(store_ld >= 64 && store_ld % 64 == 0, "store_ld must be a positive multiple of 64");
+  int64_t mx = 0;
+  for (int l = 0; l < (fd.dm.L < 2 ? fd.dm.L : 2); ++l) {  // layers alternate between two shapes
+    const WsTable t = wsplit_table(fd.dm, l, store_ld, wsplit_max_wg());
+    const int64_t b = (int64_t)t.n_blocks * t.S * WS_PART * (int64_t)sizeof(float);
+    mx = b > mx ? b : mx;
+  }
+  *bytes = mx;
+  return FMC_OK;
+}
+
+extern "C" int fmc_flow_weight_grad_split(void* stream, const fmc_flow* fl, int32_t layer, const void* store,
+                                          int64_t store_ld, const float* affine_partials, int32_t n_partials,
+                                          float* grad, float* workspace, int64_t workspace_bytes) {
+  FlowDev fd;
+  int rc = check_flow(fl, fd);
+  if (rc) return rc;
+  FMC_ARG(layer >= 0 && layer < fl->n_layers, "layer out of range");
+  FMC_ARG(store && affine_partials && grad && workspace, "NULL pointer");
+  FMC_ARG(store_ld >= 64 && store_ld % 64 == 0, "store_ld must be a positive multiple of 64");
+  FMC_ARG(n_partials >= store_ld / 32, "affine_partials holds %d slots, need %lld", n_partials,
+          (long long)(store_ld / 32));
+  const WsTable tab = wsplit_table(fd.dm, layer, store_ld, wsplit_max_wg());
+  const int64_t need = (int64_t)tab.n_blocks * tab.S * WS_PART * (int64_t)sizeof(float);
+  FMC_ARG(workspace_bytes >= need, "workspace holds %lld bytes, need %lld (fmc_flow_weight_grad_ws_size)",
+          (long long)workspace_bytes, (long long)need);
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 g((unsigned)(tab.n_blocks * tab.S)), b(64 * WS_WAVES);
+  if (fl->precision == FMC_PREC_BF16) {
+    const size_t shm = (size_t)2 * WS_KS * WS_FR * 64 * 16;
+    set_lds(flow_wgrad_split_kernel<PBF16>, shm);
+    hipLaunchKernelGGL(flow_wgrad_split_kernel<PBF16>, g, b, shm, st, fd, tab, store, store_ld, workspace);
+  } else {
+    const size_t shm = (size_t)2 * WS_KS * WS_FR * 64 * 4;
+    set_lds(flow_wgrad_split_kernel<PF32>, shm);
+    hipLaunchKernelGGL(flow_wgrad_split_kernel<PF32>, g, b, shm, st, fd, tab, store, store_ld, workspace);
+  }
+  rc = check_launch("flow weight grad (split)");
+  if (rc) return rc;
+  const int64_t ne = (int64_t)tab.n_blocks * WS_PART;
+  // (one more workgroup than the elements need: the ScalarAffine sums)
+  hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((unsigned)((ne + 255) / 256 + 1)), dim3(256), 0, st, fd, layer,
+                     tab, workspace, affine_partials, (int)(store_ld / 32), grad);
+  return check_launch("flow weight grad (reduce)");
 }
 
 extern "C" int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n, float lr,

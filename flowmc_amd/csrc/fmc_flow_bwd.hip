@@ -88,7 +88,9 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
   float* obuf = reinterpret_cast<float*>(smem + sm.ob);
   T* gpb = reinterpret_cast<T*>(smem + sm.gpb);
   float* red = reinterpret_cast<float*>(smem + sm.red);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index in a scalar register: branches on it (column blocks,
+  // dim pairs) are uniform, not exec-masked
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int d = dm.d, ldx = sm.ldx, l = a.layer;
   const int64_t m0 = (int64_t)blockIdx.x * MT;
   const float* th = f.theta;

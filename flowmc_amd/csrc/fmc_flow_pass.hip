@@ -28,7 +28,8 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   // parameters use the region the last hidden layer is not in
   T* hlast = ((NH - 1) & 1) ? reg2 : reg1;
   float* obuf = reinterpret_cast<float*>(((NH - 1) & 1) ? reg1 : reg2);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index in a scalar register: branches on it are uniform
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int d = dm.d, ldx = sm.ldx;
   const int64_t m0 = (int64_t)blockIdx.x * MT;
   const float* th = f.theta;

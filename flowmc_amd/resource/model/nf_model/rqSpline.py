@@ -60,11 +60,25 @@ class _Workspace:
             self.ev_adj = [torch.cuda.Event() for _ in range(L)]
             self.ev_wg = [torch.cuda.Event() for _ in range(L)]
         self.grad = torch.zeros(model.theta.numel(), dtype=torch.float32, device=dev)
+        # partial blocks of the split weight-gradient product (fmc_flow_weight_grad_split)
+        nbytes = C.c_int64(0)
+        _native.call("fmc_flow_weight_grad_ws_size", C.byref(model._struct()), self.ld, C.byref(nbytes))
+        self.wg_ws = torch.empty(max(1, nbytes.value // 4), dtype=torch.float32, device=dev)
         self.norm_ws = torch.empty(1024, dtype=torch.float64, device=dev)
         self.gnorm = torch.empty(1, dtype=torch.float32, device=dev)
 
 
 class MaskedCouplingRQSpline(NFModel):
+    # weight gradients by the split-K product over the batch (one workgroup
+    # of 96 KB of LDS per block and batch range, fmc_flow_weight_grad_split)
+    # or by one workgroup per 32 x 32 tile (fmc_flow_weight_grad); equal to
+    # rounding. None: the split product unless the weight gradient overlaps
+    # the next adjoint on a side stream (hidden widths <= 128), where the
+    # per-tile kernel shares the CUs better (batch 10k: 10 layers of d=64,
+    # [256, 256]: 30 vs 40 us per layer; d=32, [128, 128] overlapped: the
+    # per-tile kernel 3 % faster per training step)
+    wgrad_split = None
+
     def __repr__(self):
         return "MaskedCouplingRQSpline with n_features=" + str(self._n_features) + ", n_layers=" + str(
             self.n_layers)
@@ -383,8 +397,14 @@ class MaskedCouplingRQSpline(NFModel):
             if ws.overlap:
                 ws.ev_adj[layer].record(main)
                 ws.side.wait_event(ws.ev_adj[layer])
-            _native.call("fmc_flow_weight_grad", st_wg, C.byref(s), layer, ws.store[b].data_ptr(), ws.ld,
-                         ws.aff[b].data_ptr(), ws.n_part, ws.grad.data_ptr())
+            split = self.wgrad_split if self.wgrad_split is not None else not ws.overlap
+            if split:
+                _native.call("fmc_flow_weight_grad_split", st_wg, C.byref(s), layer, ws.store[b].data_ptr(), ws.ld,
+                             ws.aff[b].data_ptr(), ws.n_part, ws.grad.data_ptr(), ws.wg_ws.data_ptr(),
+                             ws.wg_ws.numel() * 4)
+            else:
+                _native.call("fmc_flow_weight_grad", st_wg, C.byref(s), layer, ws.store[b].data_ptr(), ws.ld,
+                             ws.aff[b].data_ptr(), ws.n_part, ws.grad.data_ptr())
             if ws.overlap:
                 ws.ev_wg[layer].record(ws.side)
                 last_read[b] = ws.ev_wg[layer]

@@ -470,6 +470,16 @@ int fmc_flow_backward_layer(void* stream, const fmc_flow* flow, int32_t layer,
 int fmc_flow_weight_grad(void* stream, const fmc_flow* flow, int32_t layer, const void* store,
                          int64_t store_ld, const float* affine_partials, int32_t n_partials,
                          float* grad);
+/* The same gradients as a split-K product over the batch (the same
+ * eqx.filter_value_and_grad in NFModel.train_step, nf_model/base.py:98-125):
+ * one workgroup per (128 x 256 block of a weight matrix, batch range), FP32
+ * partial blocks in workspace, then a fixed-order sum of the partials
+ * (deterministic; equal to fmc_flow_weight_grad up to FP32 rounding).
+ * workspace: at least fmc_flow_weight_grad_ws_size(flow, store_ld) bytes. */
+int fmc_flow_weight_grad_ws_size(const fmc_flow* flow, int64_t store_ld, int64_t* bytes);
+int fmc_flow_weight_grad_split(void* stream, const fmc_flow* flow, int32_t layer, const void* store,
+                               int64_t store_ld, const float* affine_partials, int32_t n_partials,
+                               float* grad, float* workspace, int64_t workspace_bytes);
 
 /* optax.chain(clip_by_global_norm(max_norm), adamw(lr, b1, b2, eps, weight_decay))
  * applied to the flat vector theta (src/flowMC/resource/optimizer.py:19-22);

@@ -226,6 +226,37 @@ def test_flow_adjoint_matches_oracle_autograd(gpu, prec, case):
 
 
 @pytest.mark.parametrize("prec", ["float32", "bfloat16"])
+@pytest.mark.parametrize("case,B", [(c, 200) for c in ADJ_CASES] + [((64, 2, 8, (256, 256)), 10000),
+                                                                   ((32, 2, 10, (128, 128)), 10000)])
+def test_flow_split_weight_gradient_matches_tile_kernel(gpu, prec, case, B):
+    """The split-K weight gradient (batch ranges per workgroup, fixed-order
+    sum of FP32 partials) against the one-workgroup-per-tile kernel: the same
+    products of the same store fragments summed in another order, so equal to
+    FP32 rounding (1e-5 of the largest gradient); the split one is also
+    bit-identical run to run."""
+    d, L, K, H = case
+    m, _ = make(d, L, K, H, prec, seed=8)
+    g = torch.Generator().manual_seed(9)
+    x = (torch.randn(B, d, generator=g) * 2).float().cuda()
+    w = (torch.randn(B, generator=g) / B).float().cuda()
+    try:
+        type(m).wgrad_split = False
+        gt, gxt = m.grad_log_prob_sum(x, w)
+        type(m).wgrad_split = True
+        gs, gxs = m.grad_log_prob_sum(x, w)
+        gs2, _ = m.grad_log_prob_sum(x, w)
+    finally:
+        type(m).wgrad_split = None
+    assert torch.equal(gs, gs2)
+    assert torch.equal(gxt, gxs)  # dL/dx does not involve the weight gradient
+    n = m.n_layers * m._layer_size
+    scale = float(gt[:n].abs().max())
+    err = float((gs[:n] - gt[:n]).abs().max())
+    assert err <= 1e-5 * scale, (err, scale)
+    assert float(gs[n:].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("prec", ["float32", "bfloat16"])
 def test_flow_gradient_is_additive_over_batch_tiles(gpu, prec):
     """The gradient of a 40k-sample batch (1250 adjoint workgroups, several
     per CU, split-K weight gradients over 8 wavefronts) equals the sum of the
