@@ -279,12 +279,23 @@ def test_flow_gradient_is_additive_over_batch_tiles(gpu, prec):
     assert float((gx - gxs).abs().max()) <= 1e-6 * float(gxs.abs().max())
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("hidden", [128, 256])
-def test_flow_gradient_independent_of_the_backward_schedule(gpu, hidden):
+def test_flow_gradient_independent_of_the_backward_schedule(gpu, hidden, split):
     """Weight gradients on a side stream beside the next adjoint (hidden <= 128)
-    or on one stream: the same kernels on the same data, so the same bits."""
+    or on one stream: the same kernels on the same data, so the same bits
+    (the weight-gradient kernel pinned: the automatic choice follows the
+    schedule)."""
     d, L, K = 32, 4, 10
     m, _ = make(d, L, K, (hidden, hidden), "bfloat16", seed=6)
+    type(m).wgrad_split = split
+    try:
+        _schedule_bits(m, d, L, hidden)
+    finally:
+        type(m).wgrad_split = None
+
+
+def _schedule_bits(m, d, L, hidden):
     g = torch.Generator().manual_seed(7)
     B = 10000
     x = (torch.randn(B, d, generator=g) * 2).float().cuda()
