@@ -67,6 +67,13 @@ def flow_layer_flops(d, layer, K, hidden):
     return 2 * tot
 
 
+def flow_wgrad_macs(d, layer, K, hidden):
+    """Multiply-adds per sample of one coupling layer's weight gradients
+    (dW = G^T A for every linear layer: rows x cols each, the same count as
+    the layer's forward product)."""
+    return flow_layer_flops(d, layer, K, hidden) // 2
+
+
 def flow_flops_per_sample(d, L, K, h1, h2=None):
     """Minimal conditioner FLOPs of one pass (hidden widths (h1, h2) or the
     list h1): the sum of flow_layer_flops over the layers."""
@@ -261,11 +268,12 @@ def bench_nfmcmc(args):
     global_acc = float(ga[torch.isfinite(ga)].mean())
     value = an["min_ess_bulk"] / wall
     occupancy = mode_occupancy(res["positions_production"].data, target)
-    # Roofline of the dominant kernel of the timed region: the flow adjoint
-    # (per layer, 10k-sample training batches; the largest share of the run's
-    # GPU time), timed alone with HIP events on its stream. Its algorithmic
-    # FLOPs per launch are twice the layer's forward (recompute + transposed
-    # products) over the batch.
+    # Rooflines of the three kernels that take ~90 % of the timed region's
+    # kernel time, each timed alone with HIP events on its stream: the flow
+    # adjoint of one layer at the training batch (algorithmic FLOPs: twice the
+    # layer's forward over the batch), the layer's weight gradients (the
+    # forward's multiply-adds, twice, over the batch) and the flow pass over
+    # the proposals of a global step.
     import ctypes as _C
 
     import flowmc_amd._native as N
@@ -294,6 +302,17 @@ def bench_nfmcmc(args):
     if os.path.exists(tfa) and B == 10000:
         with open(tfa) as f:
             traffic_adj = json.load(f).get("bytes_per_launch")
+    # the weight gradients of the same layer (split-K product + fixed-order
+    # sum of the partials): 2 * rows * cols FLOPs per sample and matrix
+    adjoint()
+
+    def wgrad():
+        N.call("fmc_flow_weight_grad_split", N.stream_ptr(), _C.byref(fs), lay, ws.store[0].data_ptr(), ws.ld,
+               ws.aff[0].data_ptr(), ws.n_part, ws.grad.data_ptr(), ws.wg_ws.data_ptr(), ws.wg_ws.numel() * 4)
+
+    ms_wg = time_kernel(wgrad, reps=20)
+    fl_wg = 2 * flow_wgrad_macs(d, lay, K, H) * B
+    ach_wg = fl_wg / (ms_wg * 1e-3) / 1e12
     # the next kernel: one flow inverse pass over the C*S proposals of a global step
     S = cfg["n_global"]
     ms = time_kernel(lambda: model.sample(fr.PRNGKey(3), C_ * S), reps=3)
@@ -307,17 +326,27 @@ def bench_nfmcmc(args):
     out = {
         "value": value,
         "ms_per_step": wall * 1e3 / (2 * args.steps),
-        "roofline": {"bound": "mfma", "achieved": ach_adj, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach_adj / BF16_PEAK_TFLOPS, "traffic": traffic_adj,
-                     "kernel": f"flow_bwd_kernel_h256 (adjoint of one coupling layer, {B}-sample batch, BF16)",
-                     "kernel_ms": ms_adj, "flops_per_launch": fl_adj},
+        # the flow pass has the largest share of the timed region's kernel
+        # time (600 training forwards at batch 10k, 40 sampling passes over
+        # the 409,600 proposals of a global step, 40 log-densities); its
+        # roofline is quoted at the sampling shape, the adjoint and the
+        # weight gradients (the next two) beside it
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "flow_pass_kernel (exact inverse over the 409,600 proposals of a global step, BF16)",
+                     "kernel_ms": ms, "flops_per_launch": fl},
         "extra": {"wall_s": wall, **an, "local_acceptance": local_acc, "global_acceptance": global_acc,
                   "mode_occupancy": occupancy,
-                  "roofline_flow_pass": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
-                                         "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic,
-                                         "kernel": "flow_pass_kernel (exact inverse over the 409,600 proposals "
-                                                   "of a global step, BF16)",
-                                         "kernel_ms": ms, "flops_per_launch": fl}},
+                  "roofline_adjoint": {"bound": "mfma", "achieved": ach_adj, "peak": BF16_PEAK_TFLOPS,
+                                       "unit": "TFLOP/s", "frac": ach_adj / BF16_PEAK_TFLOPS, "traffic": traffic_adj,
+                                       "kernel": f"flow_bwd_kernel_h256 (adjoint of one coupling layer, {B}-sample "
+                                                 "batch, BF16)",
+                                       "kernel_ms": ms_adj, "flops_per_launch": fl_adj},
+                  "roofline_weight_grad": {"bound": "mfma", "achieved": ach_wg, "peak": BF16_PEAK_TFLOPS,
+                                           "unit": "TFLOP/s", "frac": ach_wg / BF16_PEAK_TFLOPS,
+                                           "kernel": "flow_wgrad_split_kernel + flow_wgrad_reduce_kernel (weight "
+                                                     f"gradients of one coupling layer, {B}-sample batch)",
+                                           "kernel_ms": ms_wg, "flops_per_launch": fl_wg}},
         "config": {"workload": "nfmcmc_gmm64", "chains": C_, "dim": d, "modes": 8, "mala_step": cfg["step_size"],
                    "flow": {"layers": L, "bins": K, "hidden": H, "precision": args.precision,
                             "exact_inverse": cfg["exact_inverse"]},
