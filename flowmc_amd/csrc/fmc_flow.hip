@@ -790,18 +790,7 @@ extern "C" int fmc_flow_weight_grad(void* stream, const fmc_flow* fl, int32_t la
 }
 
 // workgroups of the split weight gradient: one per CU (96 KB of LDS each)
-static int wsplit_max_wg() {
-  static int n = 0;
-  if (n == 0) {
-    int dev = 0, cu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
-      n = cu;
-    else
-      n = 256;
-  }
-  return n;
-}
+static int wsplit_max_wg() { return device_cus(); }
 
 extern "C" int fmc_flow_weight_grad_ws_size(const fmc_flow* fl, int64_t store_ld, int64_t* bytes) {
   FlowDev fd;

@@ -862,6 +862,20 @@ struct BwdSmem {
 inline int spline_kt(int K) { return K == 8 ? 8 : (K == 10 ? 10 : 0); }
 
 // --------------------------------------------------------------- host ----
+// compute units of the current device (queried once; 256 if the query fails)
+static inline int device_cus() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0, cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
+      n = cu;
+    else
+      n = 256;
+  }
+  return n;
+}
+
 static int check_flow(const fmc_flow* fl, FlowDev& fd) {
   FMC_ARG(fl != nullptr, "flow is NULL");
   FMC_ARG(fl->n_dim >= 2 && fl->n_dim <= 512, "flow n_dim must be in [2, 512] (got %d)", fl->n_dim);

@@ -10,7 +10,26 @@ namespace fmc {
 // SIMD: three workgroups share a CU (LDS ~43 KB each at d=64, [256,256]) and
 // hide each other's LDS / L2 latency; measured faster than 64-sample tiles
 // with more B-fragment reuse per wave.
-template <class PT, int MB, int KT, int NH>
+// One panel of the pass: EX (grids of at most two workgroups per CU, e.g.
+// training batches of 10^4 samples) uses the exact-wait ring, four k-steps
+// deep, which pays where a wave has few neighbours to hide its waits; the
+// guarded 8-deep ring otherwise (sampling passes of 10^5 samples and more,
+// three workgroups per CU, where the clamped ring's extra loads cost more).
+constexpr int PASS_PF_EX = 4;
+template <bool EX, class PT, int MB, bool TR>
+__device__ __forceinline__ void pass_panel(const typename PT::T* As, int lda, const typename PT::T* Bp, int KBtot,
+                                           int nb0, int nbstep, int ncb, int kb0, int kb1, f32x16_t (&acc)[2][MB],
+                                           int lane) {
+  if constexpr (EX) {
+    BRing<PT, 2, PASS_PF_EX> ring;
+    panel_prefetch(ring, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, lane);
+    panel_run<PT, MB, 2, PASS_PF_EX, TR>(ring, As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, 0, acc, lane);
+  } else {
+    mfma_panel<PT, MB, 2, PASS_PF, TR>(As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, 0, acc, lane);
+  }
+}
+
+template <class PT, int MB, int KT, int NH, bool EX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 flow_pass_kernel(FlowDev f, PassArgs a) {
   using T = typename PT::T;
@@ -150,7 +169,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
         for (int c0 = wave; c0 < NBt; c0 += 8) {
           const int ncb = (c0 + 4 < NBt) ? 2 : 1;
           f32x16_t acc[2][MB] = {};
-          mfma_panel<PT, MB, 2, PASS_PF, true>(in, ldin, W + dm.pk_wh(l, i), KB, c0, 4, ncb, 0, KB, 0, acc, lane);
+          pass_panel<EX, PT, MB, true>(in, ldin, W + dm.pk_wh(l, i), KB, c0, 4, ncb, 0, KB, acc, lane);
 #pragma unroll
           for (int j = 0; j < 2; ++j)
             if (j < ncb) hidden_epilogue_tr<PT, MB>(out, sm.lh, bl + dm.pb_b(i), c0 + 4 * j, acc[j], lane);
@@ -173,7 +192,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       const int hf = lane >> 5;
       for (int q = wave; q < npair; q += 4) {
         f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PASS_PF, true>(hlast, sm.lh, W3r, KB, 2 * q, 1, 2, 0, KB, 0, acc, lane);
+        pass_panel<EX, PT, MB, true>(hlast, sm.lh, W3r, KB, 2 * q, 1, 2, 0, KB, acc, lane);
         const int k = 2 * q + hf;
         if (k < dtl) {
           const int j = dm.trans(l, k);
@@ -327,8 +346,13 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
 
 template <class PT, int KT, int NH>
 static void launch_pass(dim3 g, dim3 b, size_t shm, hipStream_t st, const FlowDev& fd, const PassArgs& a) {
-  set_lds(flow_pass_kernel<PT, 1, KT, NH>, shm);
-  hipLaunchKernelGGL((flow_pass_kernel<PT, 1, KT, NH>), g, b, shm, st, fd, a);
+  if ((int)g.x <= 2 * device_cus()) {
+    set_lds(flow_pass_kernel<PT, 1, KT, NH, true>, shm);
+    hipLaunchKernelGGL((flow_pass_kernel<PT, 1, KT, NH, true>), g, b, shm, st, fd, a);
+  } else {
+    set_lds(flow_pass_kernel<PT, 1, KT, NH, false>, shm);
+    hipLaunchKernelGGL((flow_pass_kernel<PT, 1, KT, NH, false>), g, b, shm, st, fd, a);
+  }
 }
 
 template <class PT>
