@@ -257,7 +257,7 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
           }
           const int m = r * 32 + (lane & 31);
           if (valid) {
-            const float gxv = rs_vjp<KT>(pp, K, f, xs[m * ldx + j], gb[m * ldx + j], glp[m], go);
+            const float gxv = rs_vjp<KT, sizeof(T) == 2>(pp, K, f, xs[m * ldx + j], gb[m * ldx + j], glp[m], go);
             gb[m * ldx + j] = gxv;
           } else {
 #pragma unroll

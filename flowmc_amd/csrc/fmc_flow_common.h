@@ -129,6 +129,25 @@ __device__ __forceinline__ float ndiv(float a, float b) {
   return fmaf(fmaf(-b, q, a), r, q);
 }
 
+// Natural log and square root on the hardware instructions (v_log_f32 is
+// log2 to about 1 ulp; v_sqrt_f32 about 1 ulp): every argument in the spline
+// is a positive normal number (bin widths and heights >= the minimum bin
+// size, slopes >= the minimum knot slope, 1 + e^-|v| in [1, 2], the
+// discriminant clamped to FLT_MIN), where the library forms only add
+// denormal scaling and extra-precision steps (12 and 16 instructions).
+// FL = true for the BF16 conditioner; the FP32 conditioner keeps the library
+// forms (its parity bounds are set against FP64 at FP32 accuracy).
+template <bool FL>
+__device__ __forceinline__ float fln(float x) {
+  if constexpr (FL) return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
+  else return __logf(x);
+}
+template <bool FL>
+__device__ __forceinline__ float fsqrt(float x) {
+  if constexpr (FL) return __builtin_amdgcn_sqrtf(x);
+  else return sqrtf(x);
+}
+
 // (m, j) walk over an MT x n tile by the 256 threads of a workgroup, without
 // a division per element: start at tid, advance by 256 elements.
 struct TileWalk {
@@ -522,7 +541,8 @@ struct RsBin {
   float xk, xk1, yk, yk1, uk, uk1;
 };
 
-__device__ __forceinline__ float rs_softplus(float v) { return fmaxf(v, 0.f) + __logf(1.f + __expf(-fabsf(v))); }
+template <bool FL = false>
+__device__ __forceinline__ float rs_softplus(float v) { return fmaxf(v, 0.f) + fln<FL>(1.f + __expf(-fabsf(v))); }
 
 // knots of the bin holding t (t along x when INV is false, along y otherwise)
 // KT > 0: the bin count is the compile-time constant KT (loops unrolled to
@@ -573,18 +593,18 @@ __device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int Kr, const Flo
   return b;
 }
 
-template <int KT>
+template <int KT, bool FL = false>
 __device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowDev& f, float x, float& y, float& ld) {
   if (x <= f.lo || x >= f.hi) {
     const float u = (x <= f.lo) ? pp[20] : pp[30];
     const float e = (x <= f.lo) ? f.lo : f.hi;
-    const float dd = rs_softplus(u + f.off) + f.ms;
+    const float dd = rs_softplus<FL>(u + f.off) + f.ms;
     y = (x - e) * dd + e;
-    ld = __logf(dd);
+    ld = fln<FL>(dd);
     return;
   }
   const RsBin b = rs_bin<false, KT>(pp, K, f, x);
-  const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
+  const float dk = rs_softplus<FL>(b.uk + f.off) + f.ms, dk1 = rs_softplus<FL>(b.uk1 + f.off) + f.ms;
   const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = ndiv(hh, w);
   const float z = fminf(fmaxf(ndiv(x - b.xk, w), 0.f), 1.f);
   const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
@@ -592,21 +612,21 @@ __device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowD
   const float num = hh * (s * z2 + dk * zz);
   const float den = s + st * zz;
   y = b.yk + ndiv(num, den);
-  ld = 2.f * __logf(s) + __logf(dk1 * z2 + 2.f * s * zz + dk * q) - 2.f * __logf(den);
+  ld = 2.f * fln<FL>(s) + fln<FL>(dk1 * z2 + 2.f * s * zz + dk * q) - 2.f * fln<FL>(den);
 }
 
-template <int KT>
+template <int KT, bool FL = false>
 __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowDev& f, float yv, float& x, float& ld) {
   if (yv <= f.lo || yv >= f.hi) {
     const float u = (yv <= f.lo) ? pp[20] : pp[30];
     const float e = (yv <= f.lo) ? f.lo : f.hi;
-    const float dd = rs_softplus(u + f.off) + f.ms;
+    const float dd = rs_softplus<FL>(u + f.off) + f.ms;
     x = (yv - e) / dd + e;
-    ld = -__logf(dd);
+    ld = -fln<FL>(dd);
     return;
   }
   const RsBin b = rs_bin<true, KT>(pp, K, f, yv);
-  const float dk = rs_softplus(b.uk + f.off) + f.ms, dk1 = rs_softplus(b.uk1 + f.off) + f.ms;
+  const float dk = rs_softplus<FL>(b.uk + f.off) + f.ms, dk1 = rs_softplus<FL>(b.uk1 + f.off) + f.ms;
   const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = ndiv(hh, w);
   const float wv = fminf(fmaxf(ndiv(yv - b.yk, hh), 0.f), 1.f);
   const float st = dk1 + dk - 2.f * s;
@@ -614,19 +634,19 @@ __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowD
   const float bq = dk - st * wv;
   const float a = s - bq;
   const float disc = bq * bq - 4.f * a * c;
-  const float sq = disc > 0.f ? sqrtf(fmaxf(disc, 1.17549435e-38f)) : 0.f;
+  const float sq = disc > 0.f ? fsqrt<FL>(fmaxf(disc, 1.17549435e-38f)) : 0.f;
   float z = (bq >= 0.f) ? ndiv(2.f * c, -bq - sq) : ndiv(-bq + sq, 2.f * a);
   z = fminf(fmaxf(z, 0.f), 1.f);
   x = w * z + b.xk;
   const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
   const float den = s + st * zz;
-  ld = -2.f * __logf(s) - __logf(dk1 * z2 + 2.f * s * zz + dk * q) + 2.f * __logf(den);
+  ld = -2.f * fln<FL>(s) - fln<FL>(dk1 * z2 + 2.f * s * zz + dk * q) + 2.f * fln<FL>(den);
 }
 
 // Register-spline forward + vector-Jacobian product (the reverse sweep of
 // spline_vjp): given dL/dy (gy) and dL/dlogdet (gl), writes dL/d(slot
 // logits) to go (zero in padding slots) and returns dL/dx.
-template <int KT>
+template <int KT, bool FL = false>
 __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const FlowDev& f, float x, float gy, float gl,
                                         float (&go)[32]) {
   constexpr int N = KT > 0 ? KT : 10;
@@ -636,7 +656,7 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
   if (x <= f.lo || x >= f.hi) {
     const bool left = x <= f.lo;
     const float u = (left ? pp[20] : pp[30]) + f.off;
-    const float dd = rs_softplus(u) + f.ms;
+    const float dd = rs_softplus<FL>(u) + f.ms;
     const float gd = gy * (x - (left ? f.lo : f.hi)) + ndiv(gl, dd);
     const float gu = ndiv(gd, 1.f + __expf(-u));
     go[20] = left ? gu : 0.f;
@@ -694,7 +714,7 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
   }
   uk += f.off;
   uk1 += f.off;
-  const float dk = rs_softplus(uk) + f.ms, dk1 = rs_softplus(uk1) + f.ms;
+  const float dk = rs_softplus<FL>(uk) + f.ms, dk1 = rs_softplus<FL>(uk1) + f.ms;
   const float w = xk1 - xk, hh = yk1 - yk, s = ndiv(hh, w);
   const float t = ndiv(x - xk, w);
   const float z = fminf(fmaxf(t, 0.f), 1.f);

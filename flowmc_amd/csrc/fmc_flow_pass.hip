@@ -218,9 +218,9 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
             const float xv = xs[m * ldx + j];
             float yv, ldv;
             if (a.dir == 0)
-              rs_fwd<KT>(pp, K, f, xv, yv, ldv);
+              rs_fwd<KT, sizeof(T) == 2>(pp, K, f, xv, yv, ldv);
             else
-              rs_inv<KT>(pp, K, f, xv, yv, ldv);
+              rs_inv<KT, sizeof(T) == 2>(pp, K, f, xv, yv, ldv);
             xs[m * ldx + j] = yv;
             ldacc[r] += ldv;
           }
