@@ -622,11 +622,14 @@ def bench_flow_train(args):
         m.train_step(batches[i % len(batches)], opt, state)
     torch.cuda.synchronize()
     l0 = float(-m.log_prob(data[:20000]).mean())
+    torch.cuda.synchronize()
+    trace_marker()
     t0 = time.perf_counter()
     for i in range(args.steps):
         loss, m, state = m.train_step(batches[i % len(batches)], opt, state)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    trace_marker()
     l1 = float(-m.log_prob(data[:20000]).mean())
     # dominant kernel: the per-layer adjoint + weight-gradient; report the whole step on MFMA
     fl = 3 * flow_flops_per_sample(d, L, K, H) * B
@@ -743,11 +746,14 @@ def bench_realnvp_train(args):
         m.train_step(batches[i % len(batches)], opt, state)
     torch.cuda.synchronize()
     l0 = float(-m.log_prob(data[:20000]).mean())
+    torch.cuda.synchronize()
+    trace_marker()
     t0 = time.perf_counter()
     for i in range(args.steps):
         loss, m, state = m.train_step(batches[i % len(batches)], opt, state)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    trace_marker()
     l1 = float(-m.log_prob(data[:20000]).mean())
     fl_fwd = 4 * H * d * L  # two MLPs, d_c + d_t = d inputs + outputs, per sample
     fl_step = 4 * fl_fwd * B  # forward + adjoint (recompute, two transposed products, two weight products)
