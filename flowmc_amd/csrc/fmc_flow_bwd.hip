@@ -183,7 +183,8 @@ __device__ __forceinline__ void flow_bwd_body(const FlowDev& f, const BwdArgs& a
               for (int r = 0; r < MB; ++r)
 #pragma unroll
                 for (int v = 0; v < 16; v += 2) {
-                  const float h0 = PT::act(acc[j][r][v] + bb), h1 = PT::act(acc[j][r][v + 1] + bb);
+                  float h0, h1;
+                  PT::act2(acc[j][r][v] + bb, acc[j][r][v + 1] + bb, h0, h1);
                   out[(r * 32 + out_row(v, lane)) * sm.lh + col] = PT::cvt(h0);
                   out[(r * 32 + out_row(v + 1, lane)) * sm.lh + col] = PT::cvt(h1);
                   dh[i][ci + j][r].put2(v >> 1, 1.f - h0 * h0, 1.f - h1 * h1);

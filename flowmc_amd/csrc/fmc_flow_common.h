@@ -46,6 +46,10 @@ struct PF32 {
   static __device__ __forceinline__ float f(T x) { return x; }
   static __device__ __forceinline__ float hsum(float x) { return x; }
   static __device__ __forceinline__ float act(float x) { return tanhf(x); }
+  static __device__ __forceinline__ void act2(float a, float b, float& ha, float& hb) {
+    ha = tanhf(a);
+    hb = tanhf(b);
+  }
   // 4 consecutive elements (rows are not 16-byte aligned: 4 scalar stores)
   static __device__ __forceinline__ void store4(T* dst, float a, float b, float c, float d) {
     dst[0] = a;
@@ -94,6 +98,19 @@ struct PBF16 {
     r = fmaf(fmaf(-den, r, 1.f), r, r);
     return fmaf(-2.f, r, 1.f);
   }
+  // two activations with one reciprocal: 1/da = r db, 1/db = r da with
+  // r = 1/(da db) (Newton-refined); arguments clamped to |x| <= 15, where
+  // tanh is 1 to FP32 precision, so da db <= (e^30 + 1)^2 stays finite
+  static __device__ __forceinline__ void act2(float a, float b, float& ha, float& hb) {
+    a = fminf(fmaxf(a, -15.f), 15.f);
+    b = fminf(fmaxf(b, -15.f), 15.f);
+    const float da = __expf(2.f * a) + 1.f, db = __expf(2.f * b) + 1.f;
+    const float p = da * db;
+    float r = __builtin_amdgcn_rcpf(p);
+    r = fmaf(fmaf(-p, r, 1.f), r, r);
+    ha = fmaf(-2.f, r * db, 1.f);
+    hb = fmaf(-2.f, r * da, 1.f);
+  }
 };
 
 __device__ __forceinline__ int out_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
@@ -111,9 +128,12 @@ __device__ __forceinline__ void hidden_epilogue_tr(typename PT::T* hb, int lh, c
     const float4 b4 = *reinterpret_cast<const float4*>(bias + f0 + 8 * g);
 #pragma unroll
     for (int r = 0; r < MB; ++r)
-      PT::store4(hb + (r * 32 + (lane & 31)) * lh + f0 + 8 * g, PT::act(acc[r][4 * g] + b4.x),
-                 PT::act(acc[r][4 * g + 1] + b4.y), PT::act(acc[r][4 * g + 2] + b4.z),
-                 PT::act(acc[r][4 * g + 3] + b4.w));
+    {
+      float h0, h1, h2, h3;
+      PT::act2(acc[r][4 * g] + b4.x, acc[r][4 * g + 1] + b4.y, h0, h1);
+      PT::act2(acc[r][4 * g + 2] + b4.z, acc[r][4 * g + 3] + b4.w, h2, h3);
+      PT::store4(hb + (r * 32 + (lane & 31)) * lh + f0 + 8 * g, h0, h1, h2, h3);
+    }
   }
 }
 
