@@ -978,7 +978,7 @@ def bench_hmc_funnels(args):
     bytes_total = sum(C_ * T * (4 * d + 8) + C_ * n_launch * (8 * d + 8) for d in dims)
     achieved = flops_total / gpu_s / 1e12
     traffic = None  # HBM bytes per 16-step launch from the counter runs of this workload
-    tf = os.path.join(ROOT, "profiles", "hmc_batched_kernel_traffic__hmc_funnels.json")
+    tf = os.path.join(ROOT, "profiles", "hmc_batched_unit_kernel_traffic__hmc_funnels.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -988,7 +988,8 @@ def bench_hmc_funnels(args):
         "ms_per_step": wall * 1e3 / T,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "hmc_batched_kernel (FP32 vector ALU)", "kernel_ms": gpu_s * 1e3 / n_launch,
+                     "kernel": ("hmc_batched_unit_kernel" if bh.uniform_kind() >= 0 else "hmc_batched_kernel")
+                     + " (FP32 vector ALU)", "kernel_ms": gpu_s * 1e3 / n_launch,
                      "flops_per_launch": flops_total / n_launch},
         "extra": {"hbm_gbs": bytes_total / gpu_s / 1e9, "hbm_frac": bytes_total / gpu_s / 1e9 / HBM_PEAK_GBS,
                   "acceptance": float(torch.cat([o[2].reshape(-1) for o in outs]).mean()),
@@ -1301,11 +1302,13 @@ def summarize_trace(path, top=25):
     print("\n".join(lines))
 
 
-def summarize_traffic(fetch_csv, write_csv, kernel, label, out_json):
+def summarize_traffic(fetch_csv, write_csv, kernel, label, out_json, n_last=0):
     """Per-launch HBM traffic of ``kernel`` from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv files), each scaled
     by what the same counter reads for the workload's 256 MiB calibration
-    copy (__amd_rocclr_copyBuffer of 268,435,456 bytes each way)."""
+    copy (__amd_rocclr_copyBuffer of 268,435,456 bytes each way). With
+    ``n_last`` > 0 only the kernel's last n_last dispatches (the timed
+    launches, after any autotuning runs) are averaged."""
     import collections
     import csv
 
@@ -1315,22 +1318,22 @@ def summarize_traffic(fetch_csv, write_csv, kernel, label, out_json):
             for r in csv.DictReader(f):
                 if r["Counter_Name"] != counter:
                     continue
-                v = vals[r["Dispatch_Id"]]
+                v = vals[int(r["Dispatch_Id"])]
                 v[0] = r["Kernel_Name"]
                 v[1] += float(r["Counter_Value"])
-        return list(vals.values())
+        return [vals[k] for k in sorted(vals)]
 
-    def kib(rows, pred):
+    def kib(rows, pred, last=0):
         xs = [v for n, v in rows if pred(n)]
         if not xs:
             raise SystemExit(f"no dispatch matches in the counter file")
-        return xs
+        return xs[-last:] if last > 0 else xs
 
     cal = 268435456
     out = {"kernel": label}
     for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
         rows = per_dispatch(path, counter)
-        k = kib(rows, lambda n: kernel in n)
+        k = kib(rows, lambda n: kernel in n, int(n_last))
         c = [v for v in kib(rows, lambda n: "copyBuffer" in n) if v * 1024 > cal / 4]
         out[f"{key}_size_kib_per_dispatch"] = sum(k) / len(k)
         out.setdefault("calibration", {"kernel": "__amd_rocclr_copyBuffer", "bytes_read": cal, "bytes_written": cal})
@@ -1339,6 +1342,7 @@ def summarize_traffic(fetch_csv, write_csv, kernel, label, out_json):
     out["bytes_read_per_launch"] = out["fetch_size_kib_per_dispatch"] * 1024 * out["calibration"]["fetch_scale"]
     out["bytes_written_per_launch"] = out["write_size_kib_per_dispatch"] * 1024 * out["calibration"]["write_scale"]
     out["bytes_per_launch"] = out["bytes_read_per_launch"] + out["bytes_written_per_launch"]
+    out["dispatches_averaged"] = len(k)
     with open(out_json, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
@@ -1346,7 +1350,7 @@ def summarize_traffic(fetch_csv, write_csv, kernel, label, out_json):
 
 def main():
     if len(sys.argv) >= 7 and sys.argv[1] == "--traffic-summary":
-        summarize_traffic(*sys.argv[2:7])
+        summarize_traffic(*sys.argv[2:8])
         return
     if len(sys.argv) >= 3 and sys.argv[1] == "--trace-summary":
         summarize_trace(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 25)

@@ -148,7 +148,7 @@ _SIGNATURES: dict[str, list] = {
                     C.POINTER(fmc_launch_cfg)],
     "fmc_hmc_run": [P, C.POINTER(fmc_target), P, P, I64, I32, I32, F32, I32, P, P, U32, U32, P, P, P, P, P, I64,
                     I64, I32, C.POINTER(fmc_launch_cfg)],
-    "fmc_hmc_run_batched": [P, P, I32, P, I32, I32, I32, I32, F32, I32, I32, I32, I32],
+    "fmc_hmc_run_batched": [P, P, I32, P, I32, I32, I32, I32, F32, I32, I32, I32, I32, I32],
     "fmc_tempered_run": [P, I32, C.POINTER(fmc_target), P, P, I64, I32, P, I32, I32, F32, I32, P, P, U32, U32, P, P,
                          P, I64, C.POINTER(fmc_launch_cfg)],
     "fmc_pt_exchange": [P, P, P, P, P, I64, I32, I32, U32, U32, P, P],

@@ -306,8 +306,10 @@ __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io,
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (UNIT_M) {
+          // sum of squares; the 1/2 is applied after the group sum (a power
+          // of two: the same bits as summing 0.5 p^2)
           const double pj = (double)f4c(p[v], e);
-          k0 += 0.5 * pj * pj;
+          k0 = fma(pj, pj, k0);
         } else {
           int j = dim_of(v, e, r, lpc);
           float mj = f4c(m[v], e);
@@ -322,6 +324,7 @@ __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io,
       gl[v] = g[v];
     }
     k0 = group_sum(k0, lpc);
+    if (UNIT_M) k0 *= 0.5;
     // row 0: drift coefficient 0, kick 1/2
 #pragma unroll
     for (int v = 0; v < NV; ++v)
@@ -347,13 +350,14 @@ __device__ __forceinline__ void hmc_group(const TargetDev& t, const ChainIO& io,
       for (int e = 0; e < 4; ++e) {
         if (UNIT_M) {
           const double pj = (double)f4c(p[v], e);
-          k1 += 0.5 * pj * pj;
+          k1 = fma(pj, pj, k1);
         } else {
           int j = dim_of(v, e, r, lpc);
           if (j < d) k1 += 0.5 * (double)f4c(m[v], e) * (double)f4c(p[v], e) * (double)f4c(p[v], e);
         }
       }
     k1 = group_sum(k1, lpc);
+    if (UNIT_M) k1 *= 0.5;
     double h0 = -(double)lp + k0;
     double h1 = -(double)lpl + k1;
     double log_acc = h0 - h1;

@@ -139,12 +139,20 @@ template <int CTRL, int BANK = 0xF>
 __device__ __forceinline__ float dpp_f(float old, float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, BANK, false));
 }
+// Every lane written, from a source lane inside its row for each control
+// used with it (quad permutations, mirrors, row broadcast), so no old value
+// is needed: the compiler folds the move into the consuming add
+// (v_add_f32_dpp), one instruction per reduction level instead of three.
+template <int CTRL>
+__device__ __forceinline__ float dpp_all(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
   const int2 w = *reinterpret_cast<const int2*>(&v);
   int2 r;
-  r.x = __builtin_amdgcn_update_dpp(w.x, w.x, CTRL, 0xF, 0xF, false);
-  r.y = __builtin_amdgcn_update_dpp(w.y, w.y, CTRL, 0xF, 0xF, false);
+  r.x = __builtin_amdgcn_mov_dpp(w.x, CTRL, 0xF, 0xF, true);
+  r.y = __builtin_amdgcn_mov_dpp(w.y, CTRL, 0xF, 0xF, true);
   return *reinterpret_cast<const double*>(&r);
 }
 constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
@@ -189,10 +197,10 @@ __device__ __forceinline__ void swap32(double v, double& lo, double& hi) {
 }
 
 __device__ __forceinline__ float group_sum(float v, int lpc) {
-  if (lpc >= 2) v += dpp_f<DPP_XOR1>(v, v);
-  if (lpc >= 4) v += dpp_f<DPP_XOR2>(v, v);
-  if (lpc >= 8) v += dpp_f<DPP_HALF_MIRROR>(v, v);
-  if (lpc >= 16) v += dpp_f<DPP_MIRROR>(v, v);
+  if (lpc >= 2) v += dpp_all<DPP_XOR1>(v);
+  if (lpc >= 4) v += dpp_all<DPP_XOR2>(v);
+  if (lpc >= 8) v += dpp_all<DPP_HALF_MIRROR>(v);
+  if (lpc >= 16) v += dpp_all<DPP_MIRROR>(v);
   float lo, hi;
   if (lpc >= 32) { swap16(v, lo, hi); v = lo + hi; }
   if (lpc >= 64) { swap32(v, lo, hi); v = lo + hi; }
@@ -209,10 +217,10 @@ __device__ __forceinline__ double group_sum(double v, int lpc) {
   return v;
 }
 __device__ __forceinline__ float group_max(float v, int lpc) {
-  if (lpc >= 2) v = fmaxf(v, dpp_f<DPP_XOR1>(v, v));
-  if (lpc >= 4) v = fmaxf(v, dpp_f<DPP_XOR2>(v, v));
-  if (lpc >= 8) v = fmaxf(v, dpp_f<DPP_HALF_MIRROR>(v, v));
-  if (lpc >= 16) v = fmaxf(v, dpp_f<DPP_MIRROR>(v, v));
+  if (lpc >= 2) v = fmaxf(v, dpp_all<DPP_XOR1>(v));
+  if (lpc >= 4) v = fmaxf(v, dpp_all<DPP_XOR2>(v));
+  if (lpc >= 8) v = fmaxf(v, dpp_all<DPP_HALF_MIRROR>(v));
+  if (lpc >= 16) v = fmaxf(v, dpp_all<DPP_MIRROR>(v));
   float lo, hi;
   if (lpc >= 32) { swap16(v, lo, hi); v = fmaxf(lo, hi); }
   if (lpc >= 64) { swap32(v, lo, hi); v = fmaxf(lo, hi); }
@@ -221,14 +229,14 @@ __device__ __forceinline__ float group_max(float v, int lpc) {
 // The value of the group's first lane, in every lane of the group.
 __device__ __forceinline__ float group_bcast0(float v, int lpc) {
   if (lpc == 1) return v;
-  if (lpc == 2) return dpp_f<DPP_PAIR_BCAST0>(v, v);
-  if (lpc == 4) return dpp_f<DPP_QUAD_BCAST0>(v, v);
+  if (lpc == 2) return dpp_all<DPP_PAIR_BCAST0>(v);
+  if (lpc == 4) return dpp_all<DPP_QUAD_BCAST0>(v);
   if (lpc == 8) {
-    const float q = dpp_f<DPP_QUAD_BCAST0>(v, v);
+    const float q = dpp_all<DPP_QUAD_BCAST0>(v);
     return dpp_f<DPP_ROW_SHR4, 0xA>(q, q);  // banks 1 and 3 take lane i-4
   }
   if (lpc == 64) return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));  // int builtin: bitcast
-  const float rb = dpp_f<DPP_ROW_BCAST0>(v, v);
+  const float rb = dpp_all<DPP_ROW_BCAST0>(v);
   if (lpc == 16) return rb;
   float lo, hi;
   swap16(rb, lo, hi);

@@ -235,10 +235,18 @@ typedef struct fmc_work_item {
 
 /* block_size: a multiple of 64 in [64, 512]; every slot's lanes_per_chain   */
 /* divides it.                                                               */
+/* uniform_kind: -1 for any mix of targets and condition matrices, or the    */
+/* FMC_TARGET_* kind that EVERY slot holds when no slot has a metric or      */
+/* momentum factor (HMC's default unit condition matrix). With it, and       */
+/* vec_per_lane 1 or 2, a kernel specialised on that kind and on each        */
+/* slot's lanes_per_chain runs the steps (same results, no run-time target   */
+/* dispatch); a slot that breaks the promise is not advanced and its logp    */
+/* state is set to NaN.                                                      */
 int fmc_hmc_run_batched(void* stream, const fmc_hmc_slot* slots, int32_t n_slots,
                         const fmc_work_item* items, int32_t n_items, int32_t max_dim,
                         int32_t n_steps, int32_t step0, float step_size, int32_t n_leapfrog,
-                        int32_t thinning, int32_t vec_per_lane, int32_t block_size);
+                        int32_t thinning, int32_t vec_per_lane, int32_t block_size,
+                        int32_t uniform_kind);
 
 /* ------------------------------------------------------------------------ */
 /* Per-step building blocks for user-supplied (non built-in) targets whose  */

@@ -544,10 +544,13 @@ def test_hmc_fused_matches_oracle_long_trajectories(gpu, name, d, L):
     assert float(acc.mean()) > 0.05
 
 
-def test_batched_hmc_matches_oracle(gpu):
+@pytest.mark.parametrize("metric", ["mixed", "unit"])
+def test_batched_hmc_matches_oracle(gpu, metric):
     # the uneven-dimension batch against the FP64 oracle, target by target:
-    # 9 funnels with d from 8 to 256, L = 10, eps = 0.05, injected noise,
-    # unit metric on some slots and a diagonal metric on others
+    # 9 funnels with d from 8 to 256, L = 10, eps = 0.05, injected noise;
+    # "mixed": unit metric on some slots and a diagonal metric on others (the
+    # run-time dispatch kernel), "unit": every slot on the unit metric (the
+    # kernel specialised on the funnel and on each slot's lanes per chain)
     from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
 
     dims = [8, 11, 17, 29, 45, 64, 100, 160, 256]
@@ -559,8 +562,11 @@ def test_batched_hmc_matches_oracle(gpu):
     log_u = [torch.log(torch.rand(S, C_, generator=g, dtype=torch.float64)) for _ in dims]
     metrics = [None if i % 2 == 0 else (0.5 + torch.rand(d, generator=g, dtype=torch.float64))
                for i, d in enumerate(dims)]
+    if metric == "unit":
+        metrics = [None] * len(dims)
     bh = BatchedHMC(targets, C_, step_size=eps, n_leapfrog=L,
                     metrics=[m.float().cuda() if m is not None else None for m in metrics])
+    assert bh.uniform_kind() == (TG.Funnel.kind if metric == "unit" else -1)
     for x, xi in zip(bh.x, x0):
         x.copy_(xi.float())
     outs = [(torch.full((C_, S + 2, d), float("nan"), device=gpu), torch.full((C_, S + 2), float("nan"), device=gpu),
@@ -587,6 +593,75 @@ def test_batched_hmc_matches_oracle(gpu):
         assert torch.isnan(outs[i][0][:, 0]).all() and torch.isnan(outs[i][0][:, 1 + S]).all()
         n_checked += int(ok.sum())
     assert n_checked >= 0.85 * C_ * len(dims)
+
+
+def _kind_targets(kind, dims, g):
+    if kind == "funnel":
+        return [TG.Funnel(d) for d in dims]
+    if kind == "gauss_diag":
+        return [TG.GaussianDiag(torch.randn(d, generator=g), 0.5 + torch.rand(d, generator=g)) for d in dims]
+    if kind == "gauss_dense":
+        out = []
+        for d in dims:
+            a = torch.randn(d, d, generator=g, dtype=torch.float64) / d ** 0.5
+            out.append(TG.GaussianDense(torch.randn(d, generator=g), a @ a.T + torch.eye(d, dtype=torch.float64)))
+        return out
+    if kind == "rosenbrock":
+        return [TG.Rosenbrock(d) for d in dims]
+    if kind == "gmm":
+        return [TG.GaussianMixtureDiag(3 * torch.randn(3, d, generator=g), 0.5 + torch.rand(3, d, generator=g))
+                for d in dims]
+    return [TG.DualMoon(d) for d in dims]
+
+
+@pytest.mark.parametrize("kind", ["funnel", "gauss_diag", "gauss_dense", "rosenbrock", "gmm", "dual_moon"])
+@pytest.mark.parametrize("nv", [1, 2])
+def test_batched_hmc_uniform_kernel_matches_dispatch_kernel(gpu, kind, nv):
+    # A batch of one target kind with the unit metric runs the kernel
+    # specialised on that kind and on each slot's lanes per chain; it must
+    # reproduce the run-time dispatch kernel on the same layout: the same
+    # accept decisions, and rows equal to FP32 rounding (the two builds may
+    # contract a target's sums into FMAs differently: the funnel's log-density
+    # differs in its last bits, an absolute 4e-6 where its terms of size ~100
+    # cancel), for Philox draws and for injected noise, with lanes per chain
+    # from 1 to 64 (spread 2 doubles them).
+    from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
+
+    g = torch.Generator().manual_seed(41)
+    dims = [2, 5, 8, 13, 31, 64, 100, 200] if nv == 2 else [2, 3, 8, 17, 40, 64, 130, 256]
+    if kind == "gauss_dense":
+        dims = dims[:6]
+    targets = _kind_targets(kind, dims, g)
+    C_, S, eps, L = 40, 6, 0.03, 4
+    x0 = [0.5 * torch.randn(C_, d, generator=g) for d in dims]
+    noise = [torch.randn(S, C_, d, generator=g).cuda() for d in dims]
+    log_u = [torch.log(torch.rand(S, C_, generator=g)).cuda() for _ in dims]
+    runs = {}
+    for spec in (True, False):
+        bh = BatchedHMC(targets, C_, step_size=eps, n_leapfrog=L)
+        bh.specialise = spec
+        assert bh.uniform_kind() == (targets[0].kind if spec else -1)
+        res = []
+        for inj in (False, True):
+            layout = bh.default_layout(nv, 256, 2 if inj else 1)
+            for x, xi in zip(bh.x, x0):
+                x.copy_(xi.cuda())
+            outs = [(torch.full((C_, S, d), float("nan"), device=gpu), torch.full((C_, S), float("nan"), device=gpu),
+                     torch.full((C_, S), float("nan"), device=gpu), 0) for d in dims]
+            kw = dict(noise=noise, log_u=log_u) if inj else {}
+            bh.run(fr.PRNGKey(5), S, outputs=outs, fused_steps=4, layout=layout, **kw)
+            torch.cuda.synchronize()
+            res.append(([t.clone() for o in outs for t in o[:3]], [x.clone() for x in bh.x],
+                        [lp.clone() for lp in bh.logp]))
+        runs[spec] = res
+    for (rows_a, x_a, lp_a), (rows_b, x_b, lp_b) in zip(runs[True], runs[False]):
+        for i, d in enumerate(dims):
+            pa, la, aa = rows_a[3 * i:3 * i + 3]
+            pb, lb, ab = rows_b[3 * i:3 * i + 3]
+            assert torch.equal(aa, ab), f"d={d}: accept decisions differ"
+            for a, b, tol in ((pa, pb, 1e-5), (x_a[i], x_b[i], 1e-5), (la, lb, 1e-4), (lp_a[i], lp_b[i], 1e-4)):
+                assert torch.isfinite(a).all()
+                assert torch.allclose(a, b, rtol=1e-5, atol=tol), (d, float((a - b).abs().max()))
 
 
 def test_batched_hmc_keeps_the_funnel_law_at_full_size(gpu):
