@@ -125,14 +125,39 @@ __device__ __forceinline__ void hidden_epilogue_tr(typename PT::T* hb, int lh, c
   const int f0 = fb * 32 + 4 * (lane >> 5);
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
-    const float4 b4 = *reinterpret_cast<const float4*>(bias + f0 + 8 * g);
+    // bias == nullptr: the accumulators started from the bias (bias_tr)
+    const float4 b4 = bias ? *reinterpret_cast<const float4*>(bias + f0 + 8 * g) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int r = 0; r < MB; ++r)
     {
       float h0, h1, h2, h3;
-      PT::act2(acc[r][4 * g] + b4.x, acc[r][4 * g + 1] + b4.y, h0, h1);
-      PT::act2(acc[r][4 * g + 2] + b4.z, acc[r][4 * g + 3] + b4.w, h2, h3);
+      if (bias) {
+        PT::act2(acc[r][4 * g] + b4.x, acc[r][4 * g + 1] + b4.y, h0, h1);
+        PT::act2(acc[r][4 * g + 2] + b4.z, acc[r][4 * g + 3] + b4.w, h2, h3);
+      } else {
+        PT::act2(acc[r][4 * g], acc[r][4 * g + 1], h0, h1);
+        PT::act2(acc[r][4 * g + 2], acc[r][4 * g + 3], h2, h3);
+      }
       PT::store4(hb + (r * 32 + (lane & 31)) * lh + f0 + 8 * g, h0, h1, h2, h3);
+    }
+  }
+}
+
+// The bias of the transposed hidden tile (features fb*32.., the lane's rows
+// as hidden_epilogue_tr reads them) as the accumulators' starting value: the
+// MFMAs then add the products to it, which saves the epilogue's additions.
+template <int MB>
+__device__ __forceinline__ void bias_tr(f32x16_t (&acc)[MB], const float* bias, int fb, int lane) {
+  const int f0 = fb * 32 + 4 * (lane >> 5);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 b4 = *reinterpret_cast<const float4*>(bias + f0 + 8 * g);
+#pragma unroll
+    for (int r = 0; r < MB; ++r) {
+      acc[r][4 * g] = b4.x;
+      acc[r][4 * g + 1] = b4.y;
+      acc[r][4 * g + 2] = b4.z;
+      acc[r][4 * g + 3] = b4.w;
     }
   }
 }
@@ -567,7 +592,9 @@ __device__ __forceinline__ float rs_softplus(float v) { return fmaxf(v, 0.f) + f
 // knots of the bin holding t (t along x when INV is false, along y otherwise)
 // KT > 0: the bin count is the compile-time constant KT (loops unrolled to
 // exactly KT, no predication); KT == 0: runtime K <= 10, loops predicated.
-template <bool INV, int KT>
+// FL (the BF16 conditioner): the softmax exponentials as exp2 of one fma,
+// p log2(e) - max log2(e), instead of a subtraction, a multiply and exp2.
+template <bool INV, int KT, bool FL = false>
 __device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int Kr, const FlowDev& f, float t) {
   constexpr int N = KT > 0 ? KT : 10;
   const int K = KT > 0 ? KT : Kr;
@@ -578,36 +605,48 @@ __device__ __forceinline__ RsBin rs_bin(const float (&pp)[32], int Kr, const Flo
       mw = fmaxf(mw, pp[i]);
       mh = fmaxf(mh, pp[10 + i]);
     }
+  constexpr float L2E = 1.4426950408889634f;
+  const float mwl = mw * L2E, mhl = mh * L2E;
   float ew[10], eh[10], sw = 0.f, sh = 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    ew[i] = (KT > 0 || i < K) ? __expf(pp[i] - mw) : 0.f;
-    eh[i] = (KT > 0 || i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+    if (FL) {
+      ew[i] = (KT > 0 || i < K) ? __builtin_amdgcn_exp2f(fmaf(pp[i], L2E, -mwl)) : 0.f;
+      eh[i] = (KT > 0 || i < K) ? __builtin_amdgcn_exp2f(fmaf(pp[10 + i], L2E, -mhl)) : 0.f;
+    } else {
+      ew[i] = (KT > 0 || i < K) ? __expf(pp[i] - mw) : 0.f;
+      eh[i] = (KT > 0 || i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+    }
     sw += ew[i];
     sh += eh[i];
   }
   const float span = (f.hi - f.lo) - K * f.mb;
   const float cw = ndiv(span, sw), ch = ndiv(span, sh);
-  RsBin b{f.lo, f.hi, f.lo, f.hi, pp[20], pp[21]};
+  // right knots of bins 0..K-1 (the last one is hi)
+  float xn[10], yn[10];
   float xc = f.lo, yc = f.lo;
-  bool found = false;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    if (KT > 0 || i < K) {
-      const float xn = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
-      const float yn = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
-      const bool hit = !found && (INV ? (t < yn) : (t < xn));
-      if (hit) {
-        b.xk = xc;
-        b.yk = yc;
-        b.xk1 = xn;
-        b.yk1 = yn;
-        b.uk = pp[20 + i];
-        b.uk1 = (i == K - 1) ? pp[30] : pp[21 + i];
-      }
-      found = found || hit;
-      xc = xn;
-      yc = yn;
+    xn[i] = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
+    yn[i] = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
+    xc = xn[i];
+    yc = yn[i];
+  }
+  // The knots increase (every bin is at least mb wide), so the bins whose
+  // right knot t has reached form a prefix: start in bin 0 and move one bin
+  // right, by selects, for every interior knot with t >= knot. This is the
+  // first bin whose right knot exceeds t (t < hi on entry).
+  RsBin b{f.lo, xn[0], f.lo, yn[0], pp[20], (K == 1) ? pp[30] : pp[21]};
+#pragma unroll
+  for (int i = 0; i + 1 < N; ++i) {
+    if (KT > 0 || i + 1 < K) {
+      const bool c = INV ? (t >= yn[i]) : (t >= xn[i]);
+      b.xk = c ? xn[i] : b.xk;
+      b.yk = c ? yn[i] : b.yk;
+      b.uk = c ? pp[21 + i] : b.uk;
+      b.xk1 = c ? xn[i + 1] : b.xk1;
+      b.yk1 = c ? yn[i + 1] : b.yk1;
+      b.uk1 = c ? ((i + 2 == K) ? pp[30] : pp[22 + i]) : b.uk1;
     }
   }
   return b;
@@ -623,7 +662,7 @@ __device__ __forceinline__ void rs_fwd(const float (&pp)[32], int K, const FlowD
     ld = fln<FL>(dd);
     return;
   }
-  const RsBin b = rs_bin<false, KT>(pp, K, f, x);
+  const RsBin b = rs_bin<false, KT, FL>(pp, K, f, x);
   const float dk = rs_softplus<FL>(b.uk + f.off) + f.ms, dk1 = rs_softplus<FL>(b.uk1 + f.off) + f.ms;
   const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = ndiv(hh, w);
   const float z = fminf(fmaxf(ndiv(x - b.xk, w), 0.f), 1.f);
@@ -645,7 +684,7 @@ __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowD
     ld = -fln<FL>(dd);
     return;
   }
-  const RsBin b = rs_bin<true, KT>(pp, K, f, yv);
+  const RsBin b = rs_bin<true, KT, FL>(pp, K, f, yv);
   const float dk = rs_softplus<FL>(b.uk + f.off) + f.ms, dk1 = rs_softplus<FL>(b.uk1 + f.off) + f.ms;
   const float w = b.xk1 - b.xk, hh = b.yk1 - b.yk, s = ndiv(hh, w);
   const float wv = fminf(fmaxf(ndiv(yv - b.yk, hh), 0.f), 1.f);

@@ -168,11 +168,14 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
         const int NBt = dm.hp(i) / 32;
         for (int c0 = wave; c0 < NBt; c0 += 8) {
           const int ncb = (c0 + 4 < NBt) ? 2 : 1;
-          f32x16_t acc[2][MB] = {};
+          f32x16_t acc[2][MB];
+          // the accumulators start from the bias (an unused second block from the first's)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) bias_tr<MB>(acc[j], bl + dm.pb_b(i), j < ncb ? c0 + 4 * j : c0, lane);
           pass_panel<EX, PT, MB, true>(in, ldin, W + dm.pk_wh(l, i), KB, c0, 4, ncb, 0, KB, acc, lane);
 #pragma unroll
           for (int j = 0; j < 2; ++j)
-            if (j < ncb) hidden_epilogue_tr<PT, MB>(out, sm.lh, bl + dm.pb_b(i), c0 + 4 * j, acc[j], lane);
+            if (j < ncb) hidden_epilogue_tr<PT, MB>(out, sm.lh, nullptr, c0 + 4 * j, acc[j], lane);
         }
         __syncthreads();
         in = out;
@@ -191,28 +194,34 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       for (int r = 0; r < MB; ++r) ldacc[r] = 0.f;
       const int hf = lane >> 5;
       for (int q = wave; q < npair; q += 4) {
-        f32x16_t acc[2][MB] = {};
-        pass_panel<EX, PT, MB, true>(hlast, sm.lh, W3r, KB, 2 * q, 1, 2, 0, KB, acc, lane);
         const int k = 2 * q + hf;
-        if (k < dtl) {
-          const int j = dm.trans(l, k);
-          const float4* bp = reinterpret_cast<const float4*>(b3r + q * 64 + hf * 32);
-          float bias[32];
+        // the accumulators start from the 32 spline biases of the lane's
+        // dim (a lane past the last dim reads the pair's first dim's)
+        f32x16_t acc[2][MB];
+        {
+          const float4* bp = reinterpret_cast<const float4*>(b3r + q * 64 + (k < dtl ? hf * 32 : 0));
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const float4 b4 = bp[u];
-            bias[4 * u] = b4.x;
-            bias[4 * u + 1] = b4.y;
-            bias[4 * u + 2] = b4.z;
-            bias[4 * u + 3] = b4.w;
+#pragma unroll
+            for (int r = 0; r < MB; ++r) {
+              acc[u >> 2][r][4 * (u & 3)] = b4.x;
+              acc[u >> 2][r][4 * (u & 3) + 1] = b4.y;
+              acc[u >> 2][r][4 * (u & 3) + 2] = b4.z;
+              acc[u >> 2][r][4 * (u & 3) + 3] = b4.w;
+            }
           }
+        }
+        pass_panel<EX, PT, MB, true>(hlast, sm.lh, W3r, KB, 2 * q, 1, 2, 0, KB, acc, lane);
+        if (k < dtl) {
+          const int j = dm.trans(l, k);
 #pragma unroll
           for (int r = 0; r < MB; ++r) {
             float pp[32];
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-              pp[v] = acc[0][r][v] + bias[v];
-              pp[16 + v] = acc[1][r][v] + bias[16 + v];
+              pp[v] = acc[0][r][v];
+              pp[16 + v] = acc[1][r][v];
             }
             const int m = r * 32 + (lane & 31);
             const float xv = xs[m * ldx + j];
