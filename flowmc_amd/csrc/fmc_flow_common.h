@@ -729,46 +729,56 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
       mw = fmaxf(mw, pp[i]);
       mh = fmaxf(mh, pp[10 + i]);
     }
+  constexpr float L2E = 1.4426950408889634f;
+  const float mwl = mw * L2E, mhl = mh * L2E;
   float ew[10], eh[10], sw = 0.f, sh = 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    ew[i] = (KT > 0 || i < K) ? __expf(pp[i] - mw) : 0.f;
-    eh[i] = (KT > 0 || i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+    if (FL) {  // exp2 of one fma, as rs_bin
+      ew[i] = (KT > 0 || i < K) ? __builtin_amdgcn_exp2f(fmaf(pp[i], L2E, -mwl)) : 0.f;
+      eh[i] = (KT > 0 || i < K) ? __builtin_amdgcn_exp2f(fmaf(pp[10 + i], L2E, -mhl)) : 0.f;
+    } else {
+      ew[i] = (KT > 0 || i < K) ? __expf(pp[i] - mw) : 0.f;
+      eh[i] = (KT > 0 || i < K) ? __expf(pp[10 + i] - mh) : 0.f;
+    }
     sw += ew[i];
     sh += eh[i];
   }
   const float span = (f.hi - f.lo) - K * f.mb;
   const float isw = ndiv(1.f, sw), ish = ndiv(1.f, sh);
   const float cw = span * isw, ch = span * ish;
-  float xk = f.lo, yk = f.lo, xk1 = f.hi, yk1 = f.hi, uk = pp[20], uk1 = pp[21];
-  float pw_lt = 0.f, ph_lt = 0.f, pwk = 0.f, phk = 0.f;  // softmax mass of bins < k, and of bin k
+  // softmax masses and right knots of bins 0..K-1
+  float pw[10], ph[10], xn[10], yn[10];
   float xc = f.lo, yc = f.lo;
-  int k = 0;
-  bool found = false;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    if (KT > 0 || i < K) {
-      const float xn = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
-      const float yn = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
-      const bool hit = !found && (x < xn);
-      if (hit) {
-        k = i;
-        xk = xc;
-        yk = yc;
-        xk1 = xn;
-        yk1 = yn;
-        uk = pp[20 + i];
-        uk1 = (i == K - 1) ? pp[30] : pp[21 + i];
-        pwk = ew[i] * isw;
-        phk = eh[i] * ish;
-      }
-      if (!found && !hit) {
-        pw_lt += ew[i] * isw;
-        ph_lt += eh[i] * ish;
-      }
-      found = found || hit;
-      xc = xn;
-      yc = yn;
+    pw[i] = ew[i] * isw;
+    ph[i] = eh[i] * ish;
+    xn[i] = (i == K - 1) ? f.hi : xc + (ew[i] * cw + f.mb);
+    yn[i] = (i == K - 1) ? f.hi : yc + (eh[i] * ch + f.mb);
+    xc = xn[i];
+    yc = yn[i];
+  }
+  // bin walk by selects over the increasing interior knots, as rs_bin; the
+  // masses of the bins passed (< k) are summed in bin order
+  float xk = f.lo, yk = f.lo, xk1 = xn[0], yk1 = yn[0], uk = pp[20], uk1 = (K == 1) ? pp[30] : pp[21];
+  float pw_lt = 0.f, ph_lt = 0.f, pwk = pw[0], phk = ph[0];  // softmax mass of bins < k, and of bin k
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i + 1 < N; ++i) {
+    if (KT > 0 || i + 1 < K) {
+      const bool c = x >= xn[i];
+      k = c ? i + 1 : k;
+      xk = c ? xn[i] : xk;
+      yk = c ? yn[i] : yk;
+      uk = c ? pp[21 + i] : uk;
+      xk1 = c ? xn[i + 1] : xk1;
+      yk1 = c ? yn[i + 1] : yk1;
+      uk1 = c ? ((i + 2 == K) ? pp[30] : pp[22 + i]) : uk1;
+      pw_lt = c ? pw_lt + pw[i] : pw_lt;
+      ph_lt = c ? ph_lt + ph[i] : ph_lt;
+      pwk = c ? pw[i + 1] : pwk;
+      phk = c ? ph[i + 1] : phk;
     }
   }
   uk += f.off;
@@ -824,8 +834,8 @@ __device__ __forceinline__ float rs_vjp(const float (&pp)[32], int Kr, const Flo
     if (KT > 0 || m < K) {
       const float gwm = span * ((m < k ? G1x : 0.f) + (m <= k ? G2x : 0.f));
       const float ghm = span * ((m < k ? G1y : 0.f) + (m <= k ? G2y : 0.f));
-      go[m] = ew[m] * isw * (gwm - dotw);
-      go[10 + m] = eh[m] * ish * (ghm - doth);
+      go[m] = pw[m] * (gwm - dotw);
+      go[10 + m] = ph[m] * (ghm - doth);
     }
   }
   const float gk = ndiv(g_dk, 1.f + __expf(-uk)), gk1 = ndiv(g_dk1, 1.f + __expf(-uk1));
