@@ -694,7 +694,10 @@ __device__ __forceinline__ void rs_inv(const float (&pp)[32], int K, const FlowD
   const float a = s - bq;
   const float disc = bq * bq - 4.f * a * c;
   const float sq = disc > 0.f ? fsqrt<FL>(fmaxf(disc, 1.17549435e-38f)) : 0.f;
-  float z = (bq >= 0.f) ? ndiv(2.f * c, -bq - sq) : ndiv(-bq + sq, 2.f * a);
+  // the cancellation-free root: 2c / (-b - sqrt) for b >= 0, else
+  // (-b + sqrt) / 2a; operands selected first, one division for both
+  const bool bpos = bq >= 0.f;
+  float z = ndiv(bpos ? 2.f * c : -bq + sq, bpos ? -bq - sq : 2.f * a);
   z = fminf(fmaxf(z, 0.f), 1.f);
   x = w * z + b.xk;
   const float z2 = z * z, zz = z - z2, q = (1.f - z) * (1.f - z);
