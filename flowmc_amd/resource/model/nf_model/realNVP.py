@@ -281,6 +281,10 @@ class RealNVP(_RnvpBase):
         return "RealNVP with n_features=" + str(self._n_features) + ", n_layers=" + str(self.n_layers)
 
     def __init__(self, n_features: int, n_layers: int, n_hidden: int, key, device=None, **kwargs):
+        if getattr(kwargs.get("base_dist"), "learnable", False):
+            # a trained covariance leaves the diagonal, which these kernels' base needs
+            raise NotImplementedError("learnable base distributions are not supported by the RealNVP kernels "
+                                      "(MaskedCouplingRQSpline trains them)")
         _native.require_gpu()
         self.device = torch.device("cuda") if device is None else torch.device(device)
         self._n_features = int(n_features)

@@ -102,8 +102,9 @@ class MaskedCouplingRQSpline(NFModel):
         self.precision = "bfloat16" if _PREC[precision] == 1 else "float32"
         self.exact_inverse = bool(kwargs.get("exact_inverse", False))
         bd = kwargs.get("base_dist")
-        if bd is not None and getattr(bd, "learnable", False):
-            raise NotImplementedError("learnable base distributions are not supported by the flow kernels")
+        # Gaussian(learnable=True) (common.py:243-273): the base mean and
+        # covariance are trained with the flow
+        self._base_learnable = bool(kwargs.get("_base_learnable", getattr(bd, "learnable", False)))
         theta = kwargs.get("_theta")
         if theta is None:
             theta = self._init_theta(key, kwargs)
@@ -112,6 +113,7 @@ class MaskedCouplingRQSpline(NFModel):
         # base Gaussian (its Cholesky factor is packed with the biases)
         bc = self.theta[-self._n_features ** 2:].view(self._n_features, self._n_features)
         self._base_dense = bool(torch.count_nonzero(bc - torch.diag(torch.diagonal(bc))) > 0)
+        self._base_dense = self._base_dense or self._base_learnable  # its covariance leaves the diagonal
         self._configure()
         self._dirty = True
         self._ws: dict[int, _Workspace] = {}
@@ -187,7 +189,7 @@ class MaskedCouplingRQSpline(NFModel):
 
     @property
     def base_dist(self):
-        return Gaussian(self._stat("base_mean"), self._stat("base_cov"), learnable=False)
+        return Gaussian(self._stat("base_mean"), self._stat("base_cov"), learnable=self._base_learnable)
 
     @property
     def layers(self):
@@ -365,12 +367,42 @@ class MaskedCouplingRQSpline(NFModel):
         self._ensure_packed()
         self._pass(FORWARD, STANDARDIZE | BASE_LOGP, xb, B, saved=ws.saved, g_logp=ws.g_logp, gy_out=ws.gy,
                    want_x=False)
+        base_g = self._base_grad(ws) if self._base_learnable else None
         self._backward(ws, B)
+        self._add_base_grad(ws, base_g)
         gx = ws.gy if self.n_layers % 2 == 0 else ws.gx
         gx = gx / torch.sqrt(torch.diagonal(self.data_cov))
         out = ws.grad.clone(), gx.clone()
         ws.g_logp.fill_(-1.0 / B)
         return out
+
+    def _base_grad(self, ws: _Workspace):
+        """Gradient of sum_n g_n log N(y_n; mean, cov) with respect to a
+        learnable base mean and covariance (Gaussian.log_prob, common.py:285-286:
+        jax.scipy.stats.multivariate_normal.logpdf, whose Cholesky factorisation
+        symmetrises its input). The forward pass leaves gy_n = -g_n w_n with
+        w_n = cov^-1 (y_n - mean) in ws.gy, so
+            d/dmean = sum_n g_n w_n = -sum_n gy_n,
+            d/dcov  = 1/2 sum_n g_n w_n w_n^T - 1/2 (sum_n g_n) cov^-1,
+        with g_n w_n w_n^T = gy_n gy_n^T / g_n (0 where g_n = 0). Read before
+        the adjoint reuses ws.gy; small device GEMMs."""
+        g = ws.g_logp
+        gy = ws.gy
+        gm = -gy.sum(0)
+        ig = torch.where(g != 0, 1.0 / torch.where(g != 0, g, torch.ones_like(g)), torch.zeros_like(g))
+        S = gy.t().double() @ (gy.double() * ig.double()[:, None])
+        cov = self._stat("base_cov").double()
+        cinv = torch.cholesky_inverse(torch.linalg.cholesky(0.5 * (cov + cov.t())))
+        gc = 0.5 * S - 0.5 * g.double().sum() * cinv
+        return gm, gc.float()
+
+    def _add_base_grad(self, ws: _Workspace, base_g):
+        if base_g is None:
+            return
+        d = self._n_features
+        om, oc = self._off["base_mean"], self._off["base_cov"]
+        ws.grad[om:om + d] += base_g[0]
+        ws.grad[oc:oc + d * d] += base_g[1].reshape(-1)
 
     def _backward(self, ws: _Workspace, B: int):
         """Per layer, last to first: the adjoint, then the weight gradient of
@@ -424,7 +456,9 @@ class MaskedCouplingRQSpline(NFModel):
                      xb.data_ptr(), B, 0, 0, 0, None, ws.logp.data_ptr(), ws.saved.data_ptr(), ws.g_logp.data_ptr(),
                      ws.gy.data_ptr())
         loss = -ws.logp.mean()
+        base_g = self._base_grad(ws) if self._base_learnable else None
         self._backward(ws, B)
+        self._add_base_grad(ws, base_g)
         state.count += 1
         mx = optim.max_norm if optim.max_norm is not None else 3.0e38
         _native.call("fmc_adamw_clip", _native.stream_ptr(), self.theta.data_ptr(), ws.grad.data_ptr(),
@@ -437,7 +471,8 @@ class MaskedCouplingRQSpline(NFModel):
     # --------------------------------------------------------------- I/O --
     def _config(self):
         return {"n_features": self._n_features, "n_layers": self.n_layers, "hidden_size": self.hidden_size,
-                "num_bins": self.num_bins, "spline_range": list(self.spline_range), "precision": self.precision}
+                "num_bins": self.num_bins, "spline_range": list(self.spline_range), "precision": self.precision,
+                "learnable_base": self._base_learnable}
 
     def save_model(self, path: str):
         np.savez(path + ".npz", theta=self.theta.detach().cpu().numpy(), config=json.dumps(self._config()))
@@ -448,7 +483,12 @@ class MaskedCouplingRQSpline(NFModel):
         cfg = json.loads(str(f["config"]))
         if cfg["n_features"] != self._n_features or cfg["n_layers"] != self.n_layers:
             raise ValueError("saved flow has a different shape")
-        return self.clone(torch.as_tensor(f["theta"], device=self.device))
+        m = self.clone(torch.as_tensor(f["theta"], device=self.device))
+        # the saved base decides the base path (a full covariance, or a learnable base)
+        m._base_learnable = bool(cfg.get("learnable_base", False))
+        bc = m._stat("base_cov")
+        m._base_dense = bool(torch.count_nonzero(bc - torch.diag(torch.diagonal(bc))) > 0) or m._base_learnable
+        return m
 
     def print_parameters(self):
         print("RQSpline parameters:")

@@ -242,14 +242,21 @@ def base_log_prob(params, y):
     return mvn.log_prob(y)
 
 
-def log_prob(params, x, K):
+def log_prob(params, x, K, learnable_base=False):
     """MaskedCouplingRQSpline.log_prob (rqSpline.py:498-504): no -sum log sigma term.
-    data_mean / data_cov / base mean / cov are stop_gradient leaves
-    (nf_model/base.py:25-31, common.py:261-273)."""
+    data_mean / data_cov are stop_gradient leaves (nf_model/base.py:25-31); so
+    are the base mean / cov unless the base is learnable (common.py:261-273).
+    A learnable covariance enters through jax.scipy.stats.multivariate_normal,
+    whose Cholesky factorisation (lax.linalg.cholesky) symmetrises its input:
+    the density sees (cov + cov^T) / 2."""
     std = torch.sqrt(torch.diagonal(params["data_cov"].detach()))
     z = (x - params["data_mean"].detach()) / std
     y, ld = forward(params, z, K)
-    base = {"base_mean": params["base_mean"].detach(), "base_cov": params["base_cov"].detach()}
+    if learnable_base:
+        c = params["base_cov"]
+        base = {"base_mean": params["base_mean"], "base_cov": 0.5 * (c + c.transpose(-1, -2))}
+    else:
+        base = {"base_mean": params["base_mean"].detach(), "base_cov": params["base_cov"].detach()}
     return ld + base_log_prob(base, y)
 
 

@@ -114,3 +114,14 @@ def test_gaussian_view_log_prob_uses_the_full_covariance():
     want = torch.distributions.MultivariateNormal(mean, cov).log_prob(x)
     got = Gaussian(mean, cov).log_prob(x)
     assert torch.allclose(got.double(), want, atol=1e-4, rtol=1e-5)
+
+
+def test_realnvp_refuses_a_learnable_base():
+    # its kernels need a diagonal base, which training would leave; the check
+    # comes before any device work
+    from flowmc_amd import random as fr
+    from flowmc_amd.resource.model.common import Gaussian
+    from flowmc_amd.resource.model.nf_model.realNVP import RealNVP
+
+    with pytest.raises(NotImplementedError, match="learnable"):
+        RealNVP(4, 2, 16, fr.PRNGKey(0), base_dist=Gaussian(torch.zeros(4), torch.eye(4), learnable=True))

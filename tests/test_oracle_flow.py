@@ -159,6 +159,40 @@ def test_identity_flow_with_dense_base_is_the_base_gaussian():
     assert torch.allclose(torch.cov(z.T), p["base_cov"], atol=0.03)
 
 
+def test_learnable_base_gradient_is_the_gaussian_closed_form():
+    """Identity flow (zero output layers and affines), learnable base
+    (common.py:243-273): the autograd gradient of sum_n w_n log_prob(x_n)
+    with respect to the base mean is sum_n w_n S^-1 r_n and with respect to
+    the covariance entries 1/2 sum_n w_n (S^-1 r_n r_n^T S^-1 - S^-1), the
+    symmetric matrix that multivariate_normal.logpdf's symmetrised Cholesky
+    input gives (r_n = x_n - mean). A dropped 1/2, a dropped -S^-1 term or a
+    sign error fails it; a frozen base gets no gradient."""
+    d, L, K = 4, 2, 5
+    p = _dense_base(F.init_params(d, L, K, [6, 6], seed=21), d)
+    for lp in p["layers"]:
+        n = F.n_linear(lp)
+        lp[f"W{n}"].zero_()
+        lp[f"b{n}"].zero_()
+    g = torch.Generator().manual_seed(22)
+    x = torch.randn(30, d, generator=g) * 1.3
+    w = torch.randn(30, generator=g)
+    mu = p["base_mean"].clone().requires_grad_(True)
+    cov = p["base_cov"].clone().requires_grad_(True)
+    q = dict(p, base_mean=mu, base_cov=cov)
+    gm, gc = torch.autograd.grad((w * F.log_prob(q, x, K, learnable_base=True)).sum(), [mu, cov])
+    S = p["base_cov"]
+    Si = torch.linalg.inv(S)
+    r = (x - p["base_mean"]) @ Si  # rows S^-1 r_n (S symmetric)
+    want_m = (w[:, None] * r).sum(0)
+    want_c = 0.5 * (r.T @ (w[:, None] * r)) - 0.5 * w.sum() * Si
+    assert torch.allclose(gm, want_m, atol=1e-10)
+    assert torch.allclose(gc, want_c, atol=1e-10)
+    assert torch.allclose(gc, gc.T, atol=1e-12)
+    frozen = dict(p, base_mean=p["base_mean"].clone().requires_grad_(True),
+                  base_cov=p["base_cov"].clone().requires_grad_(True))
+    assert not F.log_prob(frozen, x, K).requires_grad  # the only leaves needing a gradient are detached
+
+
 def test_reference_inverse_order_quirk():
     d, L, K = 4, 2, 4
     p = F.init_params(d, L, K, 8, 8, seed=5, scale=0.5)
