@@ -5,12 +5,15 @@ Workloads (``--workload``):
                 Sampler) on the 64-D 8-mode Gaussian mixture: 4096 chains,
                 MALA step 0.05, RQ-spline flow 10 layers / 8 bins / [256, 256]
                 with BF16 conditioner GEMMs, ``--steps`` training rounds then
-                ``--steps`` production rounds of 100 local + 100 global steps,
-                5 epochs per round, batch 10000. value = min bulk-ESS over dims
-                of the production samples / end-to-end wall time (training
-                included); valid when max R-hat < 1.01. The same workload
-                run by the plain FP32 per-step GPU path (bench_plain.py) is
-                reported beside it.
+                ``--steps`` production rounds of 100 local + 100 global steps;
+                training per round: ``--epochs`` (3) epochs, batch 10000,
+                lr 2e-4, 100k examples from the last 200 rows of every chain;
+                chains start from one exact mixture draw each (a second run
+                from x0 ~ N(0, I) is reported in ``extra.cold_start``).
+                value = min bulk-ESS over dims of the production samples /
+                end-to-end wall time (training included); valid when max
+                R-hat < 1.01. The same workload run by the plain FP32
+                per-step GPU path (bench_plain.py) is reported beside it.
   mala_rosenbrock_128d  fused MALA on the 128-D Rosenbrock target, 16384
                 chains, step 0.01, 64 steps per launch; one step = one MALA
                 step of all chains (chain-steps/s).
@@ -24,6 +27,15 @@ Workloads (``--workload``):
                 step (chain-steps/s).
   flow_pass     the exact-inverse flow pass over the 409,600 proposals of one
                 global step of nfmcmc_gmm64 (samples/s; profiling target).
+  flow_adjoint  one coupling layer's adjoint at the training shape of
+                nfmcmc_gmm64 (d=64, [256, 256], batch 10k; samples/s).
+  mala_gauss5d  fused MALA on the 5-D correlated Gaussian, 1000 chains, step
+                0.1, 1000 steps from x0 ~ N(0, I), then one flow training
+                round (4 layers, 8 bins, [32, 32], batch 1000, 5 epochs).
+  realnvp_train_d32  RealNVP training (d=32, 8 layers, hidden 256, batch 10k)
+                and a 409,600-sample draw.
+  pt_mala_gmm64  parallel tempering (5 temperatures) of fused MALA on the
+                64-D mixture, 4096 chains (tempered chain-steps/s).
 ``--gpus N`` with N > 1 exits: the build runs on a single GPU.
 """
 
