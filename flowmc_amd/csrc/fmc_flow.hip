@@ -218,13 +218,14 @@ __global__ void __launch_bounds__(64 * WG_SPLIT) flow_wgrad_kernel(FlowDev f, in
 // axis, 10^4 samples) is cut into S ranges and every (block, range) pair is
 // one workgroup of 8 waves. A block is WS_N G row blocks x up to WS_K A row
 // blocks (128 x 256 elements of gW); per batch slice the block's 12 store
-// fragments are staged in LDS once (double-buffered, WS_KS slices a stage)
+// fragments are staged in LDS once (double-buffered, WS_KS slices a stage:
+// 48 KB a workgroup, which fits beside an adjoint workgroup's 68 KB on a CU)
 // and wave w multiplies G block w/2 with A blocks 4(w%2) .. +3, so a 1 KB
 // fragment read from L2 feeds 2.7 MFMAs instead of 0.5. The FP32 partial
 // blocks go to a workspace and flow_wgrad_reduce_kernel sums the S partials
 // of each element in a fixed order (deterministic, like the one-workgroup-
 // per-tile kernel above, whose sums it matches to rounding).
-constexpr int WS_N = 4, WS_K = 8, WS_KS = 4, WS_FR = WS_N + WS_K, WS_WAVES = 8;
+constexpr int WS_N = 4, WS_K = 8, WS_KS = 2, WS_FR = WS_N + WS_K, WS_WAVES = 8;
 constexpr int WS_PART = WS_N * 32 * WS_K * 32 + WS_N * 32;  // floats of one partial (block + bias rows)
 
 struct WsTable {
@@ -789,8 +790,11 @@ extern "C" int fmc_flow_weight_grad(void* stream, const fmc_flow* fl, int32_t la
   return check_launch("flow weight grad");
 }
 
-// workgroups of the split weight gradient: one per CU (96 KB of LDS each)
-static int wsplit_max_wg() { return device_cus(); }
+// workgroups of the split weight gradient: one per two CUs. It runs beside
+// the next layer's adjoint (313 workgroups at batch 10k, one or two per CU);
+// a grid of one per CU delayed the adjoint (d=32, [128, 128]: 0.645 vs 0.527
+// ms per training step; d=64, [256, 256]: 1.010 vs 1.022 ms)
+static int wsplit_max_wg() { return device_cus() / 2; }
 
 extern "C" int fmc_flow_weight_grad_ws_size(const fmc_flow* fl, int64_t store_ld, int64_t* bytes) {
   FlowDev fd;

@@ -42,15 +42,15 @@ class _Workspace:
         self.g_logp = torch.full((B,), -1.0 / B, dtype=torch.float32, device=dev)
         self.gy = torch.empty((B, d), dtype=torch.float32, device=dev)
         self.gx = torch.empty((B, d), dtype=torch.float32, device=dev)
-        # With hidden widths <= 128 the weight gradient of layer l runs on a
-        # side stream beside the adjoint of layer l-1 (measured at batch 10k:
-        # d=32, [128,128]: 0.58 vs 0.67 ms per step). With [256,256] the two
-        # kernels cannot share the CUs the adjoint needs (its 240 VGPRs and
-        # 68 KB of LDS per workgroup) and the overlap is slower (1.61 vs 1.45
-        # ms), so the launches stay on one stream. Overlap needs two stores and
-        # affine-partial buffers: the weight gradient reads one while the next
-        # adjoint writes the other.
-        self.overlap = max(model.hidden_size) <= 128
+        # The weight gradient of layer l runs on a side stream beside the
+        # adjoint of layer l-1: the split product's 48 KB workgroups fit on a
+        # CU beside an adjoint workgroup (68 KB of LDS at d=64, [256, 256]).
+        # Measured per training step at batch 10k: d=64, 10 layers, [256, 256]
+        # 1.14 ms on one stream -> 1.02 ms overlapped; d=32, 8 layers,
+        # [128, 128] 0.554 ms (per-tile kernel overlapped) -> 0.527 ms. Overlap
+        # needs two stores and affine-partial buffers: the weight gradient
+        # reads one while the next adjoint writes the other.
+        self.overlap = True
         n_buf = 2 if self.overlap else 1
         self.store = [torch.empty((model._store_rows, self.ld), dtype=tdt, device=dev) for _ in range(n_buf)]
         self.n_part = self.ld // model._tile_bwd
@@ -70,13 +70,11 @@ class _Workspace:
 
 class MaskedCouplingRQSpline(NFModel):
     # weight gradients by the split-K product over the batch (one workgroup
-    # of 96 KB of LDS per block and batch range, fmc_flow_weight_grad_split)
+    # of 48 KB of LDS per block and batch range, fmc_flow_weight_grad_split)
     # or by one workgroup per 32 x 32 tile (fmc_flow_weight_grad); equal to
-    # rounding. None: the split product unless the weight gradient overlaps
-    # the next adjoint on a side stream (hidden widths <= 128), where the
-    # per-tile kernel shares the CUs better (batch 10k: 10 layers of d=64,
-    # [256, 256]: 30 vs 40 us per layer; d=32, [128, 128] overlapped: the
-    # per-tile kernel 3 % faster per training step)
+    # rounding. None: the split product (overlapped with the next adjoint,
+    # batch 10k: d=64, [256, 256] 1.02 vs 1.25 ms per training step with the
+    # per-tile kernel; d=32, [128, 128] 0.527 vs 0.557 ms)
     wgrad_split = None
 
     def __repr__(self):
@@ -429,7 +427,7 @@ class MaskedCouplingRQSpline(NFModel):
             if ws.overlap:
                 ws.ev_adj[layer].record(main)
                 ws.side.wait_event(ws.ev_adj[layer])
-            split = self.wgrad_split if self.wgrad_split is not None else not ws.overlap
+            split = self.wgrad_split if self.wgrad_split is not None else True
             if split:
                 _native.call("fmc_flow_weight_grad_split", st_wg, C.byref(s), layer, ws.store[b].data_ptr(), ws.ld,
                              ws.aff[b].data_ptr(), ws.n_part, ws.grad.data_ptr(), ws.wg_ws.data_ptr(),
