@@ -470,7 +470,7 @@ class MaskedCouplingRQSpline(NFModel):
     def _config(self):
         return {"n_features": self._n_features, "n_layers": self.n_layers, "hidden_size": self.hidden_size,
                 "num_bins": self.num_bins, "spline_range": list(self.spline_range), "precision": self.precision,
-                "learnable_base": self._base_learnable}
+                "learnable_base": self._base_learnable, "dense_base": self._base_dense}
 
     def save_model(self, path: str):
         np.savez(path + ".npz", theta=self.theta.detach().cpu().numpy(), config=json.dumps(self._config()))
@@ -482,10 +482,16 @@ class MaskedCouplingRQSpline(NFModel):
         if cfg["n_features"] != self._n_features or cfg["n_layers"] != self.n_layers:
             raise ValueError("saved flow has a different shape")
         m = self.clone(torch.as_tensor(f["theta"], device=self.device))
-        # the saved base decides the base path (a full covariance, or a learnable base)
+        # the saved flow's base path (full covariance or diagonal, learnable
+        # or not); a file without it takes the path its covariance implies
         m._base_learnable = bool(cfg.get("learnable_base", False))
-        bc = m._stat("base_cov")
-        m._base_dense = bool(torch.count_nonzero(bc - torch.diag(torch.diagonal(bc))) > 0) or m._base_learnable
+        if "dense_base" in cfg:
+            m._base_dense = bool(cfg["dense_base"])
+        else:
+            bc = m._stat("base_cov")
+            m._base_dense = bool(torch.count_nonzero(bc - torch.diag(torch.diagonal(bc))) > 0) or m._base_learnable
+        m._configure()  # the packed-bias image holds the Cholesky factor only on the full-covariance path
+        m._dirty = True
         return m
 
     def print_parameters(self):
