@@ -197,11 +197,12 @@ __device__ __forceinline__ float fsqrt(float x) {
 // a division per element: start at tid, advance by 256 elements.
 struct TileWalk {
   int m, j, dm, dj, n;
-  __device__ TileWalk(int tid, int n_) : n(n_) {
+  // the (row, column) walk of nt threads over a tile of n columns
+  __device__ TileWalk(int tid, int n_, int nt = 256) : n(n_) {
     m = tid / n;
     j = tid - m * n;
-    dm = 256 / n;
-    dj = 256 - dm * n;
+    dm = nt / n;
+    dj = nt - dm * n;
   }
   __device__ void next() {
     j += dj;
@@ -881,8 +882,8 @@ struct PassSmem {
     o += rup(MT * ldx * 4, 16);
     ldr = o;
     o += rup(MT * 4, 16);
-    ldp = o;  // per-dim log-dets of a chunk, or the 4 waves' partial log-dets
-    o += rup(MT * (dm.G() > 4 ? dm.G() : 4) * 4, 16);
+    ldp = o;  // per-dim log-dets of a chunk, or the (up to 8) waves' partial log-dets
+    o += rup(MT * (dm.G() > 8 ? dm.G() : 8) * 4, 16);
     // two regions, each holding one of: hidden activations, the conditioner
     // input (region 2 at the start of a layer) or a chunk's spline
     // parameters (the region that does not hold the last hidden layer)

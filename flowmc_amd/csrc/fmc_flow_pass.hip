@@ -16,24 +16,29 @@ namespace fmc {
 // guarded 8-deep ring otherwise (sampling passes of 10^5 samples and more,
 // three workgroups per CU, where the clamped ring's extra loads cost more).
 constexpr int PASS_PF_EX = 4;
-template <bool EX, class PT, int MB, bool TR>
+template <bool EX, class PT, int MB, bool TR, int CB = 2>
 __device__ __forceinline__ void pass_panel(const typename PT::T* As, int lda, const typename PT::T* Bp, int KBtot,
-                                           int nb0, int nbstep, int ncb, int kb0, int kb1, f32x16_t (&acc)[2][MB],
+                                           int nb0, int nbstep, int ncb, int kb0, int kb1, f32x16_t (&acc)[CB][MB],
                                            int lane) {
   if constexpr (EX) {
-    BRing<PT, 2, PASS_PF_EX> ring;
+    BRing<PT, CB, PASS_PF_EX> ring;
     panel_prefetch(ring, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, lane);
-    panel_run<PT, MB, 2, PASS_PF_EX, TR>(ring, As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, 0, acc, lane);
+    panel_run<PT, MB, CB, PASS_PF_EX, TR>(ring, As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, 0, acc, lane);
   } else {
-    mfma_panel<PT, MB, 2, PASS_PF, TR>(As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, 0, acc, lane);
+    mfma_panel<PT, MB, CB, PASS_PF, TR>(As, lda, Bp, KBtot, nb0, nbstep, ncb, kb0, kb1, 0, acc, lane);
   }
 }
 
+// EX launches (small grids) run 8 wavefronts per workgroup, each with half
+// the column blocks and dim pairs of a 4-wavefront workgroup: twice the
+// wavefronts per SIMD where the grid is too small to fill the CUs.
 template <class PT, int MB, int KT, int NH, bool EX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(EX ? 512 : 256) __attribute__((amdgpu_waves_per_eu(3)))
 flow_pass_kernel(FlowDev f, PassArgs a) {
   using T = typename PT::T;
   constexpr int MT = 32 * MB;
+  constexpr int NW = EX ? 8 : 4, NT = 64 * NW;
+  constexpr int CBH = NW == 4 ? 2 : 1;  // column blocks per hidden panel
   extern __shared__ __align__(16) unsigned char smem[];
   const FlowDims& dm = f.dm;
   PassSmem<PT> sm(dm, MT);
@@ -59,7 +64,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   const bool dense_sample = dm.base_dense && a.dir != 0 && (a.flags & FL_SAMPLE);
 
   // ---- load tile (+ standardisation / base sampling)
-  for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+  for (TileWalk tw(tid, d, NT); tw.m < MT; tw.next()) {
     const int m = tw.m, j = tw.j;
     int64_t row = m0 + m;
     float v = 0.f;
@@ -82,7 +87,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   }
   __syncthreads();
   if (dense_sample) {  // base draw mean + L xi (jax.random.multivariate_normal, Cholesky method)
-    for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+    for (TileWalk tw(tid, d, NT); tw.m < MT; tw.next()) {
       const int m = tw.m, j = tw.j;
       const float* Lr = Lb + (int64_t)j * d;
       float acc = th[dm.th_base_mean() + j];
@@ -94,7 +99,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   // exact inverse with BASE_LOGP: the pass returns the flow log-density of
   // its output, log N(z) - sum of the inverse log-dets; start from -log N(z)
   const bool inv_logq = (a.dir == 2) && (a.flags & FL_BASE);
-  for (int m = tid; m < MT; m += 256) {
+  for (int m = tid; m < MT; m += NT) {
     float v = 0.f;
     if (inv_logq && dm.base_dense) {
       // -log N(z; mean, L L^T) = |u|^2 / 2 + sum log L_jj + d/2 log(2 pi), L u = z - mean
@@ -129,7 +134,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
   for (int step = 0; step < dm.L; ++step) {
     const int l = (a.dir == 0) ? step : dm.L - 1 - step;
     if (a.saved) {
-      for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+      for (TileWalk tw(tid, d, NT); tw.m < MT; tw.next()) {
         const int m = tw.m, j = tw.j;
         if (m0 + m < a.n) a.saved[((int64_t)l * a.n + m0 + m) * d + j] = xs[m * ldx + j];
       }
@@ -139,17 +144,17 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     // ---- ScalarAffine (forward / reference inverse apply it before the spline)
     if (affine_first) {
       const float e = expf(a.dir == 0 ? sc : -sc);
-      for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+      for (TileWalk tw(tid, d, NT); tw.m < MT; tw.next()) {
         const int m = tw.m, j = tw.j;
         float v = xs[m * ldx + j];
         xs[m * ldx + j] = (a.dir == 0) ? (v + sh) * e : v * e - sh;
       }
-      for (int m = tid; m < MT; m += 256) ldr[m] += (a.dir == 0 ? sc : -sc) * (float)d;
+      for (int m = tid; m < MT; m += NT) ldr[m] += (a.dir == 0 ? sc : -sc) * (float)d;
       __syncthreads();
     }
     // ---- conditioner input (conditioning dims only)
     const int dc = dm.dc(l), kc = dm.kc(l);
-    for (TileWalk tw(tid, kc); tw.m < MT; tw.next()) {
+    for (TileWalk tw(tid, kc, NT); tw.m < MT; tw.next()) {
       const int m = tw.m, k = tw.j;
       a0[m * sm.la0 + k] = PT::cvt(k < dc ? xs[m * ldx + dm.cond(l, k)] : 0.f);
     }
@@ -166,16 +171,16 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       for (int i = 0; i < NH; ++i) {
         T* out = (i & 1) ? reg2 : reg1;
         const int NBt = dm.hp(i) / 32;
-        for (int c0 = wave; c0 < NBt; c0 += 8) {
-          const int ncb = (c0 + 4 < NBt) ? 2 : 1;
-          f32x16_t acc[2][MB];
+        for (int c0 = wave; c0 < NBt; c0 += CBH * NW) {
+          const int ncb = (CBH == 2 && c0 + NW < NBt) ? 2 : 1;
+          f32x16_t acc[CBH][MB];
           // the accumulators start from the bias (an unused second block from the first's)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) bias_tr<MB>(acc[j], bl + dm.pb_b(i), j < ncb ? c0 + 4 * j : c0, lane);
-          pass_panel<EX, PT, MB, true>(in, ldin, W + dm.pk_wh(l, i), KB, c0, 4, ncb, 0, KB, acc, lane);
+          for (int j = 0; j < CBH; ++j) bias_tr<MB>(acc[j], bl + dm.pb_b(i), j < ncb ? c0 + NW * j : c0, lane);
+          pass_panel<EX, PT, MB, true, CBH>(in, ldin, W + dm.pk_wh(l, i), KB, c0, NW, ncb, 0, KB, acc, lane);
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
-            if (j < ncb) hidden_epilogue_tr<PT, MB>(out, sm.lh, nullptr, c0 + 4 * j, acc[j], lane);
+          for (int j = 0; j < CBH; ++j)
+            if (j < ncb) hidden_epilogue_tr<PT, MB>(out, sm.lh, nullptr, c0 + NW * j, acc[j], lane);
         }
         __syncthreads();
         in = out;
@@ -188,12 +193,12 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
       const int npair = dm.n_pairs(l), KB = hlp / PT::KSTEP, dtl = dm.dt(l), K = dm.K;
       const T* W3r = W + dm.pk_w3r(l);
       const float* b3r = bl + dm.pb_b3r(l);
-      float* ldw = ldp;  // [4 waves][MT] partial log-dets
+      float* ldw = ldp;  // [NW waves][MT] partial log-dets
       float ldacc[MB];
 #pragma unroll
       for (int r = 0; r < MB; ++r) ldacc[r] = 0.f;
       const int hf = lane >> 5;
-      for (int q = wave; q < npair; q += 4) {
+      for (int q = wave; q < npair; q += NW) {
         const int k = 2 * q + hf;
         // the accumulators start from the 32 spline biases of the lane's
         // dim (a lane past the last dim reads the pair's first dim's)
@@ -241,7 +246,11 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
         if (lane < 32) ldw[wave * MT + r * 32 + lane] = ldacc[r] + o;
       }
       __syncthreads();
-      for (int m = tid; m < MT; m += 256) ldr[m] += (ldw[m] + ldw[MT + m]) + (ldw[2 * MT + m] + ldw[3 * MT + m]);
+      for (int m = tid; m < MT; m += NT) {
+        float s = (ldw[m] + ldw[MT + m]) + (ldw[2 * MT + m] + ldw[3 * MT + m]);
+        if constexpr (NW == 8) s += (ldw[4 * MT + m] + ldw[5 * MT + m]) + (ldw[6 * MT + m] + ldw[7 * MT + m]);
+        ldr[m] += s;
+      }
       __syncthreads();
     }
     // ---- GEMM3 in chunks of G transformed dims + spline epilogue (K > 10)
@@ -249,15 +258,15 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     for (int c = 0; c < nch; ++c) {
       const int nc = dm.chunk_nc(l, c), cnt = dm.chunk_cnt(l, c), coff = dm.chunk_off(l, c);
       const int KB = hlp / PT::KSTEP, NBt = nc / 32;
-      for (int c0 = wave; c0 < NBt; c0 += 8) {
-        const int ncb = (c0 + 4 < NBt) ? 2 : 1;
-        f32x16_t acc[2][MB] = {};
-        mfma_panel<PT, MB, 2, PASS_PF>(hlast, sm.lh, W + dm.pk_w3(l), KB, coff / 32 + c0, 4, ncb, 0, KB, 0, acc,
-                                        lane);
+      for (int c0 = wave; c0 < NBt; c0 += CBH * NW) {
+        const int ncb = (CBH == 2 && c0 + NW < NBt) ? 2 : 1;
+        f32x16_t acc[CBH][MB] = {};
+        mfma_panel<PT, MB, CBH, PASS_PF>(hlast, sm.lh, W + dm.pk_w3(l), KB, coff / 32 + c0, NW, ncb, 0, KB, 0, acc,
+                                          lane);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < CBH; ++j) {
           if (j < ncb) {
-            const int col = (c0 + 4 * j) * 32 + (lane & 31);
+            const int col = (c0 + NW * j) * 32 + (lane & 31);
             const float bb = bl[dm.pb_b3(l) + coff + col];
 #pragma unroll
             for (int r = 0; r < MB; ++r)
@@ -267,7 +276,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
         }
       }
       __syncthreads();
-      for (int i = tid; i < MT * cnt; i += 256) {
+      for (int i = tid; i < MT * cnt; i += NT) {
         int m = i % MT, jj = i / MT;
         int j = dm.trans(l, c * dm.G() + jj);
         const float* pp = obuf + m * sm.lo_ + jj * P;
@@ -280,7 +289,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
         ldp[m * dm.G() + jj] = ldv;
       }
       __syncthreads();
-      for (int m = tid; m < MT; m += 256) {
+      for (int m = tid; m < MT; m += NT) {
         float s = ldr[m];
         for (int jj = 0; jj < cnt; ++jj) s += ldp[m * dm.G() + jj];
         ldr[m] = s;
@@ -289,16 +298,16 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     }
     if (!affine_first) {  // exact inverse: affine^-1 after spline^-1
       const float e = expf(-sc);
-      for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+      for (TileWalk tw(tid, d, NT); tw.m < MT; tw.next()) {
         const int m = tw.m, j = tw.j;
         xs[m * ldx + j] = xs[m * ldx + j] * e - sh;
       }
-      for (int m = tid; m < MT; m += 256) ldr[m] += -sc * (float)d;
+      for (int m = tid; m < MT; m += NT) ldr[m] += -sc * (float)d;
       __syncthreads();
     }
   }
   // ---- outputs
-  for (int m = tid; m < MT; m += 256) {
+  for (int m = tid; m < MT; m += NT) {
     int64_t row = m0 + m;
     if (row >= a.n) continue;
     float ld = ldr[m];
@@ -341,7 +350,7 @@ flow_pass_kernel(FlowDev f, PassArgs a) {
     if (a.logdet) a.logdet[row] = ld;
   }
   if (a.x_out) {
-    for (TileWalk tw(tid, d); tw.m < MT; tw.next()) {
+    for (TileWalk tw(tid, d, NT); tw.m < MT; tw.next()) {
       const int m = tw.m, j = tw.j;
       int64_t row = m0 + m;
       if (row >= a.n) continue;
@@ -357,7 +366,7 @@ template <class PT, int KT, int NH>
 static void launch_pass(dim3 g, dim3 b, size_t shm, hipStream_t st, const FlowDev& fd, const PassArgs& a) {
   if ((int)g.x <= 2 * device_cus()) {
     set_lds(flow_pass_kernel<PT, 1, KT, NH, true>, shm);
-    hipLaunchKernelGGL((flow_pass_kernel<PT, 1, KT, NH, true>), g, b, shm, st, fd, a);
+    hipLaunchKernelGGL((flow_pass_kernel<PT, 1, KT, NH, true>), g, dim3(512), shm, st, fd, a);
   } else {
     set_lds(flow_pass_kernel<PT, 1, KT, NH, false>, shm);
     hipLaunchKernelGGL((flow_pass_kernel<PT, 1, KT, NH, false>), g, b, shm, st, fd, a);
