@@ -1394,7 +1394,7 @@ def main():
                     help="Rosenbrock MALA / GRW, batched HMC: launch the fused steps eagerly instead of replaying a HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py: the benchmarks run on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -17,7 +17,7 @@ import torch
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libflowmc_hip.so"
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 FLOW_MAX_HIDDEN = 4  # FMC_FLOW_MAX_HIDDEN of include/flowmc.h
 NF_MH_NOALLOC_MAX = 1 << 22  # FMC_NF_MH_NOALLOC_MAX of include/flowmc.h
 

@@ -19,12 +19,12 @@
 extern "C" {
 #endif
 
-#define FMC_ABI_VERSION 2
+#define FMC_ABI_VERSION 3
 
 typedef enum fmc_status {
   FMC_OK = 0,
   FMC_ERR_INVALID_ARG = 1,  /* null pointer, bad size, bad enum            */
-  FMC_ERR_UNSUPPORTED = 2,  /* valid request this build cannot serve       */
+  FMC_ERR_UNSUPPORTED = 2,  /* valid request the library cannot serve      */
   FMC_ERR_LAUNCH = 3,       /* kernel launch failed                        */
   FMC_ERR_HIP = 4           /* other HIP runtime error                     */
 } fmc_status;
