@@ -664,6 +664,25 @@ def test_batched_hmc_uniform_kernel_matches_dispatch_kernel(gpu, kind, nv):
                 assert torch.allclose(a, b, rtol=1e-5, atol=tol), (d, float((a - b).abs().max()))
 
 
+def test_batched_hmc_uniform_kind_mismatch_marks_the_slots(gpu):
+    # a launch that promises a uniform kind the slots do not hold (or a
+    # metric on the unit-metric kernel) leaves those chains where they were
+    # and sets their log-prob state to NaN instead of evaluating the wrong
+    # target
+    from flowmc_amd.resource.kernel.batched_HMC import BatchedHMC
+
+    dims = [8, 40]
+    bh = BatchedHMC([TG.Funnel(d) for d in dims], 64, step_size=0.05, n_leapfrog=4)
+    bh.init_positions(fr.PRNGKey(0))
+    x0 = [x.clone() for x in bh.x]
+    bh.uniform_kind = lambda: TG.GaussianDiag.kind  # the wrong promise
+    bh.run(fr.PRNGKey(1), 4, fused_steps=4)
+    torch.cuda.synchronize()
+    for x, xi, lp in zip(bh.x, x0, bh.logp):
+        assert torch.equal(x, xi)
+        assert bool(torch.isnan(lp).all())
+
+
 def test_batched_hmc_keeps_the_funnel_law_at_full_size(gpu):
     # 32 funnels (d 8..256, log-uniform), 256 chains each, L = 10, eps = 0.05,
     # chains started from exact funnel draws: after 96 HMC steps the pooled
