@@ -654,6 +654,19 @@ struct AdamHyper {
   float bc1, bc2;  // 1 - b1^t, 1 - b2^t
 };
 
+// b^e by square-and-multiply in FP64: the same sequence of products on the
+// host (step given as an argument) and on the device (step read from device
+// memory), so both give the same bias corrections bit for bit
+__host__ __device__ inline double ipow_d(double b, int e) {
+  double r = 1.0;
+  while (e > 0) {
+    if (e & 1) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return r;
+}
+
 // Global norm: fixed-order tree sum of the per-block partials, one block.
 // sqrt of the sum goes to part[n_part] (and, as float, to norm_out).
 __global__ void __launch_bounds__(256) norm_kernel(double* part, int n_part, float* norm_out) {
@@ -685,7 +698,13 @@ __device__ __forceinline__ void adamw_one(float& th, float g, float& mi, float& 
 }
 
 __global__ void __launch_bounds__(256) adamw_kernel(float* theta, const float* __restrict__ g, float* m, float* v,
-                                                    int64_t n, AdamHyper h, const double* nrm) {
+                                                    int64_t n, AdamHyper h, const double* nrm,
+                                                    const int32_t* step_dev) {
+  if (step_dev) {  // step count in device memory (a launch recorded once and replayed)
+    const int t = *step_dev;
+    h.bc1 = (float)(1.0 - ipow_d((double)h.b1, t));
+    h.bc2 = (float)(1.0 - ipow_d((double)h.b2, t));
+  }
   const float gn = (float)nrm[0];
   const bool clip = !(gn < h.max_norm);
   const int64_t n4 = n / 4;
@@ -847,11 +866,11 @@ extern "C" int fmc_flow_weight_grad_split(void* stream, const fmc_flow* fl, int3
   return check_launch("flow weight grad (reduce)");
 }
 
-extern "C" int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n, float lr,
-                              float b1, float b2, float eps, float weight_decay, float max_norm, int32_t step,
-                              double* workspace, int32_t workspace_len, float* grad_norm) {
+static int adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n, float lr,
+                      float b1, float b2, float eps, float weight_decay, float max_norm, int32_t step,
+                      const int32_t* step_dev, double* workspace, int32_t workspace_len, float* grad_norm) {
   FMC_ARG(theta && grad && m && v && workspace, "NULL pointer");
-  FMC_ARG(n >= 0 && step >= 1, "n must be >= 0 and step >= 1");
+  FMC_ARG(n >= 0 && (step_dev != nullptr || step >= 1), "n must be >= 0 and step >= 1");
   FMC_ARG(workspace_len >= 2, "workspace_len must be >= 2");
   FMC_ARG(((uintptr_t)theta | (uintptr_t)grad | (uintptr_t)m | (uintptr_t)v) % 16 == 0,
           "theta, grad, m and v must be 16-byte aligned");
@@ -865,11 +884,28 @@ extern "C" int fmc_adamw_clip(void* stream, float* theta, const float* grad, flo
   if (rc) return rc;
   hipLaunchKernelGGL(norm_kernel, dim3(1), dim3(256), 0, st, workspace, nb, grad_norm);
   if ((rc = check_launch("adamw norm"))) return rc;
-  AdamHyper h{lr, b1, b2, eps, weight_decay, max_norm, (float)(1.0 - pow((double)b1, step)),
-              (float)(1.0 - pow((double)b2, step))};
+  const int t = step_dev ? 1 : step;  // (the kernel recomputes the corrections from step_dev)
+  AdamHyper h{lr, b1, b2, eps, weight_decay, max_norm, (float)(1.0 - ipow_d((double)b1, t)),
+              (float)(1.0 - ipow_d((double)b2, t))};
   int nb2 = (int)((n / 4 + 255) / 256);
   if (nb2 > 2048) nb2 = 2048;
   if (nb2 < 1) nb2 = 1;
-  hipLaunchKernelGGL(adamw_kernel, dim3(nb2), dim3(256), 0, st, theta, grad, m, v, n, h, workspace + nb);
+  hipLaunchKernelGGL(adamw_kernel, dim3(nb2), dim3(256), 0, st, theta, grad, m, v, n, h, workspace + nb, step_dev);
   return check_launch("adamw update");
+}
+
+extern "C" int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n, float lr,
+                              float b1, float b2, float eps, float weight_decay, float max_norm, int32_t step,
+                              double* workspace, int32_t workspace_len, float* grad_norm) {
+  return adamw_clip(stream, theta, grad, m, v, n, lr, b1, b2, eps, weight_decay, max_norm, step, nullptr, workspace,
+                    workspace_len, grad_norm);
+}
+
+extern "C" int fmc_adamw_clip_dev(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n,
+                                  float lr, float b1, float b2, float eps, float weight_decay, float max_norm,
+                                  const int32_t* step_dev, double* workspace, int32_t workspace_len,
+                                  float* grad_norm) {
+  FMC_ARG(step_dev != nullptr, "step_dev is NULL");
+  return adamw_clip(stream, theta, grad, m, v, n, lr, b1, b2, eps, weight_decay, max_norm, 0, step_dev, workspace,
+                    workspace_len, grad_norm);
 }

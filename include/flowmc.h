@@ -497,6 +497,16 @@ int fmc_flow_weight_grad_split(void* stream, const fmc_flow* flow, int32_t layer
 int fmc_adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n,
                    float lr, float b1, float b2, float eps, float weight_decay, float max_norm,
                    int32_t step, double* workspace, int32_t workspace_len, float* grad_norm);
+/* The same update with the step count read from device memory (*step_dev,
+ * counting from 1) when the kernel runs: a launch recorded once in a HIP
+ * graph and replayed takes each replay's count (the bias corrections
+ * 1 - b^t are the same FP64 products as fmc_adamw_clip's). No reference
+ * counterpart beyond fmc_adamw_clip's: for callers that replay a captured
+ * training step. */
+int fmc_adamw_clip_dev(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n,
+                       float lr, float b1, float b2, float eps, float weight_decay, float max_norm,
+                       const int32_t* step_dev, double* workspace, int32_t workspace_len,
+                       float* grad_norm);
 
 #ifdef __cplusplus
 }

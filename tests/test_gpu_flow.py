@@ -345,6 +345,34 @@ def test_adamw_kernel_matches_oracle(gpu, clip):
     assert float((dt.cpu().double() - th).abs().max()) <= 1e-6
 
 
+@pytest.mark.parametrize("hidden", [(32, 32), (128, 128)])
+def test_graph_train_steps_match_eager_steps(gpu, hidden):
+    """Training steps replayed from a recorded HIP graph (graph_train) give
+    the eager steps' loss, parameters and Adam moments bit for bit, step
+    after step, with new batches and eager work between the replays; the
+    model then evaluates with the updated parameters."""
+    d, L, K = 8, 3, 8
+    m1 = MaskedCouplingRQSpline(d, L, list(hidden), K, fr.PRNGKey(0))
+    m2 = m1.clone()  # (the clone shares m1's workspace: their steps interleave on it)
+    m1.graph_train = False
+    m2.graph_train = True
+    opt = adamw(1e-3, max_norm=1.0)
+    s1, s2 = opt.init(m1), opt.init(m2)
+    g = torch.Generator().manual_seed(8)
+    for step in range(4):
+        xb = (torch.randn(512, d, generator=g) * 2).cuda()
+        l1, m1, s1 = m1.train_step(xb, opt, s1)
+        l2, m2, s2 = m2.train_step(xb, opt, s2)
+        _ = float(m2.log_prob(xb[:16]).sum())  # eager work (and a repack) between replays
+        torch.cuda.synchronize()
+        assert torch.equal(l1, l2), step
+        assert torch.equal(m1.theta, m2.theta), step
+        assert torch.equal(s1.mu, s2.mu) and torch.equal(s1.nu, s2.nu), step
+        assert s1.count == s2.count == step + 1
+    x = torch.randn(64, d, generator=g).cuda()
+    assert torch.equal(m1.log_prob(x), m2.log_prob(x))
+
+
 def test_train_step_matches_oracle_fp32(gpu):
     d, L, K, H = 6, 2, 5, (32, 32)
     m, p = make(d, L, K, H, "float32", seed=5)
