@@ -90,11 +90,14 @@ class NFModel(Resource):
         data = torch.as_tensor(data, dtype=torch.float32)
         if not data.is_cuda:
             data = data.cuda()
-        loss_values = torch.zeros(num_epochs, dtype=torch.float32)
-        best_model, best_state, best_loss = self, state, 1e9
+        state0 = state
         model = self.clone()
         state = state.clone()
-        model.set_data_stats(data.mean(0), torch.cov(data.T).reshape(self.n_features, self.n_features))
+        # jnp.cov(data.T) (ddof 1) without torch.cov, whose degrees-of-freedom
+        # check reads a device scalar back (a host sync before every training)
+        n = data.shape[0]
+        xc = data.T - data.T.mean(1, keepdim=True)
+        model.set_data_stats(data.mean(0), ((xc @ xc.T) / max(n - 1, 1)).reshape(self.n_features, self.n_features))
         it = range(num_epochs)
         if verbose:
             try:
@@ -103,16 +106,38 @@ class NFModel(Resource):
                 it = trange(num_epochs, desc="Training NF", miniters=max(1, int(num_epochs / 10)))
             except ImportError:  # pragma: no cover
                 pass
+        # The best epoch (the first of lowest last-minibatch loss, base.py:196-200)
+        # is tracked on the device: its parameters and Adam moments are kept by
+        # selects, and the losses are read back once after the loop instead of
+        # a host sync per epoch (verbose reads them as the reference prints).
+        dev = data.device
+        losses = torch.empty(num_epochs, dtype=torch.float32, device=dev)
+        best_loss = torch.full((), 1e9, dtype=torch.float32, device=dev)
+        best_epoch = torch.full((), -1, dtype=torch.float32, device=dev)
+        best = None
+        counts = []
         for epoch in it:
             rng, input_rng = frandom.split(rng)
             value, model, state = model.train_epoch(input_rng, optim, state, data, batch_size)
-            v = float(value)
-            loss_values[epoch] = v
-            if v < best_loss:
-                best_model, best_state, best_loss = model.clone(), state.clone(), v
+            losses[epoch] = torch.as_tensor(value, dtype=torch.float32, device=dev)
+            counts.append(state.count)
+            better = losses[epoch] < best_loss
+            if best is None:
+                best = [model.theta.clone(), state.mu.clone(), state.nu.clone()]
+            for dst, src in zip(best, (model.theta, state.mu, state.nu)):
+                torch.where(better, src, dst, out=dst)
+            best_loss = torch.where(better, losses[epoch], best_loss)
+            best_epoch = torch.where(better, torch.full_like(best_epoch, epoch), best_epoch)
             if verbose and hasattr(it, "set_description"):
                 if num_epochs > 10 and epoch % int(num_epochs / 10) == 0:
-                    it.set_description(f"Training NF, current loss: {v:.3f}")
+                    it.set_description(f"Training NF, current loss: {float(losses[epoch]):.3f}")
+        back = torch.cat([losses, best_epoch.reshape(1)]).cpu()
+        loss_values, be = back[:num_epochs].clone(), int(back[num_epochs])
+        if be < 0:
+            best_model, best_state = self, state0
+        else:
+            best_model = model.clone(best[0])
+            best_state = type(state)(best[1], best[2], counts[be])
         return rng, best_model, best_state, loss_values
 
     def save_resource(self, path):

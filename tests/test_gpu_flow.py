@@ -395,6 +395,41 @@ def test_train_step_matches_oracle_fp32(gpu):
     assert state.count == 1
 
 
+def test_train_keeps_the_first_lowest_loss_epoch(gpu):
+    """NFModel.train picks the epoch of lowest last-minibatch loss on the
+    device (base.py:196-200: the first such epoch); the same loop with a host
+    read-back of every epoch's loss and a clone at each improvement, as the
+    reference writes it, returns the same parameters, Adam moments and count
+    and the same loss values."""
+    from flowmc_amd.resource.model.nf_model.base import NFModel
+
+    d = 4
+    g = torch.Generator().manual_seed(17)
+    data = (torch.randn(3000, d, generator=g) * 1.5).cuda()
+    m = MaskedCouplingRQSpline(d, 3, [32, 32], 8, fr.PRNGKey(4))
+    opt = Optimizer(m, learning_rate=3e-2)  # large steps: the loss goes up and down
+    rng, best, st, losses = m.train(fr.PRNGKey(5), data, opt.optim, opt.optim_state, 6, 500, verbose=False)
+    # the reference's loop
+    model, state = m.clone(), opt.optim_state.clone()
+    n = data.shape[0]
+    xc = data.T - data.T.mean(1, keepdim=True)
+    cov = (xc @ xc.T) / (n - 1)  # train's jnp.cov (ddof 1), as torch.cov gives it
+    assert torch.allclose(cov, torch.cov(data.T), rtol=1e-5, atol=1e-6)
+    model.set_data_stats(data.mean(0), cov)
+    key, ref_best, ref_loss, ref_losses = fr.PRNGKey(5), None, 1e9, []
+    for _ in range(6):
+        key, sub = fr.split(key)
+        value, model, state = NFModel.train_epoch(model, sub, opt.optim, state, data, 500)
+        v = float(value)
+        ref_losses.append(v)
+        if v < ref_loss:
+            ref_best, ref_loss = (model.clone(), state.clone()), v
+    assert torch.equal(losses, torch.tensor(ref_losses))
+    assert torch.equal(best.theta, ref_best[0].theta)
+    assert st.count == ref_best[1].count
+    assert torch.equal(st.mu, ref_best[1].mu) and torch.equal(st.nu, ref_best[1].nu)
+
+
 def test_training_reduces_loss_and_best_model(gpu):
     d = 4
     g = torch.Generator().manual_seed(7)
