@@ -181,6 +181,7 @@ _SIGNATURES: dict[str, list] = {
     "fmc_flow_weight_grad_split": [P, C.POINTER(fmc_flow), I32, P, I64, P, I32, P, P, I64],
     "fmc_adamw_clip": [P, P, P, P, P, I64, F32, F32, F32, F32, F32, F32, I32, P, I32, P],
     "fmc_adamw_clip_dev": [P, P, P, P, P, I64, F32, F32, F32, F32, F32, F32, P, P, I32, P],
+    "fmc_flow_backward": [P, P, P, P, I64, P, P, P, P, P, I64, P, P, I32, P, I32, P, I64],
 }
 _RESTYPES = {"fmc_last_error": C.c_char_p}
 

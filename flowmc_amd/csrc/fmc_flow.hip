@@ -909,3 +909,79 @@ extern "C" int fmc_adamw_clip_dev(void* stream, float* theta, const float* grad,
   return adamw_clip(stream, theta, grad, m, v, n, lr, b1, b2, eps, weight_decay, max_norm, 0, step_dev, workspace,
                     workspace_len, grad_norm);
 }
+
+// The whole backward sweep of a training step (nf_model/base.py:98-125, the
+// gradient of NFModel.loss_fn): per layer, last to first, the adjoint on
+// `stream`, then that layer's weight gradient. With a side stream the weight
+// gradient of layer l runs there beside the adjoint of layer l-1: the two
+// stores / affine-partial buffers alternate between layers, an adjoint waits
+// for the weight gradient that last read its store, and `stream` joins the
+// side stream before returning. The events are the call's own (created and
+// released here), so overlapping calls on other streams do not interfere.
+extern "C" int fmc_flow_backward(void* stream, void* side_stream, const fmc_flow* fl, const float* saved, int64_t n,
+                                 const float* g_logp, float* gy, float* gx, void* store0, void* store1,
+                                 int64_t store_ld, float* aff0, float* aff1, int32_t n_partials, float* grad,
+                                 int32_t split, float* wg_workspace, int64_t wg_workspace_bytes) {
+  FMC_ARG(fl != nullptr && saved && g_logp && gy && gx && store0 && aff0 && grad, "NULL pointer");
+  const bool two = side_stream != nullptr;
+  FMC_ARG(!two || (store1 != nullptr && aff1 != nullptr), "the side-stream schedule needs store1 and aff1");
+  FMC_ARG(!split || wg_workspace != nullptr, "the split weight gradient needs its workspace");
+  FMC_ARG(fl->n_layers >= 1 && n >= 1, "bad n_layers / n");
+  hipStream_t main_st = (hipStream_t)stream, side = two ? (hipStream_t)side_stream : main_st;
+  auto hip_fail = [](const char* what) {
+    set_error(FMC_ERR_HIP, "%s failed", what);
+    return (int)FMC_ERR_HIP;
+  };
+  hipEvent_t ev_adj = nullptr, ev_wg[2] = {nullptr, nullptr};
+  if (two) {
+    for (hipEvent_t* e : {&ev_adj, &ev_wg[0], &ev_wg[1]}) {
+      if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+        for (hipEvent_t x : {ev_adj, ev_wg[0], ev_wg[1]})
+          if (x) (void)hipEventDestroy(x);
+        return hip_fail("hipEventCreateWithFlags");
+      }
+    }
+  }
+  bool pending[2] = {false, false};
+  int rc = FMC_OK;
+  const int L = fl->n_layers;
+  const int64_t layer_elems = n * (int64_t)fl->n_dim;
+  float *gin = gy, *gout = gx;
+  for (int i = 0; i < L; ++i) {
+    const int layer = L - 1 - i, b = two ? (i & 1) : 0;
+    void* st = b ? store1 : store0;
+    float* af = b ? aff1 : aff0;
+    if (pending[b] && hipStreamWaitEvent(main_st, ev_wg[b], 0) != hipSuccess) {
+      rc = hip_fail("hipStreamWaitEvent");
+      break;
+    }
+    rc = fmc_flow_backward_layer(stream, fl, layer, saved + layer * layer_elems, n, g_logp, gin, gout, st, store_ld, af,
+                                 n_partials);
+    if (rc) break;
+    if (two && (hipEventRecord(ev_adj, main_st) != hipSuccess || hipStreamWaitEvent(side, ev_adj, 0) != hipSuccess)) {
+      rc = hip_fail("hipEventRecord / hipStreamWaitEvent");
+      break;
+    }
+    rc = split ? fmc_flow_weight_grad_split(side, fl, layer, st, store_ld, af, n_partials, grad, wg_workspace,
+                                            wg_workspace_bytes)
+               : fmc_flow_weight_grad(side, fl, layer, st, store_ld, af, n_partials, grad);
+    if (rc) break;
+    if (two) {
+      if (hipEventRecord(ev_wg[b], side) != hipSuccess) {
+        rc = hip_fail("hipEventRecord");
+        break;
+      }
+      pending[b] = true;
+    }
+    float* t = gin;
+    gin = gout;
+    gout = t;
+  }
+  if (two) {
+    for (int b = 0; b < 2; ++b)
+      if (pending[b] && hipStreamWaitEvent(main_st, ev_wg[b], 0) != hipSuccess && rc == FMC_OK)
+        rc = hip_fail("hipStreamWaitEvent");
+    for (hipEvent_t x : {ev_adj, ev_wg[0], ev_wg[1]}) (void)hipEventDestroy(x);
+  }
+  return rc;
+}

@@ -57,8 +57,6 @@ class _Workspace:
         self.aff = [torch.empty((self.n_part, 2), dtype=torch.float32, device=dev) for _ in range(n_buf)]
         if self.overlap:
             self.side = torch.cuda.Stream(device=dev)  # default priority
-            self.ev_adj = [torch.cuda.Event() for _ in range(L)]
-            self.ev_wg = [torch.cuda.Event() for _ in range(L)]
         self.grad = torch.zeros(model.theta.numel(), dtype=torch.float32, device=dev)
         # partial blocks of the split weight-gradient product (fmc_flow_weight_grad_split)
         nbytes = C.c_int64(0)
@@ -410,44 +408,23 @@ class MaskedCouplingRQSpline(NFModel):
 
     def _backward(self, ws: _Workspace, B: int):
         """Per layer, last to first: the adjoint, then the weight gradient of
-        that layer. When the workspace overlaps them, the weight gradient runs
-        on the side stream (it only reads its layer's store and affine
-        partials) beside the next adjoint on the current stream; the stores
-        alternate, an adjoint waits until the weight gradient that last read
-        its store has finished, and the current stream joins the side stream
-        before returning. Results do not depend on the schedule."""
+        that layer, in one library call (fmc_flow_backward). When the
+        workspace overlaps them, the weight gradient runs on the side stream
+        (it only reads its layer's store and affine partials) beside the next
+        adjoint on the current stream; the stores alternate, an adjoint waits
+        until the weight gradient that last read its store has finished, and
+        the current stream joins the side stream before returning. Results
+        do not depend on the schedule."""
         s = self._struct()
-        main = torch.cuda.current_stream(self.device)
-        st = main.cuda_stream
-        st_wg = ws.side.cuda_stream if ws.overlap else st
+        st = torch.cuda.current_stream(self.device).cuda_stream
         ws.grad.zero_()
-        gy, gx = ws.gy, ws.gx
-        last_read = [None, None]
-        for i, layer in enumerate(reversed(range(self.n_layers))):
-            b = (i & 1) if ws.overlap else 0
-            if last_read[b] is not None:
-                main.wait_event(last_read[b])
-            _native.call("fmc_flow_backward_layer", st, C.byref(s), layer, ws.saved[layer].data_ptr(), B,
-                         ws.g_logp.data_ptr(), gy.data_ptr(), gx.data_ptr(), ws.store[b].data_ptr(), ws.ld,
-                         ws.aff[b].data_ptr(), ws.n_part)
-            if ws.overlap:
-                ws.ev_adj[layer].record(main)
-                ws.side.wait_event(ws.ev_adj[layer])
-            split = self.wgrad_split if self.wgrad_split is not None else True
-            if split:
-                _native.call("fmc_flow_weight_grad_split", st_wg, C.byref(s), layer, ws.store[b].data_ptr(), ws.ld,
-                             ws.aff[b].data_ptr(), ws.n_part, ws.grad.data_ptr(), ws.wg_ws.data_ptr(),
-                             ws.wg_ws.numel() * 4)
-            else:
-                _native.call("fmc_flow_weight_grad", st_wg, C.byref(s), layer, ws.store[b].data_ptr(), ws.ld,
-                             ws.aff[b].data_ptr(), ws.n_part, ws.grad.data_ptr())
-            if ws.overlap:
-                ws.ev_wg[layer].record(ws.side)
-                last_read[b] = ws.ev_wg[layer]
-            gy, gx = gx, gy
-        for e in last_read:
-            if e is not None:
-                main.wait_event(e)
+        split = self.wgrad_split if self.wgrad_split is not None else True
+        two = ws.overlap and len(ws.store) > 1
+        _native.call("fmc_flow_backward", st, ws.side.cuda_stream if two else None, C.byref(s), ws.saved.data_ptr(),
+                     B, ws.g_logp.data_ptr(), ws.gy.data_ptr(), ws.gx.data_ptr(), ws.store[0].data_ptr(),
+                     ws.store[1].data_ptr() if two else None, ws.ld, ws.aff[0].data_ptr(),
+                     ws.aff[1].data_ptr() if two else None, ws.n_part, ws.grad.data_ptr(), int(split),
+                     ws.wg_ws.data_ptr(), ws.wg_ws.numel() * 4)
 
     # True: training steps of one batch size, parameter buffers, optimiser
     # state and hyperparameters are recorded once as a HIP graph (pack,

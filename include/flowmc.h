@@ -473,6 +473,21 @@ int fmc_flow_backward_layer(void* stream, const fmc_flow* flow, int32_t layer,
                             const float* x_layer, int64_t n, const float* g_logp, const float* gy,
                             float* gx, void* store, int64_t store_ld, float* affine_partials,
                             int32_t n_partials);
+/* The backward sweep of a training step: for layer = n_layers-1 .. 0 the
+ * adjoint (fmc_flow_backward_layer, reading saved[layer] of the forward's
+ * saved layer inputs [n_layers, n, d], gy -> gx, the two swapping between
+ * layers: the input gradient ends in gy for an even layer count, else gx),
+ * then that layer's weight gradient into grad (fmc_flow_weight_grad_split
+ * with wg_workspace when split != 0, else fmc_flow_weight_grad). With
+ * side_stream != NULL the weight gradient of each layer runs there beside
+ * the next adjoint, on store0 / aff0 and store1 / aff1 in turn, and
+ * `stream` joins side_stream before the call returns; NULL runs everything
+ * on `stream` with store0 / aff0. Replaces the per-layer launches of the
+ * gradient of NFModel.loss_fn (nf_model/base.py:98-125) with one call. */
+int fmc_flow_backward(void* stream, void* side_stream, const fmc_flow* flow, const float* saved, int64_t n,
+                      const float* g_logp, float* gy, float* gx, void* store0, void* store1,
+                      int64_t store_ld, float* aff0, float* aff1, int32_t n_partials, float* grad,
+                      int32_t split, float* wg_workspace, int64_t wg_workspace_bytes);
 /* Weight / bias / ScalarAffine gradients of `layer` into grad (theta layout),
  * from the store and the store_ld / 32 affine partials of the backward. */
 int fmc_flow_weight_grad(void* stream, const fmc_flow* flow, int32_t layer, const void* store,
