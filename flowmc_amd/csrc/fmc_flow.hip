@@ -562,44 +562,61 @@ __global__ void __launch_bounds__(256) flow_pack_kernel(FlowDev f, PackTable tab
   const int64_t off = par ? pick(tab.off[1], which) : pick(tab.off[0], which);
   const int N = par ? pick(tab.n[1], which) : pick(tab.n[0], which);
   const int Kdim = par ? pick(tab.kd[1], which) : pick(tab.kd[0], which);
-  const int64_t size = (int64_t)N * Kdim;
+  // One image holds at most a few hundred thousand elements (the host checks
+  // that it fits 31 bits), so the decode runs in 32-bit arithmetic. A thread
+  // builds one lane's EPL consecutive elements (one MFMA B fragment, a single
+  // vector store); the source row of the untransposed images (kinds 0, 1, 4)
+  // is decoded once per fragment.
+  static_assert((PT::EPL & (PT::EPL - 1)) == 0 && sizeof(typename PT::BF) == PT::EPL * sizeof(typename PT::T),
+                "a B fragment is EPL elements");
+  const int n_frag = N * Kdim / PT::EPL;  // Kdim is a multiple of KSTEP
   typename PT::T* out = packed + pk_layer + off;
-  const int KB = Kdim / PT::KSTEP;
+  const unsigned KB = (unsigned)(Kdim / PT::KSTEP);
   const float* Wi = thl + pick(tab.thw, i);  // W_i [h_i x in_i] (kinds 0 and 2)
   const int in_i = pick(tab.in_w, i), rows_i = pick(tab.rows, i);
-  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < size; idx += (int64_t)gridDim.x * 256) {
-    const int ee = (int)(idx % PT::EPL);
-    const int lane = (int)((idx / PT::EPL) % 64);
-    const int64_t blk = idx / (64 * PT::EPL);
-    const int kb = (int)(blk % KB), nb = (int)(blk / KB);
+  for (int q = blockIdx.x * 256 + threadIdx.x; q < n_frag; q += gridDim.x * 256) {
+    const int lane = q & 63;
+    const unsigned blk = (unsigned)q >> 6;
+    const int nb = (int)(blk / KB), kb = (int)(blk - (unsigned)nb * KB);
     const int n = nb * 32 + (lane & 31);
-    const int k = kb * PT::KSTEP + (lane >> 5) * PT::EPL + ee;
-    float v = 0.f;
-    switch (kind) {
-      case 0:  // row n of W_i, input column k (a conditioning dim for i = 0)
-        if (n < rows_i && (i == 0 ? k < dm.dc(l) : k < in_i)) v = Wi[(int64_t)n * in_i + (i == 0 ? dm.cond(l, k) : k)];
-        break;
-      case 1: {
-        int row = dm.w3_row(l, n);
-        if (row >= 0 && k < hl) v = w3[(int64_t)row * hl + k];
-      } break;
-      case 2:  // transpose: row n = input column, k = output row of W_i
-        if (k < rows_i && (i == 0 ? n < dm.dc(l) : n < in_i)) v = Wi[(int64_t)k * in_i + (i == 0 ? dm.cond(l, n) : n)];
-        break;
-      case 3: {
-        int row = dm.w3_row(l, k);
-        if (row >= 0 && n < hl) v = w3[(int64_t)row * hl + n];
-      } break;
-      case 4: {
-        int row = dm.w3r_row(l, n);
-        if (row >= 0 && k < hl) v = w3[(int64_t)row * hl + k];
-      } break;
-      default: {
-        int row = dm.w3k_row(l, k);
-        if (row >= 0 && n < hl) v = w3[(int64_t)row * hl + n];
-      } break;
+    const int k0 = kb * PT::KSTEP + (lane >> 5) * PT::EPL;
+    int64_t src = -1;  // kinds 0, 1, 4: offset of source row n
+    if (kind == 0) {
+      if (n < rows_i) src = (int64_t)n * in_i;
+    } else if (kind == 1 || kind == 4) {
+      const int row = kind == 1 ? dm.w3_row(l, n) : dm.w3r_row(l, n);
+      if (row >= 0) src = (int64_t)row * hl;
     }
-    out[idx] = PT::cvt(v);
+    typename PT::T el[PT::EPL];
+#pragma unroll
+    for (int e = 0; e < PT::EPL; ++e) {
+      const int k = k0 + e;
+      float v = 0.f;
+      switch (kind) {
+        case 0:  // row n of W_i, input column k (a conditioning dim for i = 0)
+          if (src >= 0 && (i == 0 ? k < dm.dc(l) : k < in_i)) v = Wi[src + (i == 0 ? dm.cond(l, k) : k)];
+          break;
+        case 1:
+        case 4:
+          if (src >= 0 && k < hl) v = w3[src + k];
+          break;
+        case 2:  // transpose: row n = input column, k = output row of W_i
+          if (k < rows_i && (i == 0 ? n < dm.dc(l) : n < in_i)) v = Wi[(int64_t)k * in_i + (i == 0 ? dm.cond(l, n) : n)];
+          break;
+        case 3: {
+          const int row = dm.w3_row(l, k);
+          if (row >= 0 && n < hl) v = w3[(int64_t)row * hl + n];
+        } break;
+        default: {
+          const int row = dm.w3k_row(l, k);
+          if (row >= 0 && n < hl) v = w3[(int64_t)row * hl + n];
+        } break;
+      }
+      el[e] = PT::cvt(v);
+    }
+    typename PT::BF frag;
+    __builtin_memcpy(&frag, el, sizeof(frag));
+    *reinterpret_cast<typename PT::BF*>(out + (int64_t)q * PT::EPL) = frag;
   }
 }
 
@@ -638,7 +655,16 @@ __global__ void __launch_bounds__(256) base_chol_kernel(const float* __restrict_
 // --------------------------------------------------------------- AdamW ---
 __global__ void __launch_bounds__(256) sumsq_kernel(const float* __restrict__ g, int64_t n, double* part) {
   double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+  const int64_t n4 = n / 4;  // g is 16-byte aligned (checked by the caller)
+  const float4* g4 = reinterpret_cast<const float4*>(g);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256) {
+    const float4 v = g4[i];
+    s += (double)v.x * v.x;
+    s += (double)v.y * v.y;
+    s += (double)v.z * v.z;
+    s += (double)v.w * v.w;
+  }
+  for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     double v = g[i];
     s += v * v;
   }
@@ -769,8 +795,20 @@ extern "C" int fmc_flow_pack(void* stream, const fmc_flow* fl) {
     const int64_t c[3] = {(int64_t)dm.n3p(l) * dm.hlp(), (int64_t)dm.n3r(l) * dm.hlp(), dm.pb_size(l)};
     for (int64_t v : c) mx = v > mx ? v : mx;
   }
-  unsigned gx = (unsigned)((mx + 255) / 256 < 128 ? (mx + 255) / 256 : 128);
+  FMC_ARG(mx < ((int64_t)1 << 31), "a packed weight image exceeds 2^31 elements");
+  FMC_ARG((uintptr_t)fl->packed % 16 == 0, "packed must be 16-byte aligned");
   const PackTable tab = pack_table(dm);
+  const int epl = fl->precision == FMC_PREC_BF16 ? 8 : 1;  // elements per B fragment
+  for (int par = 0; par < 2; ++par) {
+    FMC_ARG(tab.pk_size[par] % epl == 0, "packed layer size is not a whole number of fragments");
+    for (int w = 0; w < tab.n_img; ++w)
+      FMC_ARG(tab.off[par][w] % epl == 0, "packed image offset is not a whole number of fragments");
+  }
+  // one fragment per thread on the largest image up to 128 workgroups per
+  // image: every workgroup decodes its image's table entries first, so a
+  // larger grid costs more than it hides
+  const int64_t nfx = (mx / epl + 255) / 256;
+  unsigned gx = (unsigned)(nfx < 128 ? nfx : 128);
   dim3 grid(gx < 1 ? 1 : gx, (unsigned)((tab.n_img + 1) * dm.L));
   hipStream_t st = (hipStream_t)stream;
   if (fl->precision == FMC_PREC_BF16)
@@ -875,7 +913,8 @@ static int adamw_clip(void* stream, float* theta, const float* grad, float* m, f
   FMC_ARG(((uintptr_t)theta | (uintptr_t)grad | (uintptr_t)m | (uintptr_t)v) % 16 == 0,
           "theta, grad, m and v must be 16-byte aligned");
   if (n == 0) return FMC_OK;
-  int nb = (int)((n + 255) / 256);
+  int nb = (int)((n / 4 + 255) / 256);
+  if (nb < 1) nb = 1;
   if (nb > workspace_len - 1) nb = workspace_len - 1;
   if (nb > 1024) nb = 1024;
   hipStream_t st = (hipStream_t)stream;
@@ -887,10 +926,10 @@ static int adamw_clip(void* stream, float* theta, const float* grad, float* m, f
   const int t = step_dev ? 1 : step;  // (the kernel recomputes the corrections from step_dev)
   AdamHyper h{lr, b1, b2, eps, weight_decay, max_norm, (float)(1.0 - ipow_d((double)b1, t)),
               (float)(1.0 - ipow_d((double)b2, t))};
-  int nb2 = (int)((n / 4 + 255) / 256);
-  if (nb2 > 2048) nb2 = 2048;
+  int64_t nb2 = (n / 4 + 255) / 256;  // one float4 per thread up to 2M parameters
+  if (nb2 > 8192) nb2 = 8192;
   if (nb2 < 1) nb2 = 1;
-  hipLaunchKernelGGL(adamw_kernel, dim3(nb2), dim3(256), 0, st, theta, grad, m, v, n, h, workspace + nb, step_dev);
+  hipLaunchKernelGGL(adamw_kernel, dim3((unsigned)nb2), dim3(256), 0, st, theta, grad, m, v, n, h, workspace + nb, step_dev);
   return check_launch("adamw update");
 }
 
