@@ -225,27 +225,45 @@ __global__ void hmc_accept_kernel(float* x, float* logp, const float* __restrict
 // out[r] = 1 iff every entry of row r is finite (jnp.isfinite(data).all(-1)
 // in TrainModel, src/flowMC/strategy/train_model.py:66-80). A row is read by
 // g lanes of one wavefront (g a power of two), float4-wide when vec is set;
-// one ballot per wavefront combines the lanes of each row.
+// one ballot per wavefront combines the lanes of each row. The grid strides
+// over passes of 256 / g rows, FR_UNROLL passes per iteration, so every lane
+// has that many loads in flight (the buffer is the whole sample history: a
+// streaming read of gigabytes).
+constexpr int FR_UNROLL = 4;
 __global__ void __launch_bounds__(256) finite_rows_kernel(const float* __restrict__ x, int64_t n_rows, int d,
                                                           int g, int vec, int32_t* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   const int sub = lane & (g - 1);
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / g;
-  bool ok = true;
-  if (row < n_rows) {
-    const float* p = x + row * d;
-    if (vec) {
-      for (int j = sub * 4; j < d; j += g * 4) {
-        const float4 v = *reinterpret_cast<const float4*>(p + j);
-        ok = ok && isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
+  const int rpb = 256 / g;  // rows per workgroup pass
+  const int64_t stride = (int64_t)gridDim.x * rpb;
+  const unsigned long long grp = (g == 64) ? ~0ull : (((1ull << g) - 1ull) << (lane & ~(g - 1)));
+  // base is uniform over the workgroup, so every wavefront reaches each ballot
+  for (int64_t base = (int64_t)blockIdx.x * rpb; base < n_rows; base += FR_UNROLL * stride) {
+    bool ok[FR_UNROLL];
+#pragma unroll
+    for (int u = 0; u < FR_UNROLL; ++u) {
+      const int64_t row = base + u * stride + threadIdx.x / g;
+      bool o = true;
+      if (row < n_rows) {
+        const float* p = x + row * d;
+        if (vec) {
+          for (int j = sub * 4; j < d; j += g * 4) {
+            const float4 v = *reinterpret_cast<const float4*>(p + j);
+            o &= isfinite(v.x) & isfinite(v.y) & isfinite(v.z) & isfinite(v.w);
+          }
+        } else {
+          for (int j = sub; j < d; j += g) o &= isfinite(p[j]);
+        }
       }
-    } else {
-      for (int j = sub; j < d; j += g) ok = ok && isfinite(p[j]);
+      ok[u] = o;
+    }
+#pragma unroll
+    for (int u = 0; u < FR_UNROLL; ++u) {
+      const int64_t row = base + u * stride + threadIdx.x / g;
+      const unsigned long long bad = __ballot(!ok[u]);
+      if (sub == 0 && row < n_rows) out[row] = (bad & grp) ? 0 : 1;
     }
   }
-  const unsigned long long bad = __ballot(!ok);
-  const unsigned long long grp = (g == 64) ? ~0ull : (((1ull << g) - 1ull) << (lane & ~(g - 1)));
-  if (sub == 0 && row < n_rows) out[row] = (bad & grp) ? 0 : 1;
 }
 
 // --------------------------------------------------- tempering exchange ---
@@ -481,8 +499,9 @@ extern "C" int fmc_finite_rows(void* stream, const float* data, int64_t n_rows, 
   const int units = vec ? n_dim / 4 : n_dim;
   int g = 1;
   while (g < units && g < 64) g *= 2;
-  const int64_t grid = (n_rows * g + 255) / 256;
-  FMC_ARG(grid <= 0x7fffffffLL, "too many rows for one launch");
+  // eight workgroups per CU on 256 CUs hold the whole grid at once
+  int64_t grid = (n_rows * g + 255) / 256;
+  if (grid > 2048) grid = 2048;
   hipLaunchKernelGGL(finite_rows_kernel, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, data, n_rows,
                      n_dim, g, vec, out);
   return check_launch("finite rows");

@@ -40,6 +40,23 @@ def test_finite_rows_kernel(gpu, d):
         assert torch.equal(out.bool(), torch.isfinite(view).all(-1)), d
 
 
+@pytest.mark.parametrize("d", [5, 64])
+def test_finite_rows_kernel_many_passes(gpu, d):
+    """More rows than one launch covers in a pass (2048 workgroups x 256 / g
+    rows, four passes per loop iteration), with a ragged last iteration."""
+    n = 2048 * 64 * 4 * 3 + 1237 if d == 64 else 2048 * 32 * 4 + 999
+    g = torch.Generator(device="cuda").manual_seed(100 + d)
+    x = torch.randn(n, d, device="cuda", generator=g)
+    bad_rows = torch.randint(0, n, (500,), device="cuda", generator=g)
+    bad_cols = torch.randint(0, d, (500,), device="cuda", generator=g)
+    x[bad_rows, bad_cols] = float("nan")
+    x[-1, d - 1] = float("inf")
+    out = torch.full((n,), 7, dtype=torch.int32, device="cuda")
+    N.call("fmc_finite_rows", N.stream_ptr(), x.data_ptr(), n, d, out.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(out.bool(), torch.isfinite(x).all(-1))
+
+
 def _buffer(C_, T, d, filled, seed=0):
     g = torch.Generator(device="cuda").manual_seed(seed)
     buf = torch.full((C_, T, d), -float("inf"), device="cuda")
