@@ -449,7 +449,20 @@ class MaskedCouplingRQSpline(NFModel):
         self._dirty = True
         return loss, self, state
 
-    def _train_body(self, xb, ws, optim, state, step_dev):
+    def _train_step_no_loss(self, x, optim, state):
+        """train_step without the loss reduction: the same parameter update,
+        for the steps of an epoch whose loss nobody reads."""
+        if self.graph_train and not self._base_learnable:
+            _, model, state = self.train_step(x, optim, state)
+            return model, state
+        xb, _ = self._as_batch(x)
+        ws = self._workspace(xb.shape[0])
+        self._ensure_packed()
+        self._train_body(xb, ws, optim, state, None, want_loss=False)
+        self._dirty = True
+        return self, state
+
+    def _train_body(self, xb, ws, optim, state, step_dev, want_loss=True):
         """Forward, loss, adjoint, clip + AdamW of one batch; with step_dev the
         AdamW step count is read from device memory (the graph body)."""
         B = xb.shape[0]
@@ -457,7 +470,9 @@ class MaskedCouplingRQSpline(NFModel):
         _native.call("fmc_flow_pass", _native.stream_ptr(), C.byref(s), FORWARD, STANDARDIZE | BASE_LOGP,
                      xb.data_ptr(), B, 0, 0, 0, None, ws.logp.data_ptr(), ws.saved.data_ptr(), ws.g_logp.data_ptr(),
                      ws.gy.data_ptr())
-        if step_dev is None:
+        if not want_loss:
+            loss = None
+        elif step_dev is None:
             loss = -ws.logp.mean()
         else:  # in place, into the graph's loss buffer
             loss = torch.mean(ws.logp, 0, out=ws.loss_static).neg_()

@@ -432,6 +432,33 @@ def test_train_keeps_the_first_lowest_loss_epoch(gpu):
     assert torch.equal(st.mu, ref_best[1].mu) and torch.equal(st.nu, ref_best[1].nu)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_train_epoch_matches_a_loop_of_train_steps(gpu, graph):
+    """train_epoch skips the loss reduction on all but its last minibatch
+    (base.py:127-151 returns only that loss); parameters, Adam moments, step
+    count and the returned loss equal a plain loop of train_step over the
+    same permutation, bit for bit."""
+    from flowmc_amd.resource.model.nf_model.base import NFModel
+
+    d, bs = 4, 500
+    g = torch.Generator().manual_seed(23)
+    data = (torch.randn(2600, d, generator=g) * 1.3).cuda()
+    m = MaskedCouplingRQSpline(d, 3, [32, 32], 8, fr.PRNGKey(8))
+    m.graph_train = graph
+    opt = Optimizer(m, learning_rate=1e-2)
+    a, sa = m.clone(), opt.optim_state.clone()
+    value, a, sa = NFModel.train_epoch(a, fr.PRNGKey(9), opt.optim, sa, data, bs)
+    b, sb = m.clone(), opt.optim_state.clone()
+    steps = data.shape[0] // bs
+    perms = fr.permutation(fr.PRNGKey(9), data.shape[0], device=data.device)[: steps * bs].reshape(steps, bs)
+    for perm in perms:
+        vb, b, sb = b.train_step(data[perm], opt.optim, sb)
+    assert sa.count == sb.count == steps
+    assert torch.equal(value, vb)
+    assert torch.equal(a.theta, b.theta)
+    assert torch.equal(sa.mu, sb.mu) and torch.equal(sa.nu, sb.nu)
+
+
 def test_training_reduces_loss_and_best_model(gpu):
     d = 4
     g = torch.Generator().manual_seed(7)
