@@ -859,19 +859,30 @@ extern "C" int fmc_flow_weight_grad_ws_size(const fmc_flow* fl, int64_t store_ld
   if (rc) return rc;
   FMC_ARG(bytes != nullptr, "bytes is NULL");
   FMC_ARG(store_ld >= 64 && store_ld % 64 == 0, "store_ld must be a positive multiple of 64");
+  // the split sized for the whole chip (the last layer of a backward sweep,
+  // when nothing runs beside it) needs the most
   int64_t mx = 0;
   for (int l = 0; l < (fd.dm.L < 2 ? fd.dm.L : 2); ++l) {  // layers alternate between two shapes
-    const WsTable t = wsplit_table(fd.dm, l, store_ld, wsplit_max_wg());
-    const int64_t b = (int64_t)t.n_blocks * t.S * WS_PART * (int64_t)sizeof(float);
-    mx = b > mx ? b : mx;
+    for (int w : {wsplit_max_wg(), device_cus()}) {
+      const WsTable t = wsplit_table(fd.dm, l, store_ld, w);
+      const int64_t b = (int64_t)t.n_blocks * t.S * WS_PART * (int64_t)sizeof(float);
+      mx = b > mx ? b : mx;
+    }
   }
   *bytes = mx;
   return FMC_OK;
 }
 
-extern "C" int fmc_flow_weight_grad_split(void* stream, const fmc_flow* fl, int32_t layer, const void* store,
-                                          int64_t store_ld, const float* affine_partials, int32_t n_partials,
-                                          float* grad, float* workspace, int64_t workspace_bytes) {
+static int64_t wsplit_bytes(const FlowDims& dm, int layer, int64_t store_ld, int max_wg) {
+  const WsTable t = wsplit_table(dm, layer, store_ld, max_wg);
+  return (int64_t)t.n_blocks * t.S * WS_PART * (int64_t)sizeof(float);
+}
+
+// The split weight gradient with its batch ranges sized for max_wg
+// workgroups in all (half the chip beside an adjoint, all of it alone).
+static int weight_grad_split(void* stream, const fmc_flow* fl, int32_t layer, const void* store, int64_t store_ld,
+                             const float* affine_partials, int32_t n_partials, float* grad, float* workspace,
+                             int64_t workspace_bytes, int max_wg) {
   FlowDev fd;
   int rc = check_flow(fl, fd);
   if (rc) return rc;
@@ -880,7 +891,7 @@ extern "C" int fmc_flow_weight_grad_split(void* stream, const fmc_flow* fl, int3
   FMC_ARG(store_ld >= 64 && store_ld % 64 == 0, "store_ld must be a positive multiple of 64");
   FMC_ARG(n_partials >= store_ld / 32, "affine_partials holds %d slots, need %lld", n_partials,
           (long long)(store_ld / 32));
-  const WsTable tab = wsplit_table(fd.dm, layer, store_ld, wsplit_max_wg());
+  const WsTable tab = wsplit_table(fd.dm, layer, store_ld, max_wg);
   const int64_t need = (int64_t)tab.n_blocks * tab.S * WS_PART * (int64_t)sizeof(float);
   FMC_ARG(workspace_bytes >= need, "workspace holds %lld bytes, need %lld (fmc_flow_weight_grad_ws_size)",
           (long long)workspace_bytes, (long long)need);
@@ -902,6 +913,13 @@ extern "C" int fmc_flow_weight_grad_split(void* stream, const fmc_flow* fl, int3
   hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((unsigned)((ne + 255) / 256 + 1)), dim3(256), 0, st, fd, layer,
                      tab, workspace, affine_partials, (int)(store_ld / 32), grad);
   return check_launch("flow weight grad (reduce)");
+}
+
+extern "C" int fmc_flow_weight_grad_split(void* stream, const fmc_flow* fl, int32_t layer, const void* store,
+                                          int64_t store_ld, const float* affine_partials, int32_t n_partials,
+                                          float* grad, float* workspace, int64_t workspace_bytes) {
+  return weight_grad_split(stream, fl, layer, store, store_ld, affine_partials, n_partials, grad, workspace,
+                           workspace_bytes, wsplit_max_wg());
 }
 
 static int adamw_clip(void* stream, float* theta, const float* grad, float* m, float* v, int64_t n, float lr,
@@ -966,6 +984,8 @@ extern "C" int fmc_flow_backward(void* stream, void* side_stream, const fmc_flow
   FMC_ARG(!two || (store1 != nullptr && aff1 != nullptr), "the side-stream schedule needs store1 and aff1");
   FMC_ARG(!split || wg_workspace != nullptr, "the split weight gradient needs its workspace");
   FMC_ARG(fl->n_layers >= 1 && n >= 1, "bad n_layers / n");
+  FlowDev fd;
+  if (const int rcf = check_flow(fl, fd)) return rcf;
   hipStream_t main_st = (hipStream_t)stream, side = two ? (hipStream_t)side_stream : main_st;
   auto hip_fail = [](const char* what) {
     set_error(FMC_ERR_HIP, "%s failed", what);
@@ -1001,9 +1021,17 @@ extern "C" int fmc_flow_backward(void* stream, void* side_stream, const fmc_flow
       rc = hip_fail("hipEventRecord / hipStreamWaitEvent");
       break;
     }
-    rc = split ? fmc_flow_weight_grad_split(side, fl, layer, st, store_ld, af, n_partials, grad, wg_workspace,
-                                            wg_workspace_bytes)
-               : fmc_flow_weight_grad(side, fl, layer, st, store_ld, af, n_partials, grad);
+    if (split) {
+      // the last layer's weight gradient runs alone: batch ranges for the
+      // whole chip when the workspace holds them (the same choice on one
+      // stream or two, so the schedule does not change the sums)
+      int max_wg = wsplit_max_wg();
+      if (layer == 0 && wsplit_bytes(fd.dm, 0, store_ld, device_cus()) <= wg_workspace_bytes) max_wg = device_cus();
+      rc = weight_grad_split(side, fl, layer, st, store_ld, af, n_partials, grad, wg_workspace, wg_workspace_bytes,
+                             max_wg);
+    } else {
+      rc = fmc_flow_weight_grad(side, fl, layer, st, store_ld, af, n_partials, grad);
+    }
     if (rc) break;
     if (two) {
       if (hipEventRecord(ev_wg[b], side) != hipSuccess) {
