@@ -482,8 +482,11 @@ int fmc_flow_backward_layer(void* stream, const fmc_flow* flow, int32_t layer,
  * side_stream != NULL the weight gradient of each layer runs there beside
  * the next adjoint, on store0 / aff0 and store1 / aff1 in turn, and
  * `stream` joins side_stream before the call returns; NULL runs everything
- * on `stream` with store0 / aff0. Replaces the per-layer launches of the
- * gradient of NFModel.loss_fn (nf_model/base.py:98-125) with one call. */
+ * on `stream` with store0 / aff0. The split weight gradient of layer 0,
+ * which nothing runs beside, spreads its batch ranges over the whole device
+ * when wg_workspace_bytes holds that (fmc_flow_weight_grad_ws_size reports
+ * the larger size). Replaces the per-layer launches of the gradient of
+ * NFModel.loss_fn (nf_model/base.py:98-125) with one call. */
 int fmc_flow_backward(void* stream, void* side_stream, const fmc_flow* flow, const float* saved, int64_t n,
                       const float* g_logp, float* gy, float* gx, void* store0, void* store1,
                       int64_t store_ld, float* aff0, float* aff1, int32_t n_partials, float* grad,
