@@ -975,6 +975,20 @@ extern "C" int fmc_adamw_clip_dev(void* stream, float* theta, const float* grad,
 // for the weight gradient that last read its store, and `stream` joins the
 // side stream before returning. The events are the call's own (created and
 // released here), so overlapping calls on other streams do not interfere.
+// p[0:n) = 0 (a kernel rather than a memset, so that a caller may record the
+// sweep into a HIP graph); float4 stores when p is 16-byte aligned
+__global__ void __launch_bounds__(256) zero_kernel(float* __restrict__ p, int64_t n, int vec) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x, stride = (int64_t)gridDim.x * 256;
+  int64_t i0 = t;
+  if (vec) {
+    float4* p4 = reinterpret_cast<float4*>(p);
+    const int64_t n4 = n / 4;
+    for (int64_t i = t; i < n4; i += stride) p4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    i0 = 4 * n4 + t;
+  }
+  for (int64_t i = i0; i < n; i += stride) p[i] = 0.f;
+}
+
 extern "C" int fmc_flow_backward(void* stream, void* side_stream, const fmc_flow* fl, const float* saved, int64_t n,
                                  const float* g_logp, float* gy, float* gx, void* store0, void* store1,
                                  int64_t store_ld, float* aff0, float* aff1, int32_t n_partials, float* grad,
@@ -1020,6 +1034,14 @@ extern "C" int fmc_flow_backward(void* stream, void* side_stream, const fmc_flow
     if (two && (hipEventRecord(ev_adj, main_st) != hipSuccess || hipStreamWaitEvent(side, ev_adj, 0) != hipSuccess)) {
       rc = hip_fail("hipEventRecord / hipStreamWaitEvent");
       break;
+    }
+    if (i == 0) {  // clear grad where the weight gradients are written, ahead of the first
+      const int64_t nz = fd.dm.th_total();
+      const int vec = ((uintptr_t)grad % 16) == 0;
+      const int64_t nzb = ((vec ? nz / 4 : nz) + 255) / 256;
+      hipLaunchKernelGGL(zero_kernel, dim3((unsigned)(nzb < 1 ? 1 : nzb > 2048 ? 2048 : nzb)), dim3(256), 0, side, grad,
+                         nz, vec);
+      if ((rc = check_launch("gradient clear"))) break;
     }
     if (split) {
       // the last layer's weight gradient runs alone: batch ranges for the

@@ -79,15 +79,16 @@ class NFModel(Resource):
         steps = n // batch_size
         if steps > 0:
             perms = frandom.permutation(rng, n, device=data.device)[: steps * batch_size].reshape(steps, batch_size)
+            batches = data[perms]  # the epoch's minibatches in one gather
             # only the last step's loss is returned: the steps before it skip
             # the loss reduction where the model offers that
             quiet = getattr(model, "_train_step_no_loss", None)
-            for i, perm in enumerate(perms):
+            for i in range(steps):
                 if quiet is not None and i + 1 < steps:
-                    model, state = quiet(data[perm], optim, state)
+                    model, state = quiet(batches[i], optim, state)
                     quiet = getattr(model, "_train_step_no_loss", None)
                 else:
-                    value, model, state = model.train_step(data[perm], optim, state)
+                    value, model, state = model.train_step(batches[i], optim, state)
         else:
             value, model, state = model.train_step(data, optim, state)
         return value, model, state

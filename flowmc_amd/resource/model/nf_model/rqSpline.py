@@ -413,11 +413,11 @@ class MaskedCouplingRQSpline(NFModel):
         (it only reads its layer's store and affine partials) beside the next
         adjoint on the current stream; the stores alternate, an adjoint waits
         until the weight gradient that last read its store has finished, and
-        the current stream joins the side stream before returning. Results
-        do not depend on the schedule."""
+        the current stream joins the side stream before returning. The call
+        clears ws.grad itself (beside the first adjoint when overlapping).
+        Results do not depend on the schedule."""
         s = self._struct()
         st = torch.cuda.current_stream(self.device).cuda_stream
-        ws.grad.zero_()
         split = self.wgrad_split if self.wgrad_split is not None else True
         two = ws.overlap and len(ws.store) > 1
         _native.call("fmc_flow_backward", st, ws.side.cuda_stream if two else None, C.byref(s), ws.saved.data_ptr(),

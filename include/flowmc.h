@@ -478,7 +478,10 @@ int fmc_flow_backward_layer(void* stream, const fmc_flow* flow, int32_t layer,
  * saved layer inputs [n_layers, n, d], gy -> gx, the two swapping between
  * layers: the input gradient ends in gy for an even layer count, else gx),
  * then that layer's weight gradient into grad (fmc_flow_weight_grad_split
- * with wg_workspace when split != 0, else fmc_flow_weight_grad). With
+ * with wg_workspace when split != 0, else fmc_flow_weight_grad). grad
+ * (theta_size floats) is cleared by the call first, in stream order ahead of
+ * the first weight gradient, so entries no layer writes (the masked input
+ * columns, the data and base statistics) read zero. With
  * side_stream != NULL the weight gradient of each layer runs there beside
  * the next adjoint, on store0 / aff0 and store1 / aff1 in turn, and
  * `stream` joins side_stream before the call returns; NULL runs everything

@@ -294,6 +294,27 @@ def test_flow_gradient_independent_of_the_backward_schedule(gpu, hidden, split):
         type(m).wgrad_split = None
 
 
+@pytest.mark.parametrize("split", [False, True])
+def test_flow_backward_clears_the_gradient(gpu, split):
+    """fmc_flow_backward clears grad before the weight gradients: a
+    workspace gradient left holding NaN gives the same bits as a fresh one
+    (masked input columns and the data statistics are never written)."""
+    d, L, K, B = 8, 3, 6, 700
+    m, _ = make(d, L, K, (32, 32), "bfloat16", seed=9)
+    type(m).wgrad_split = split
+    try:
+        g = torch.Generator().manual_seed(10)
+        x = (torch.randn(B, d, generator=g) * 2).float().cuda()
+        w = (torch.randn(B, generator=g) / B).float().cuda()
+        g1, gx1 = m.grad_log_prob_sum(x, w)
+        m._ws[B].grad.fill_(float("nan"))
+        g2, gx2 = m.grad_log_prob_sum(x, w)
+        assert not torch.isnan(g2).any()
+        assert torch.equal(g1, g2) and torch.equal(gx1, gx2)
+    finally:
+        type(m).wgrad_split = None
+
+
 def _schedule_bits(m, d, L, hidden):
     g = torch.Generator().manual_seed(7)
     B = 10000
