@@ -99,8 +99,10 @@ class NFModel(Resource):
         if not data.is_cuda:
             data = data.cuda()
         state0 = state
-        model = self.clone()
-        state = state.clone()
+        if hasattr(self, "_training_copy"):  # working buffers reused from call to call
+            model, state = self._training_copy(state)
+        else:
+            model, state = self.clone(), state.clone()
         # jnp.cov(data.T) (ddof 1) without torch.cov, whose degrees-of-freedom
         # check reads a device scalar back (a host sync before every training)
         n = data.shape[0]

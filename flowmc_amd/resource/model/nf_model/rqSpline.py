@@ -113,6 +113,7 @@ class MaskedCouplingRQSpline(NFModel):
         self._configure()
         self._dirty = True
         self._ws: dict[int, _Workspace] = {}
+        self._slots: dict = {}  # train()'s working copies, shared with clones like _ws
 
     # ------------------------------------------------------------ layout --
     def _configure(self):
@@ -251,6 +252,31 @@ class MaskedCouplingRQSpline(NFModel):
         m._ws = self._ws
         return m
 
+    def _training_copy(self, state):
+        """The (model, optimiser state) pair that train() updates: the
+        clone and state copy of nf_model/base.py:153-210, kept with the shared
+        workspaces (one pair per precision and base kind) and refilled from
+        self and state on every call. Every train call of this flow and its
+        clones then updates the same parameter and moment buffers, so a
+        recorded training step (graph_train) stays valid from one call to
+        the next. The pair never leaves train(): it returns a fresh clone of
+        the best epoch."""
+        key = (self.precision, self._base_learnable, self._base_dense)
+        slot = self._slots.get(key)
+        if slot is None or slot[1].mu.shape != state.mu.shape:
+            slot = self._slots[key] = (self.clone(), state.clone())
+            return slot
+        m, st = slot
+        own = {k: m.__dict__[k] for k in ("theta", "packed", "packed_bias")}
+        m.__dict__.update({k: v for k, v in self.__dict__.items() if k not in own})
+        m.__dict__.update(own)
+        m.theta.copy_(self.theta)
+        m._dirty = True
+        st.mu.copy_(state.mu)
+        st.nu.copy_(state.nu)
+        st.count = state.count
+        return slot
+
     def set_data_stats(self, mean, cov):
         self.data_mean.copy_(mean)
         self.data_cov.copy_(cov)
@@ -265,6 +291,7 @@ class MaskedCouplingRQSpline(NFModel):
         m.precision = "bfloat16" if _PREC[p] == 1 else "float32"
         m._configure()
         m._ws = {}  # the stores hold the conditioner's precision
+        m._slots = {}
         return m
 
     # -------------------------------------------------------------- kernels
@@ -430,12 +457,11 @@ class MaskedCouplingRQSpline(NFModel):
     # state and hyperparameters are recorded once as a HIP graph (pack,
     # forward, the two-stream adjoint / weight-gradient schedule, clip +
     # AdamW with the step count in device memory) and replayed, bit for bit
-    # the eager steps. A recording costs ~2 ms and saves ~16-30 us a step, so
-    # it pays for long training on one model (batch 10k, d=32: 0.510 -> 0.492
-    # ms per step); the NF-MCMC loop trains a fresh clone for ~30 steps a
-    # round, where it lost (0.881 vs 0.909 s for the 64-D run), so it is off
-    # by default.
-    graph_train = False
+    # the eager steps. A recording costs ~2 ms and saves ~16-30 us a step
+    # (batch 10k, d=32: 0.510 -> 0.492 ms per step). train() updates the
+    # same working buffers on every call (_training_copy), so one recording
+    # serves every training round of a sampling run.
+    graph_train = True
 
     def train_step(self, x, optim, state):
         """nf_model/base.py:102-125: loss, hand-written gradient, clip + AdamW (in place)."""
@@ -546,6 +572,7 @@ class MaskedCouplingRQSpline(NFModel):
         m._configure()  # the packed-bias image holds the Cholesky factor only on the full-covariance path
         m._dirty = True
         m._ws = {}
+        m._slots = {}
         return m
 
     def print_parameters(self):

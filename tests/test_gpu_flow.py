@@ -480,6 +480,36 @@ def test_train_epoch_matches_a_loop_of_train_steps(gpu, graph):
     assert torch.equal(sa.mu, sb.mu) and torch.equal(sa.nu, sb.nu)
 
 
+def test_train_rounds_recorded_match_eager(gpu):
+    """Three train() calls in a row, each on the previous call's best model
+    (a sampling run's training rounds): with recorded steps (one recording
+    reused across the calls through the shared working copy) and eagerly,
+    the same losses, parameters and Adam moments bit for bit; the caller's
+    models are left unchanged."""
+    d = 4
+    g = torch.Generator().manual_seed(31)
+    data = (torch.randn(2600, d, generator=g) * 1.2).cuda()
+    out = {}
+    for graph in (False, True):
+        m = MaskedCouplingRQSpline(d, 3, [32, 32], 8, fr.PRNGKey(12))
+        m.graph_train = graph
+        opt = Optimizer(m, learning_rate=1e-2)
+        model, state, key, hist = m, opt.optim_state, fr.PRNGKey(13), []
+        for _ in range(3):
+            before = model.theta.clone()
+            key, model2, state, losses = model.train(key, data, opt.optim, state, 3, 500, verbose=False)
+            assert torch.equal(model.theta, before)
+            assert model2 is not model or bool((losses >= 1e9).all())
+            model = model2
+            hist.append(losses)
+        if graph:
+            assert len(m._slots) == 1 and getattr(m._ws[500], "graph", None) is not None
+        out[graph] = (torch.stack(hist), model.theta.clone(), state.mu.clone(), state.nu.clone(), state.count)
+    a, b = out[False], out[True]
+    assert torch.equal(a[0], b[0])
+    assert torch.equal(a[1], b[1]) and torch.equal(a[2], b[2]) and torch.equal(a[3], b[3]) and a[4] == b[4]
+
+
 def test_training_reduces_loss_and_best_model(gpu):
     d = 4
     g = torch.Generator().manual_seed(7)
