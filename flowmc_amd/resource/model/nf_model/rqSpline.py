@@ -478,11 +478,11 @@ class MaskedCouplingRQSpline(NFModel):
     def _train_step_no_loss(self, x, optim, state):
         """train_step without the loss reduction: the same parameter update,
         for the steps of an epoch whose loss nobody reads."""
-        if self.graph_train and not self._base_learnable:
-            _, model, state = self.train_step(x, optim, state)
-            return model, state
         xb, _ = self._as_batch(x)
         ws = self._workspace(xb.shape[0])
+        if self.graph_train and not self._base_learnable:
+            _, model, state = self._train_step_graph(xb, ws, optim, state, want_loss=False)
+            return model, state
         self._ensure_packed()
         self._train_body(xb, ws, optim, state, None, want_loss=False)
         self._dirty = True
@@ -490,18 +490,14 @@ class MaskedCouplingRQSpline(NFModel):
 
     def _train_body(self, xb, ws, optim, state, step_dev, want_loss=True):
         """Forward, loss, adjoint, clip + AdamW of one batch; with step_dev the
-        AdamW step count is read from device memory (the graph body)."""
+        AdamW step count is read from device memory (the graph body, which
+        leaves the loss to the caller)."""
         B = xb.shape[0]
         s = self._struct()
         _native.call("fmc_flow_pass", _native.stream_ptr(), C.byref(s), FORWARD, STANDARDIZE | BASE_LOGP,
                      xb.data_ptr(), B, 0, 0, 0, None, ws.logp.data_ptr(), ws.saved.data_ptr(), ws.g_logp.data_ptr(),
                      ws.gy.data_ptr())
-        if not want_loss:
-            loss = None
-        elif step_dev is None:
-            loss = -ws.logp.mean()
-        else:  # in place, into the graph's loss buffer
-            loss = torch.mean(ws.logp, 0, out=ws.loss_static).neg_()
+        loss = -ws.logp.mean() if want_loss else None
         base_g = self._base_grad(ws) if self._base_learnable else None
         self._backward(ws, B)
         self._add_base_grad(ws, base_g)
@@ -519,7 +515,10 @@ class MaskedCouplingRQSpline(NFModel):
                          ws.norm_ws.data_ptr(), 1024, ws.gnorm.data_ptr())
         return loss
 
-    def _train_step_graph(self, xb, ws, optim, state):
+    def _train_step_graph(self, xb, ws, optim, state, want_loss=True):
+        """One replay of the recorded step; the loss (read from the forward's
+        log-densities, which the rest of the step leaves alone) only when
+        asked for."""
         B = xb.shape[0]
         # every address and value the recorded launches carry: a replay is
         # only valid for the same buffers (the workspace is shared by clones)
@@ -530,20 +529,19 @@ class MaskedCouplingRQSpline(NFModel):
             ws.graph = None
             ws.x_static = torch.empty((B, self._n_features), dtype=torch.float32, device=self.device)
             ws.step_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
-            ws.loss_static = torch.zeros((), dtype=torch.float32, device=self.device)
             self._ensure_packed()  # (recording runs nothing: theta stays as it is)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 s = self._struct()
                 _native.call("fmc_flow_pack", _native.stream_ptr(), C.byref(s))
-                self._train_body(ws.x_static, ws, optim, state, ws.step_dev)
+                self._train_body(ws.x_static, ws, optim, state, ws.step_dev, want_loss=False)
             ws.graph, ws.graph_key = g, key
         ws.x_static.copy_(xb)
         state.count += 1
         ws.step_dev.fill_(state.count)
         ws.graph.replay()
         self._dirty = True
-        return ws.loss_static.clone(), self, state
+        return (-ws.logp.mean() if want_loss else None), self, state
 
     # --------------------------------------------------------------- I/O --
     def _config(self):
