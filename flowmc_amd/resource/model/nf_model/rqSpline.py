@@ -529,17 +529,21 @@ class MaskedCouplingRQSpline(NFModel):
             ws.graph = None
             ws.x_static = torch.empty((B, self._n_features), dtype=torch.float32, device=self.device)
             ws.step_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
+            ws.step_next = None  # the count step_dev holds for the next replay
             self._ensure_packed()  # (recording runs nothing: theta stays as it is)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 s = self._struct()
                 _native.call("fmc_flow_pack", _native.stream_ptr(), C.byref(s))
                 self._train_body(ws.x_static, ws, optim, state, ws.step_dev, want_loss=False)
+                ws.step_dev.add_(1)  # the next replay's count, without a launch outside the graph
             ws.graph, ws.graph_key = g, key
         ws.x_static.copy_(xb)
         state.count += 1
-        ws.step_dev.fill_(state.count)
+        if ws.step_next != state.count:  # first replay, or a state whose count moved elsewhere
+            ws.step_dev.fill_(state.count)
         ws.graph.replay()
+        ws.step_next = state.count + 1
         self._dirty = True
         return (-ws.logp.mean() if want_loss else None), self, state
 
