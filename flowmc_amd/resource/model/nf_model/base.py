@@ -21,6 +21,26 @@ from .... import random as frandom
 from ...base import Resource
 
 
+def data_stats(data: torch.Tensor):
+    """jnp.mean(data, axis=0) and jnp.cov(data.T) (ddof 1) of train()
+    (base.py:187-188) without torch.cov, whose degrees-of-freedom check reads
+    a device scalar back (a host sync before every training). The rows are
+    split into up to 256 chunks whose d x d products run as one batched GEMM
+    and are then summed: a single GEMM over 100k rows ran on a handful of
+    workgroups (285 us for d = 64)."""
+    n, d = data.shape
+    mean = data.mean(0)
+    xc = data - mean
+    p = max(1, min(256, n // 256))
+    c = n // p
+    head = xc[: p * c].reshape(p, c, d)
+    cov = torch.bmm(head.transpose(1, 2).contiguous(), head).sum(0)
+    if p * c < n:
+        tail = xc[p * c:]
+        cov = cov + tail.T.contiguous() @ tail
+    return mean, cov / max(n - 1, 1)
+
+
 class NFModel(Resource):
     @property
     def n_features(self):
@@ -103,11 +123,8 @@ class NFModel(Resource):
             model, state = self._training_copy(state)
         else:
             model, state = self.clone(), state.clone()
-        # jnp.cov(data.T) (ddof 1) without torch.cov, whose degrees-of-freedom
-        # check reads a device scalar back (a host sync before every training)
-        n = data.shape[0]
-        xc = data.T - data.T.mean(1, keepdim=True)
-        model.set_data_stats(data.mean(0), ((xc @ xc.T) / max(n - 1, 1)).reshape(self.n_features, self.n_features))
+        mean, cov = data_stats(data)
+        model.set_data_stats(mean, cov.reshape(self.n_features, self.n_features))
         it = range(num_epochs)
         if verbose:
             try:

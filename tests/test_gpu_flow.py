@@ -424,7 +424,7 @@ def test_train_keeps_the_first_lowest_loss_epoch(gpu):
     read-back of every epoch's loss and a clone at each improvement, as the
     reference writes it, returns the same parameters, Adam moments and count
     and the same loss values."""
-    from flowmc_amd.resource.model.nf_model.base import NFModel
+    from flowmc_amd.resource.model.nf_model.base import NFModel, data_stats
 
     d = 4
     g = torch.Generator().manual_seed(17)
@@ -434,11 +434,10 @@ def test_train_keeps_the_first_lowest_loss_epoch(gpu):
     rng, best, st, losses = m.train(fr.PRNGKey(5), data, opt.optim, opt.optim_state, 6, 500, verbose=False)
     # the reference's loop
     model, state = m.clone(), opt.optim_state.clone()
-    n = data.shape[0]
-    xc = data.T - data.T.mean(1, keepdim=True)
-    cov = (xc @ xc.T) / (n - 1)  # train's jnp.cov (ddof 1), as torch.cov gives it
+    mean, cov = data_stats(data)  # train's jnp.mean / jnp.cov (ddof 1), as torch gives them
     assert torch.allclose(cov, torch.cov(data.T), rtol=1e-5, atol=1e-6)
-    model.set_data_stats(data.mean(0), cov)
+    assert torch.allclose(mean, data.mean(0), rtol=0, atol=0)
+    model.set_data_stats(mean, cov)
     key, ref_best, ref_loss, ref_losses = fr.PRNGKey(5), None, 1e9, []
     for _ in range(6):
         key, sub = fr.split(key)

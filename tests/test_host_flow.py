@@ -125,3 +125,21 @@ def test_realnvp_refuses_a_learnable_base():
 
     with pytest.raises(NotImplementedError, match="learnable"):
         RealNVP(4, 2, 16, fr.PRNGKey(0), base_dist=Gaussian(torch.zeros(4), torch.eye(4), learnable=True))
+
+
+@pytest.mark.parametrize("n", [2, 255, 3001, 70000])
+def test_data_stats_match_numpy(n):
+    """train()'s data mean and covariance (jnp.mean, jnp.cov with ddof 1,
+    nf_model/base.py:187-188): chunked batched products plus the ragged tail
+    against numpy's FP64 np.cov."""
+    import numpy as np
+
+    from flowmc_amd.resource.model.nf_model.base import data_stats
+
+    g = torch.Generator().manual_seed(n)
+    x = torch.randn(n, 5, generator=g) * torch.tensor([1.0, 2.0, 3.0, 0.5, 4.0]) + 3.0
+    x[:, 1] += 0.7 * x[:, 0]  # correlated columns: off-diagonal terms
+    mean, cov = data_stats(x)
+    xd = x.double().numpy()
+    np.testing.assert_allclose(mean.numpy(), xd.mean(0), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(cov.numpy(), np.cov(xd.T), rtol=1e-4, atol=1e-4)
