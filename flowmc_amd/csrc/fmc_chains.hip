@@ -228,8 +228,14 @@ __global__ void hmc_accept_kernel(float* x, float* logp, const float* __restrict
 // one ballot per wavefront combines the lanes of each row. The grid strides
 // over passes of 256 / g rows, FR_UNROLL passes per iteration, so every lane
 // has that many loads in flight (the buffer is the whole sample history: a
-// streaming read of gigabytes).
+// streaming read of gigabytes). On the float4 path the first lane of each
+// row reads the row's first four entries ahead of the others: a row that
+// fails there (the unfilled rows of a sample buffer hold -inf) is decided
+// without reading the rest of it.
 constexpr int FR_UNROLL = 4;
+__device__ __forceinline__ bool finite4(const float4 v) {
+  return isfinite(v.x) & isfinite(v.y) & isfinite(v.z) & isfinite(v.w);
+}
 __global__ void __launch_bounds__(256) finite_rows_kernel(const float* __restrict__ x, int64_t n_rows, int d,
                                                           int g, int vec, int32_t* __restrict__ out) {
   const int lane = threadIdx.x & 63;
@@ -237,25 +243,41 @@ __global__ void __launch_bounds__(256) finite_rows_kernel(const float* __restric
   const int rpb = 256 / g;  // rows per workgroup pass
   const int64_t stride = (int64_t)gridDim.x * rpb;
   const unsigned long long grp = (g == 64) ? ~0ull : (((1ull << g) - 1ull) << (lane & ~(g - 1)));
+  const int lead = lane & ~(g - 1);  // the row's first lane
   // base is uniform over the workgroup, so every wavefront reaches each ballot
   for (int64_t base = (int64_t)blockIdx.x * rpb; base < n_rows; base += FR_UNROLL * stride) {
     bool ok[FR_UNROLL];
+    if (vec) {
+      bool head[FR_UNROLL];
 #pragma unroll
-    for (int u = 0; u < FR_UNROLL; ++u) {
-      const int64_t row = base + u * stride + threadIdx.x / g;
-      bool o = true;
-      if (row < n_rows) {
-        const float* p = x + row * d;
-        if (vec) {
-          for (int j = sub * 4; j < d; j += g * 4) {
-            const float4 v = *reinterpret_cast<const float4*>(p + j);
-            o &= isfinite(v.x) & isfinite(v.y) & isfinite(v.z) & isfinite(v.w);
-          }
-        } else {
+      for (int u = 0; u < FR_UNROLL; ++u) {
+        const int64_t row = base + u * stride + threadIdx.x / g;
+        bool h = true;
+        if (sub == 0 && row < n_rows) h = finite4(*reinterpret_cast<const float4*>(x + row * d));
+        head[u] = h;
+      }
+#pragma unroll
+      for (int u = 0; u < FR_UNROLL; ++u) {
+        const int64_t row = base + u * stride + threadIdx.x / g;
+        const bool dead = (__ballot(!head[u]) >> lead) & 1ull;
+        bool o = !dead;
+        if (!dead && row < n_rows) {
+          const float* p = x + row * d;
+          for (int j = sub * 4; j < d; j += g * 4) o &= finite4(*reinterpret_cast<const float4*>(p + j));
+        }
+        ok[u] = o;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < FR_UNROLL; ++u) {
+        const int64_t row = base + u * stride + threadIdx.x / g;
+        bool o = true;
+        if (row < n_rows) {
+          const float* p = x + row * d;
           for (int j = sub; j < d; j += g) o &= isfinite(p[j]);
         }
+        ok[u] = o;
       }
-      ok[u] = o;
     }
 #pragma unroll
     for (int u = 0; u < FR_UNROLL; ++u) {

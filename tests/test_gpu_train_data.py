@@ -50,6 +50,8 @@ def test_finite_rows_kernel_many_passes(gpu, d):
     bad_rows = torch.randint(0, n, (500,), device="cuda", generator=g)
     bad_cols = torch.randint(0, d, (500,), device="cuda", generator=g)
     x[bad_rows, bad_cols] = float("nan")
+    x[::7, 0] = -float("inf")  # decided by a row's first four entries
+    x[3::11] = -float("inf")  # whole rows, as in an unfilled sample buffer
     x[-1, d - 1] = float("inf")
     out = torch.full((n,), 7, dtype=torch.int32, device="cuda")
     N.call("fmc_finite_rows", N.stream_ptr(), x.data_ptr(), n, d, out.data_ptr())
